@@ -1,0 +1,205 @@
+"""Benchmark of the MI355X PIC step (one JSON line on stdout).
+
+Workload (the metric's configuration): 1024 x 1024 periodic grid, 3rd-order
+B-spline weighting, two species (electrons, Xe+ at mass ratio 239,332) at
+128 particles per cell = 268,435,456 particles, uniform Bz with
+omega_ce/omega_pe = 0.5, electrostatic, dx = 0.5 Debye length,
+dt = 0.1/omega_pe, FP32 particles.  A step is one cycle of the reference's
+loop (main.mm:154-254): Poisson solve (GPU MG-CG to a 1e-8 relative
+residual, FP64 refinement) -> fused gather + Boris push + int64 deposit of
+both species -> cell/tile sort every 20 steps.  Particles are generated on
+the device (seeded PCG32 streams), so inputs are resident in HBM when the
+timed region starts.
+
+The roofline object is for the fused push+deposit kernel: algorithmic bytes
+40 B per particle (read and write of x, y, vx, vy, vz in FP32) over the
+kernel's mean duration from HIP events recorded on the context stream during
+the timed steps.  cpu_baseline times the C/OpenMP oracle's push+deposit on a
+bounded sample of the same workload on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+ME = 9.10938188e-28          # g (Constant.h)
+EC = 4.803204197e-10         # statC
+CL = 2.99792458e10           # cm/s
+KB = 1.3806503e-16
+EVTOK = 11604.0
+MXE = 2.1801714e-22          # Xe+ mass (data/condition_large.json)
+HBM_PEAK = 8.0e12            # MI355X HBM3E, B/s
+METRIC = "particle-steps/s + HBM%, 1024² 3rd-order; Poisson time to 1e-8: NN-PCG vs MG-CG"
+
+
+def plasma(n0=5e10, te_ev=10.0, ti_ev=1.0, ngx=1024, ppc=128):
+    te, ti = te_ev * EVTOK, ti_ev * EVTOK
+    wpe = np.sqrt(4 * np.pi * n0 * EC * EC / ME)
+    lde = np.sqrt(KB * te / (4 * np.pi * n0 * EC * EC))
+    dx = 0.5 * lde
+    dt = 0.1 / wpe
+    bz = 0.5 * wpe * ME * CL / EC
+    w = n0 * dx * dx / ppc                                  # Init.mm:192-200 InitialNumDens
+    vth_e = np.sqrt(2 * KB * te / ME)                       # Collision.mm:264 (sqrt(2kT/m))
+    vth_i = np.sqrt(2 * KB * ti / MXE)
+    return dict(dx=dx, dt=dt, bz=bz, w=w, vth_e=vth_e, vth_i=vth_i, wpe=wpe)
+
+
+def gpu_run(args):
+    import gample_amd as gp
+    P = plasma(ngx=args.ngx, ppc=args.ppc)
+    ngx = ngy = args.ngx
+    n = args.ppc * ngx * ngy
+    cfg = gp.make_config(ngx, ngy, P["dx"], P["dx"], order=args.order, boris=2, b0=(0.0, 0.0, P["bz"]))
+    ctx = gp.Context(cfg)
+    se = ctx.add_species(-EC, ME, P["w"], n, vth_hint=P["vth_e"] / np.sqrt(2))
+    si = ctx.add_species(EC, MXE, P["w"], n, vth_hint=P["vth_i"] / np.sqrt(2))
+    G = gp.Gen()
+    G.gen_type = 0
+    G.xmin, G.xmax, G.ymin, G.ymax = 0.0, ngx * float(np.float32(P["dx"])), 0.0, ngy * float(np.float32(P["dx"]))
+    G.ele_vth, G.ion_vth = P["vth_e"], P["vth_i"]
+    ctx.load_pair(se, si, G, 0, 0, n)
+    ctx.sort(se)
+    ctx.sort(si)
+    ctx.reset_grids()
+    ctx.deposit(se)
+    ctx.deposit(si)
+    dt, tol, sort_every = P["dt"], 1e-8, 20
+    cyc = 0
+    for _ in range(args.warmup):
+        cyc += 1
+        ctx.step(dt, tol=tol, maxiter=400, sort_interval=sort_every, cycle=cyc)
+    ctx.sync()
+    ctx.timing(True)
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cyc += 1
+        st, _ = ctx.step(dt, tol=tol, maxiter=400, sort_interval=sort_every, cycle=cyc)
+        iters.append(st.iterations)
+    ctx.sync()
+    t1 = time.perf_counter()
+    push_ms, push_n = ctx.kernel_time(0)
+    sort_ms, sort_n = ctx.kernel_time(1)
+    solve_ms, solve_n = ctx.kernel_time(2)
+    ctx.timing(False)
+    _, dg, _ = ctx.solve_fields(dt, tol=tol, diag=True)
+    nn = ctx.count(se) + ctx.count(si)
+    ctx.close()
+    return dict(n_total=nn, seconds=t1 - t0, push_ms=push_ms, push_n=push_n, sort_ms=sort_ms, sort_n=sort_n,
+                solve_ms=solve_ms, solve_n=solve_n, iters=iters, P=P, n_species=n)
+
+
+def cpu_baseline(args, P):
+    """C/OpenMP oracle push + deposit on a bounded sample of the workload."""
+    from oracle import pyoracle as orc
+    ngx = args.ngx
+    n = args.cpu_sample
+    threads = os.cpu_count() or 1
+    rng = np.random.default_rng(0)
+    g = orc.grid(ngx, ngx, args.order, boris=2)
+    ngb = args.order // 2
+    nx = ngx + 2 * ngb
+    Ex = (rng.standard_normal((nx + 1, nx + 2)) * 1e-3).astype(np.float32)
+    Ey = (rng.standard_normal((nx + 2, nx + 1)) * 1e-3).astype(np.float32)
+    Bz = np.full((nx + 2, nx + 2), P["bz"], np.float32)
+    sp = []
+    for vth, q, m in ((P["vth_e"], -EC, ME), (P["vth_i"], EC, MXE)):
+        x = (ngb + rng.random(n) * ngx).astype(np.float32)
+        y = (ngb + rng.random(n) * ngx).astype(np.float32)
+        v = (vth / np.sqrt(2) * rng.standard_normal((3, n))).astype(np.float32)
+        cE = float(np.float32(0.5 * float(np.float32(q)) * P["dt"] / float(np.float32(m))))
+        cB = float(np.float32(0.5 * float(np.float32(q)) * P["dt"] / float(np.float32(m)) / CL))
+        cX = float(np.float32(P["dt"] / float(np.float32(P["dx"]))))
+        sp.append(([x, y, v[0], v[1], v[2]], (cE, cB, cX, cX)))
+
+    def one_step(state):
+        out = []
+        for parts, cs in state:
+            p2 = orc.push_fast(g, Ex, Ey, Bz, cs, parts, uniform_bz=True, bz0=float(np.float32(P["bz"])),
+                               threads=threads)
+            orc.deposit_fx(g, p2, threads=threads)
+            out.append((p2, cs))
+        return out
+
+    sp = one_step(sp)                     # warm-up
+    t0 = time.perf_counter()
+    for _ in range(args.cpu_steps):
+        sp = one_step(sp)
+    t1 = time.perf_counter()
+    rate = 2 * n * args.cpu_steps / (t1 - t0)
+    return dict(value=rate, unit="particle-steps/s", cores=threads, kind="port",
+                sample=f"oracle push_fast + deposit_fx (C/OpenMP, -O2 x86-64-v3) of 2 x {n:,} particles on the "
+                       f"{ngx}x{ngx} 3rd-order grid, {args.cpu_steps} timed steps after 1 warm-up; Poisson not "
+                       f"included (CPU MG is not built)")
+
+
+def traffic_from_profiles():
+    f = os.path.join(ROOT, "profiles", "push_deposit_traffic.json")
+    if os.path.exists(f):
+        try:
+            return json.load(open(f))
+        except Exception:
+            return None
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--ngx", type=int, default=1024)
+    ap.add_argument("--ppc", type=int, default=128)
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU (particle-sharded, int64 "
+                                   "all-reduce of the charge grid) design is described in DESIGN.md",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    r = gpu_run(args)
+    ms_step = 1e3 * r["seconds"] / args.steps
+    value = r["n_total"] * args.steps / r["seconds"]
+    push_avg_ms = r["push_ms"] / max(1, r["push_n"])
+    bytes_per_launch = 40.0 * r["n_species"]
+    achieved = bytes_per_launch / (push_avg_ms * 1e-3)
+    tr = traffic_from_profiles()
+    roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": (tr or {}).get("bytes_per_launch"),
+            "kernel": "k_push<3,uniform B,rho deposit,push> (fused gather+Boris+int64 deposit)",
+            "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": push_avg_ms,
+            "launches": r["push_n"]}
+    line = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 (fp64 Poisson refinement and reductions)", "data": "synthetic",
+        "config": {"workload": f"PIC step {args.ngx}x{args.ngx} periodic, order {args.order} B-spline, "
+                               f"{args.ppc} ppc x 2 species ({r['n_total']:,} particles), uniform Bz "
+                               f"(wce/wpe=0.5), electrostatic, Poisson MG-CG to 1e-8 (FP64 refined), "
+                               f"tile sort every 20 steps",
+                   "global_batch": r["n_total"], "parallelism": "single GPU"},
+        "roofline": roof,
+        "breakdown_ms_per_step": {"push_deposit": r["push_ms"] / args.steps, "sort": r["sort_ms"] / args.steps,
+                                  "field_solve": r["solve_ms"] / args.steps},
+        "poisson": {"mg_cg_iterations_mean": float(np.mean(r["iters"])), "ms_per_solve":
+                    r["solve_ms"] / max(1, r["solve_n"]), "nn_pcg": None},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(args, r["P"])
+        line["cpu_baseline"] = cb
+        line["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,510 @@
+// gp_api.hip -- the extern "C" boundary of libgample_amd (include/gample_amd.h).
+//
+// Every entry point validates its arguments on the host, converts C++
+// failures into status codes and records the message for gp_last_error().
+#include "gp_internal.h"
+#include <cmath>
+#include <cstring>
+#include <new>
+
+static thread_local std::string g_err;
+
+int gp_fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int gp_time_mark(gp_ctx *c, int which)
+{
+    hipEvent_t e;
+    GP_HIP(hipEventCreate(&e));
+    GP_HIP(hipEventRecord(e, c->stream));
+    c->tev[which].push_back(e);
+    return GP_OK;
+}
+
+int gp_time_collect(gp_ctx *c)
+{
+    GP_HIP(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 4; k++) {
+        auto &v = c->tev[k];
+        for (size_t i = 0; i + 1 < v.size(); i += 2) {
+            float ms = 0.f;
+            GP_HIP(hipEventElapsedTime(&ms, v[i], v[i + 1]));
+            c->t_ms[k] += ms;
+            c->t_n[k] += 1;
+        }
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
+        v.clear();
+    }
+    return GP_OK;
+}
+
+int gp_launch_ke(gp_ctx *c, int sid);
+int gp_finalize_grids(gp_ctx *c);
+
+#define GP_TRY(...)                                                      \
+    try {                                                                \
+        __VA_ARGS__                                                      \
+    } catch (const std::bad_alloc &) {                                   \
+        return gp_fail(GP_ERR_NOMEM, "host allocation failed");          \
+    } catch (const std::exception &e) {                                  \
+        return gp_fail(GP_ERR_STATE, e.what());                          \
+    }
+
+#define GP_CHECK_CTX(c) \
+    do { if (!(c)) return gp_fail(GP_ERR_ARG, "null context"); } while (0)
+#define GP_CHECK_SID(c, sid) \
+    do { if ((sid) < 0 || (sid) >= (int)(c)->sp.size()) return gp_fail(GP_ERR_ARG, "bad species id"); } while (0)
+
+extern "C" {
+
+const char *gp_last_error(void) { return g_err.c_str(); }
+
+int gp_ctx_create(const gp_config *cfg, gp_ctx **out)
+{
+    if (!cfg || !out) return gp_fail(GP_ERR_ARG, "null argument");
+    *out = nullptr;
+    const gp_config &k = *cfg;
+    if (k.weight_order != 1 && k.weight_order != 3 && k.weight_order != 5)
+        return gp_fail(GP_ERR_ARG, "WeightingOrder must be 1, 3 or 5");
+    if (k.boris_order != 2 && k.boris_order != 4 && k.boris_order != 6)
+        return gp_fail(GP_ERR_ARG, "BorisOrder must be 2, 4 or 6");
+    if (k.ngx < 2 || k.ngy < 2 || !(k.dx > 0) || !(k.dy > 0)) return gp_fail(GP_ERR_ARG, "bad grid");
+    for (int w = 0; w < 4; w++) {
+        if (k.pbc[w] < -1 || k.pbc[w] > 2) return gp_fail(GP_ERR_ARG, "bad particle boundary code");
+        if (k.fbc[w] < 0 || k.fbc[w] > 2) return gp_fail(GP_ERR_ARG, "bad field boundary code");
+    }
+    if (k.em_model == 2 && !k.cur_dens) return gp_fail(GP_ERR_ARG, "the Darwin model needs cur_dens = 1");
+    GP_TRY({
+        gp_ctx *c = new gp_ctx();
+        c->cfg = k;
+        c->order = k.weight_order;
+        c->ngb = k.weight_order / 2;   // Init.mm:140
+        c->ngx = k.ngx; c->ngy = k.ngy;
+        c->nx = k.ngx + 2 * c->ngb; c->ny = k.ngy + 2 * c->ngb;
+        c->dxf = (float)k.dx; c->dyf = (float)k.dy;
+        c->ntx = (k.ngx + GP_TILE - 1) / GP_TILE;
+        c->nty = (k.ngy + GP_TILE - 1) / GP_TILE;
+        c->ntiles = c->ntx * c->nty;
+        c->tile_bits = 1;
+        while ((1 << c->tile_bits) < c->ntiles) c->tile_bits++;
+        hipError_t e = hipSetDevice(k.device);
+        if (e != hipSuccess) { delete c; return gp_fail(GP_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)); }
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { delete c; return gp_fail(GP_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+        int rc = gp_fields_init(c);
+        if (rc) { gp_ctx_destroy(c); return rc; }
+        *out = c;
+        return GP_OK;
+    })
+}
+
+int gp_ctx_destroy(gp_ctx *c)
+{
+    if (!c) return GP_OK;
+    (void)hipDeviceSynchronize();
+    for (auto &S : c->sp) {
+        void *ps[] = {S.a.x, S.a.y, S.a.vx, S.a.vy, S.a.vz, S.b.x, S.b.y, S.b.vx, S.b.vy, S.b.vz, S.tile_off,
+                      S.rho_acc, S.jx_acc, S.jy_acc, S.dead, S.dead_tmp, S.counters, S.ke_part, S.ke_out,
+                      S.key, S.key2, S.val, S.val2};
+        for (void *p : ps) if (p) (void)hipFree(p);
+    }
+    for (auto &v : c->tev)
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
+    gp_nn_destroy(c);
+    gp_fields_free(c);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GP_OK;
+}
+
+int gp_sync(gp_ctx *c)
+{
+    GP_CHECK_CTX(c);
+    GP_HIP(hipStreamSynchronize(c->stream));
+    return GP_OK;
+}
+
+int gp_dims(const gp_ctx *c, int out[6])
+{
+    if (!c || !out) return gp_fail(GP_ERR_ARG, "null argument");
+    out[0] = c->nx; out[1] = c->ny; out[2] = c->ps.nkx; out[3] = c->ps.nky; out[4] = c->ps.ikmin; out[5] = c->ps.jkmin;
+    return GP_OK;
+}
+
+int gp_species_create(gp_ctx *c, const gp_species_desc *d, int *sid)
+{
+    GP_CHECK_CTX(c);
+    if (!d || !sid) return gp_fail(GP_ERR_ARG, "null argument");
+    if ((int)c->sp.size() >= GP_MAX_SPECIES) return gp_fail(GP_ERR_CAPACITY, "too many species");
+    if (d->capacity < 1 || d->capacity > 0x7fffff00LL) return gp_fail(GP_ERR_ARG, "capacity must be in [1, 2^31)");
+    if (!(d->m > 0)) return gp_fail(GP_ERR_ARG, "mass must be positive");
+    GP_TRY({
+        c->sp.emplace_back();
+        gp_species &S = c->sp.back();
+        S.q = d->q; S.m = d->m; S.w = d->w;
+        S.qf = (float)d->q; S.mf = (float)d->m; S.wf = (float)d->w;
+        // Particle.mm:540: q*w in float, divided by dx*dy in double
+        S.scale = (float)((double)(S.qf * S.wf) / (c->cfg.dx * c->cfg.dy));
+        double vn = d->vnorm > 0 ? d->vnorm : (d->vth_hint > 0 ? 8.0 * d->vth_hint : 2.99792458e10);
+        S.vnorm = std::ldexp(1.0, (int)std::ceil(std::log2(vn)));
+        S.inv_vnorm = 1.0 / S.vnorm;
+        S.cap = d->capacity;
+        const size_t cap = (size_t)d->capacity;
+        auto al = [&](void **p, size_t bytes) -> int {
+            hipError_t e = hipMalloc(p, bytes);
+            if (e != hipSuccess) return gp_fail(GP_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+            return GP_OK;
+        };
+        int rc = 0;
+        float **fa[] = {&S.a.x, &S.a.y, &S.a.vx, &S.a.vy, &S.a.vz, &S.b.x, &S.b.y, &S.b.vx, &S.b.vy, &S.b.vz};
+        for (float **p : fa) if ((rc = al((void **)p, cap * sizeof(float)))) break;
+        const size_t nN = (size_t)(c->nx + 1) * (c->ny + 1), nEx = (size_t)(c->nx + 2) * (c->ny + 1),
+                     nEy = (size_t)(c->nx + 1) * (c->ny + 2);
+        if (!rc) rc = al((void **)&S.tile_off, sizeof(int32_t) * (c->ntiles + 1));
+        if (!rc) rc = al((void **)&S.rho_acc, nN * 8);
+        if (!rc && c->cfg.cur_dens) rc = al((void **)&S.jx_acc, nEx * 8);
+        if (!rc && c->cfg.cur_dens) rc = al((void **)&S.jy_acc, nEy * 8);
+        if (!rc) rc = al((void **)&S.dead, cap * 4);
+        if (!rc) rc = al((void **)&S.dead_tmp, cap * 4);
+        if (!rc) rc = al((void **)&S.counters, 8 * 8);
+        S.ke_nblk = c->ntiles + gp_cdiv((int64_t)cap, 16 * GP_BLOCK) + 1024 + 1;
+        if (!rc) rc = al((void **)&S.ke_part, (size_t)S.ke_nblk * 8);
+        if (!rc) rc = al((void **)&S.ke_out, 8);
+        if (!rc) rc = al((void **)&S.key, (cap + 1) * 4);
+        if (!rc) rc = al((void **)&S.key2, (cap + 2) * 4);
+        if (!rc) rc = al((void **)&S.val, (cap + 1) * 4);
+        if (!rc) rc = al((void **)&S.val2, (cap + 1) * 4);
+        if (rc) return rc;
+        GP_HIP(hipMemsetAsync(S.tile_off, 0, sizeof(int32_t) * (c->ntiles + 1), c->stream));
+        GP_HIP(hipMemsetAsync(S.rho_acc, 0, nN * 8, c->stream));
+        if (c->cfg.cur_dens) {
+            GP_HIP(hipMemsetAsync(S.jx_acc, 0, nEx * 8, c->stream));
+            GP_HIP(hipMemsetAsync(S.jy_acc, 0, nEy * 8, c->stream));
+        }
+        GP_HIP(hipMemsetAsync(S.counters, 0, 64, c->stream));
+        GP_HIP(hipMemsetAsync(S.ke_out, 0, 8, c->stream));
+        GP_HIP(hipStreamSynchronize(c->stream));
+        *sid = (int)c->sp.size() - 1;
+        return GP_OK;
+    })
+}
+
+int gp_species_upload(gp_ctx *c, int sid, int64_t n, const float *x, const float *y, const float *vx,
+                      const float *vy, const float *vz)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    gp_species &S = c->sp[sid];
+    if (n < 0 || n > S.cap) return gp_fail(GP_ERR_CAPACITY, "pNum > pNumMax on upload");
+    if (n > 0 && (!x || !y || !vx || !vy || !vz)) return gp_fail(GP_ERR_ARG, "null particle array");
+    const size_t b = (size_t)n * 4;
+    if (n > 0) {
+        GP_HIP(hipMemcpyAsync(S.a.x, x, b, hipMemcpyHostToDevice, c->stream));
+        GP_HIP(hipMemcpyAsync(S.a.y, y, b, hipMemcpyHostToDevice, c->stream));
+        GP_HIP(hipMemcpyAsync(S.a.vx, vx, b, hipMemcpyHostToDevice, c->stream));
+        GP_HIP(hipMemcpyAsync(S.a.vy, vy, b, hipMemcpyHostToDevice, c->stream));
+        GP_HIP(hipMemcpyAsync(S.a.vz, vz, b, hipMemcpyHostToDevice, c->stream));
+    }
+    GP_HIP(hipStreamSynchronize(c->stream));
+    S.n = n;
+    S.n_binned = 0;
+    return GP_OK;
+}
+
+int gp_species_download(gp_ctx *c, int sid, int64_t cap, int64_t *n, float *x, float *y, float *vx, float *vy,
+                        float *vz)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    gp_species &S = c->sp[sid];
+    if (n) *n = S.n;
+    if (cap < S.n) return gp_fail(GP_ERR_CAPACITY, "host buffer smaller than the particle count");
+    const size_t b = (size_t)S.n * 4;
+    if (S.n > 0) {
+        if (x) GP_HIP(hipMemcpyAsync(x, S.a.x, b, hipMemcpyDeviceToHost, c->stream));
+        if (y) GP_HIP(hipMemcpyAsync(y, S.a.y, b, hipMemcpyDeviceToHost, c->stream));
+        if (vx) GP_HIP(hipMemcpyAsync(vx, S.a.vx, b, hipMemcpyDeviceToHost, c->stream));
+        if (vy) GP_HIP(hipMemcpyAsync(vy, S.a.vy, b, hipMemcpyDeviceToHost, c->stream));
+        if (vz) GP_HIP(hipMemcpyAsync(vz, S.a.vz, b, hipMemcpyDeviceToHost, c->stream));
+    }
+    GP_HIP(hipStreamSynchronize(c->stream));
+    return GP_OK;
+}
+
+int gp_species_count(gp_ctx *c, int sid, int64_t *n)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    if (!n) return gp_fail(GP_ERR_ARG, "null argument");
+    *n = c->sp[sid].n;
+    return GP_OK;
+}
+
+int gp_species_grid_get(gp_ctx *c, int sid, int which, int64_t *host, int64_t count)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    gp_species &S = c->sp[sid];
+    const int64_t nN = (int64_t)(c->nx + 1) * (c->ny + 1), nEx = (int64_t)(c->nx + 2) * (c->ny + 1),
+                  nEy = (int64_t)(c->nx + 1) * (c->ny + 2);
+    const int64_t *src = nullptr;
+    int64_t need = 0;
+    if (which == GP_G_RHO_ACC) { src = S.rho_acc; need = nN; }
+    else if (which == GP_G_JX_ACC) { src = S.jx_acc; need = nEx; }
+    else if (which == GP_G_JY_ACC) { src = S.jy_acc; need = nEy; }
+    else return gp_fail(GP_ERR_ARG, "bad accumulator id");
+    if (!src) return gp_fail(GP_ERR_STATE, "current accumulators need cur_dens = 1");
+    if (count != need || !host) return gp_fail(GP_ERR_ARG, "accumulator size mismatch");
+    GP_HIP(hipMemcpyAsync(host, src, need * 8, hipMemcpyDeviceToHost, c->stream));
+    GP_HIP(hipStreamSynchronize(c->stream));
+    return GP_OK;
+}
+
+int gp_load_pair(gp_ctx *c, int se, int si, const gp_gen *g, uint64_t seed, int64_t first_id, int64_t n)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, se);
+    if (si >= 0) GP_CHECK_SID(c, si);
+    if (!g || n < 0 || first_id < 0) return gp_fail(GP_ERR_ARG, "bad generation arguments");
+    if (g->gen_type < 0 || g->gen_type > 7) return gp_fail(GP_ERR_ARG, "unknown generation type");
+    if (!(g->xmax > g->xmin) || !(g->ymax > g->ymin)) return gp_fail(GP_ERR_ARG, "empty generation region");
+    int rc = gp_launch_load(c, se, si, g, seed, first_id, n);
+    if (rc) return rc;
+    c->sp[se].n_binned = std::min(c->sp[se].n_binned, c->sp[se].n);
+    GP_HIP(hipStreamSynchronize(c->stream));
+    return GP_OK;
+}
+
+int gp_inject_wall(gp_ctx *c, int sid, int64_t n, float vth, const float u[3], double dt, uint64_t seed,
+                   uint64_t stream, int deposit)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    if (n < 0 || !u || !(vth >= 0) || !(dt > 0)) return gp_fail(GP_ERR_ARG, "bad injection arguments");
+    return gp_launch_inject_wall(c, sid, n, vth, u, dt, seed, stream, deposit);
+}
+
+int gp_reset_grids(gp_ctx *c)
+{
+    GP_CHECK_CTX(c);
+    const size_t nN = (size_t)(c->nx + 1) * (c->ny + 1), nEx = (size_t)(c->nx + 2) * (c->ny + 1),
+                 nEy = (size_t)(c->nx + 1) * (c->ny + 2);
+    for (auto &S : c->sp) {
+        GP_HIP(hipMemsetAsync(S.rho_acc, 0, nN * 8, c->stream));
+        if (S.jx_acc) GP_HIP(hipMemsetAsync(S.jx_acc, 0, nEx * 8, c->stream));
+        if (S.jy_acc) GP_HIP(hipMemsetAsync(S.jy_acc, 0, nEy * 8, c->stream));
+    }
+    return GP_OK;
+}
+
+int gp_deposit(gp_ctx *c, int sid)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    return gp_launch_deposit(c, sid, 0, c->sp[sid].n);
+}
+
+int gp_solve_fields(gp_ctx *c, double dt, double tol, int maxiter, int solver, gp_solve_stats *st, gp_diag *diag)
+{
+    GP_CHECK_CTX(c);
+    if (!(tol > 0) || maxiter < 1) return gp_fail(GP_ERR_ARG, "tol must be > 0 and maxiter >= 1");
+    GP_TRY({
+        if (c->timing) { int rc = gp_time_mark(c, 2); if (rc) return rc; }
+        int rc = gp_solve_fields_impl(c, dt, tol, maxiter, solver, st, diag);
+        if (c->timing && rc >= 0) { int r2 = gp_time_mark(c, 2); if (r2) return r2; }
+        return rc;
+    })
+}
+
+int gp_solve_darwin(gp_ctx *c, double dt, double tol, int maxiter, int solver, gp_solve_stats st[3])
+{
+    GP_CHECK_CTX(c);
+    if (!st) return gp_fail(GP_ERR_ARG, "null stats");
+    GP_TRY({ return gp_solve_darwin_impl(c, dt, tol, maxiter, solver, st); })
+}
+
+int gp_push(gp_ctx *c, int sid, double dt, int deposit)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    if (!(dt > 0)) return gp_fail(GP_ERR_ARG, "dt must be positive");
+    return gp_launch_push(c, sid, dt, deposit);
+}
+
+int gp_reduce(gp_ctx *c, int sid, int64_t counts[4], double *kinetic)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    int64_t tmp[4];
+    return gp_launch_reduce(c, sid, counts ? counts : tmp, kinetic);
+}
+
+int gp_sort(gp_ctx *c, int sid)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    int rc = gp_launch_sort(c, sid);
+    if (rc) return rc;
+    return GP_OK;
+}
+
+int gp_moments(gp_ctx *c, int sid, double *host10)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    if (!host10) return gp_fail(GP_ERR_ARG, "null output");
+    return gp_launch_moments(c, sid, host10);
+}
+
+int gp_step(gp_ctx *c, double dt, double tol, int maxiter, int solver, int sort_interval, int cycle,
+            gp_solve_stats *st, gp_diag *diag)
+{
+    GP_CHECK_CTX(c);
+    if (!(dt > 0)) return gp_fail(GP_ERR_ARG, "dt must be positive");
+    int status = gp_solve_fields(c, dt, tol, maxiter, solver, st, diag);
+    if (status < 0) return status;
+    if (c->cfg.em_model == 2) {
+        gp_solve_stats ds[3];
+        int rc = gp_solve_darwin(c, dt, tol, maxiter, 0, ds);
+        if (rc < 0) return rc;
+    }
+    bool has_delete = false;
+    for (int w = 0; w < 4; w++) has_delete |= c->cfg.pbc[w] == GP_PBC_DELETE;
+    for (int s = 0; s < (int)c->sp.size(); s++) {
+        int rc = gp_launch_push(c, s, dt, 1);
+        if (rc) return rc;
+    }
+    for (int s = 0; s < (int)c->sp.size(); s++) {
+        int64_t cnt[4] = {0, 0, 0, 0};
+        double ke = 0.0;
+        if (has_delete || diag) {
+            int rc = gp_launch_reduce(c, s, cnt, diag ? &ke : nullptr);
+            if (rc) return rc;
+        }
+        if (diag && s < 8) {
+            diag->kinetic[s] = ke;
+            diag->count[s] = c->sp[s].n;
+            for (int w = 0; w < 4; w++) diag->absorbed[s][w] = cnt[w];
+        }
+    }
+    if (sort_interval > 0 && cycle % sort_interval == 0)
+        for (int s = 0; s < (int)c->sp.size(); s++) {
+            int rc = gp_launch_sort(c, s);
+            if (rc) return rc;
+        }
+    return status;
+}
+
+int gp_field_get(gp_ctx *c, int which, void *host, int64_t bytes)
+{
+    GP_CHECK_CTX(c);
+    if (!host) return gp_fail(GP_ERR_ARG, "null output");
+    const int64_t nx = c->nx, ny = c->ny;
+    const void *src = nullptr;
+    int64_t need = 0;
+    switch (which) {
+    case GP_F_RHO: src = c->rho; need = (nx + 1) * (ny + 1) * 8; break;
+    case GP_F_EX: src = c->Ex; need = (nx + 2) * (ny + 1) * 4; break;
+    case GP_F_EY: src = c->Ey; need = (nx + 1) * (ny + 2) * 4; break;
+    case GP_F_EZ: src = c->Ez; need = (nx + 1) * (ny + 1) * 4; break;
+    case GP_F_BX: src = c->Bx; need = (nx + 1) * (ny + 2) * 4; break;
+    case GP_F_BY: src = c->By; need = (nx + 2) * (ny + 1) * 4; break;
+    case GP_F_BZ: src = c->Bz; need = (nx + 2) * (ny + 2) * 4; break;
+    case GP_F_PHI: src = c->phi; need = (nx + 3) * (ny + 3) * 8; break;
+    case GP_F_JX: src = c->jx; need = (nx + 2) * (ny + 1) * 8; break;
+    case GP_F_JY: src = c->jy; need = (nx + 1) * (ny + 2) * 8; break;
+    case GP_F_DEL_EX: src = c->dEx; need = (nx + 2) * (ny + 1) * 4; break;
+    case GP_F_DEL_EY: src = c->dEy; need = (nx + 1) * (ny + 2) * 4; break;
+    case GP_F_DEL_BZ: src = c->dBz; need = (nx + 2) * (ny + 2) * 4; break;
+    case GP_F_SOL: src = c->ps.x64; need = (int64_t)c->ps.n * 8; break;
+    default: return gp_fail(GP_ERR_ARG, "unknown field id");
+    }
+    if (bytes != need) return gp_fail(GP_ERR_ARG, "field size mismatch: expected " + std::to_string(need) + " bytes");
+    GP_HIP(hipMemcpyAsync(host, src, need, hipMemcpyDeviceToHost, c->stream));
+    GP_HIP(hipStreamSynchronize(c->stream));
+    return GP_OK;
+}
+
+int gp_field_set(gp_ctx *c, int which, const void *host, int64_t bytes)
+{
+    GP_CHECK_CTX(c);
+    if (!host) return gp_fail(GP_ERR_ARG, "null input");
+    const int64_t nx = c->nx, ny = c->ny;
+    void *dst = nullptr;
+    int64_t need = 0;
+    switch (which) {
+    case GP_F_EX: dst = c->Ex; need = (nx + 2) * (ny + 1) * 4; break;
+    case GP_F_EY: dst = c->Ey; need = (nx + 1) * (ny + 2) * 4; break;
+    case GP_F_EZ: dst = c->Ez; need = (nx + 1) * (ny + 1) * 4; break;
+    case GP_F_BX: dst = c->Bx; need = (nx + 1) * (ny + 2) * 4; break;
+    case GP_F_BY: dst = c->By; need = (nx + 2) * (ny + 1) * 4; break;
+    case GP_F_BZ: dst = c->Bz; need = (nx + 2) * (ny + 2) * 4; c->uniform_b = 0; break;
+    case GP_F_SOL: dst = c->ps.x64; need = (int64_t)c->ps.n * 8; break;
+    default: return gp_fail(GP_ERR_ARG, "field is not settable");
+    }
+    if (bytes != need) return gp_fail(GP_ERR_ARG, "field size mismatch: expected " + std::to_string(need) + " bytes");
+    GP_HIP(hipMemcpyAsync(dst, host, need, hipMemcpyHostToDevice, c->stream));
+    GP_HIP(hipStreamSynchronize(c->stream));
+    return GP_OK;
+}
+
+int gp_field_energy(gp_ctx *c, double dt, double *we_ref, double *we_cgs, double *joule)
+{
+    GP_CHECK_CTX(c);
+    return gp_energy_impl(c, dt, we_ref, we_cgs, joule);
+}
+
+int gp_timing(gp_ctx *c, int enable)
+{
+    GP_CHECK_CTX(c);
+    int rc = gp_time_collect(c);
+    if (rc) return rc;
+    c->timing = enable ? 1 : 0;
+    for (int k = 0; k < 4; k++) { c->t_ms[k] = 0; c->t_n[k] = 0; }
+    return GP_OK;
+}
+
+int gp_kernel_time(gp_ctx *c, int which, double *total_ms, int64_t *launches)
+{
+    GP_CHECK_CTX(c);
+    if (which < 0 || which > 3 || !total_ms || !launches) return gp_fail(GP_ERR_ARG, "bad timing query");
+    int rc = gp_time_collect(c);
+    if (rc) return rc;
+    *total_ms = c->t_ms[which];
+    *launches = c->t_n[which];
+    return GP_OK;
+}
+
+int gp_nn_init(gp_ctx *c, int base_channels, uint64_t seed)
+{
+    GP_CHECK_CTX(c);
+    GP_TRY({ return gp_nn_create(c, base_channels, seed); })
+}
+int gp_nn_train(gp_ctx *c, const double *rho, const double *phi, int nsamp, int epochs, int batch, float lr,
+                float *loss_out)
+{
+    GP_CHECK_CTX(c);
+    GP_TRY({ return gp_nn_train_impl(c, rho, phi, nsamp, epochs, batch, lr, loss_out); })
+}
+int gp_nn_infer(gp_ctx *c, const double *rhs, double *out)
+{
+    GP_CHECK_CTX(c);
+    GP_TRY({ return gp_nn_infer_impl(c, rhs, out); })
+}
+int gp_nn_save(gp_ctx *c, const char *path)
+{
+    GP_CHECK_CTX(c);
+    GP_TRY({ return gp_nn_save_impl(c, path); })
+}
+int gp_nn_load(gp_ctx *c, const char *path)
+{
+    GP_CHECK_CTX(c);
+    GP_TRY({ return gp_nn_load_impl(c, path); })
+}
+
+}  // extern "C"

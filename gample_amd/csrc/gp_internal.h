@@ -1,0 +1,150 @@
+// gp_internal.h -- device-side layout and host-side context of libgample_amd.
+//
+// Particles: struct-of-arrays (x, y, vx, vy, vz float), positions in grid
+// units shifted by the ghost width ngb as in the reference
+// (Particle.mm:589-594).  Each species owns two SoA buffers (live + scratch)
+// for the sort and the ping-pong of the gather step.  An absorbed particle
+// is marked by x = -piflag (1..4) until gp_reduce removes it; positions in
+// the domain are always >= 0, so no separate flag array is stored.
+//
+// Storage order is tile-binned: after gp_sort the particles of tile t
+// (TILE x TILE cells) occupy [tile_off[t], tile_off[t+1]); a workgroup
+// processes one tile with the tile's field stencil and charge accumulator
+// staged in LDS.  Particles past tile_off[ntiles] (appended after the last
+// sort) are processed by extra workgroups through global memory.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/gample_amd.h"
+
+#define GP_TILE 8            // tile edge, cells
+#define GP_DRIFT 8           // cells a particle may leave its tile and still use LDS
+#define GP_BLOCK 256         // threads per workgroup of the particle kernels
+#define GP_MAX_SPECIES 8
+#define GP_RHO_FRAC 46       // fractional bits of the rho accumulators
+#define GP_CUR_FRAC 40       // fractional bits of current/moment accumulators
+
+// status plumbing (gp_api.hip)
+int gp_fail(int code, const std::string &msg);
+#define GP_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return gp_fail(GP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct gp_soa {
+    float *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
+};
+
+struct gp_species {
+    double q = 0, m = 0, w = 0;
+    float qf = 0, mf = 0, wf = 0;     // the reference keeps these as float members
+    float scale = 0;                  // q*w/(dx*dy) as Particle.mm:540 computes it
+    double vnorm = 1, inv_vnorm = 1;  // power of two
+    int64_t cap = 0, n = 0;
+    gp_soa a, b;                      // live, scratch
+    int64_t n_binned = 0;             // particles covered by tile_off (<= n)
+    int32_t *tile_off = nullptr;      // ntiles+1
+    // accumulators (int64 fixed point), reference layouts
+    int64_t *rho_acc = nullptr, *jx_acc = nullptr, *jy_acc = nullptr;
+    // absorption bookkeeping
+    uint32_t *dead = nullptr;         // indices of absorbed particles (cap)
+    uint32_t *dead_tmp = nullptr;
+    unsigned long long *counters = nullptr;  // [0] n_dead, [1..4] per wall
+    double *ke_part = nullptr;        // per-workgroup kinetic-energy partials
+    int ke_nblk = 0;
+    double *ke_out = nullptr;         // device scalar
+    // sort scratch
+    uint32_t *key = nullptr, *key2 = nullptr, *val = nullptr, *val2 = nullptr;
+};
+
+struct gp_mg_level {
+    int mx, my;          // unknowns per axis
+    int perx, pery;      // periodic flags
+    double hx2i, hy2i;   // 1/hx^2, 1/hy^2 at this level
+    float *u = nullptr, *f = nullptr, *r = nullptr;
+};
+
+struct gp_solver {
+    // unknown grid of the fine problem
+    int nkx = 0, nky = 0, ikmin = 0, jkmin = 0, n = 0;
+    int singular = 0;
+    std::vector<gp_mg_level> lv;
+    // FP64 fine vectors
+    double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr, *rhs_bc = nullptr;
+    // FP32 PCG vectors
+    float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *e = nullptr;
+    double *red = nullptr;     // reduction partials
+    double *scal = nullptr;    // device scalars
+    double *hscal = nullptr;   // pinned host mirror
+    int red_blocks = 0;
+};
+
+struct gp_nn;   // gp_nn.hip
+
+struct gp_ctx {
+    gp_config cfg;
+    int ngx, ngy, ngb, order, nx, ny;
+    float dxf, dyf;                 // EMField keeps dx, dy as float (EMField.mm:66-67)
+    int ntx, nty, ntiles, tile_bits;
+    hipStream_t stream = nullptr;
+    std::vector<gp_species> sp;
+    // fields, reference layouts (EMField.mm:133-141)
+    float *Ex = nullptr, *Ey = nullptr, *Ez = nullptr, *Bx = nullptr, *By = nullptr, *Bz = nullptr;
+    float *dEx = nullptr, *dEy = nullptr, *dBz = nullptr;
+    double *rho = nullptr, *jx = nullptr, *jy = nullptr, *phi = nullptr;
+    int uniform_b = 1;
+    float bz0 = 0;
+    gp_solver ps;                   // Poisson
+    gp_solver ds;                   // Darwin (Neumann/periodic, singular)
+    double *dsol_chi = nullptr, *dsol_ax = nullptr, *dsol_ay = nullptr, *last_ax = nullptr, *last_ay = nullptr;
+    double *red = nullptr;          // generic reduction scratch
+    double *hred = nullptr;         // pinned
+    int have_solution = 0;
+    // timing: event pairs recorded on the context stream around each launch
+    // of a category (0 push, 1 sort, 2 field solve), read back on demand
+    int timing = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> tev[4];
+    double t_ms[4] = {0, 0, 0, 0};
+    int64_t t_n[4] = {0, 0, 0, 0};
+    gp_nn *nn = nullptr;
+};
+
+// ---- particle side (gp_particles.hip) ----
+int gp_launch_push(gp_ctx *c, int sid, double dt, int deposit);
+int gp_launch_deposit(gp_ctx *c, int sid, int64_t first, int64_t count);
+int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke);
+int gp_launch_sort(gp_ctx *c, int sid);
+int gp_launch_load(gp_ctx *c, int se, int si, const gp_gen *g, uint64_t seed, int64_t first_id, int64_t n);
+int gp_launch_inject_wall(gp_ctx *c, int sid, int64_t n, float vth, const float u[3], double dt,
+                          uint64_t seed, uint64_t stream, int deposit);
+int gp_launch_moments(gp_ctx *c, int sid, double *host10);
+int gp_radix_sort_u32(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp,
+                      int64_t n, int bits, int init_vals, uint32_t **keys_out, uint32_t **vals_out);
+
+// ---- field side (gp_fields.hip) ----
+int gp_fields_init(gp_ctx *c);
+void gp_fields_free(gp_ctx *c);
+int gp_solve_fields_impl(gp_ctx *c, double dt, double tol, int maxiter, int solver,
+                         gp_solve_stats *st, gp_diag *diag);
+int gp_solve_darwin_impl(gp_ctx *c, double dt, double tol, int maxiter, int solver, gp_solve_stats st[3]);
+int gp_energy_impl(gp_ctx *c, double dt, double *we_ref, double *we_cgs, double *joule);
+
+// ---- learned preconditioner (gp_nn.hip) ----
+int gp_nn_create(gp_ctx *c, int base_ch, uint64_t seed);
+void gp_nn_destroy(gp_ctx *c);
+int gp_nn_train_impl(gp_ctx *c, const double *rho, const double *phi, int nsamp, int epochs, int batch,
+                     float lr, float *loss_out);
+int gp_nn_apply(gp_ctx *c, gp_solver *s, const float *in_dev, float *out_dev);
+int gp_nn_infer_impl(gp_ctx *c, const double *rhs, double *out);
+int gp_nn_save_impl(gp_ctx *c, const char *path);
+int gp_nn_load_impl(gp_ctx *c, const char *path);
+
+int gp_time_mark(gp_ctx *c, int which);     // record one event of a begin/end pair
+int gp_time_collect(gp_ctx *c);             // synchronise and fold pairs into t_ms
+
+static inline int gp_cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }

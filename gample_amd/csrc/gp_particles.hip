@@ -1,0 +1,1073 @@
+// gp_particles.hip -- particle kernels for gfx950.
+//
+//  k_push<ORDER, UNIFORM_B, DEP, PUSH>
+//      One workgroup per tile of GP_TILE x GP_TILE cells.  The workgroup
+//      stages the field stencil window of its tile (Ex, Ey, and Bz unless B
+//      is uniform) in LDS, streams its particles through registers with
+//      coalesced SoA loads, gathers (B-spline order 1/3/5), applies the
+//      Boris rotation, moves, applies the particle BCs, and deposits the new
+//      position's charge (and current) into an int64 fixed-point window in
+//      LDS, which it flushes to the global int64 accumulators with atomics.
+//      Integer addition is associative, so the deposit is bitwise the same
+//      for any particle order and any workgroup schedule.  Particles whose
+//      cell is outside the tile window (drifted since the last sort, or
+//      appended after it) take the same arithmetic through global memory.
+//      Reference: updateParticles + integrateChargeDensity,
+//      Particle.mm:65-278 and :281-389.
+//
+//  Compaction, radix sort, loading, injection and moments follow.
+#include "gp_internal.h"
+#include <algorithm>
+#include <cmath>
+
+namespace {
+
+constexpr int WIN = GP_TILE + 2 * GP_DRIFT;   // cell window of a tile
+
+// ---------------------------------------------------------------------------
+// B-spline weights.  Horner forms of the reference polynomials
+// (Particle.mm:97-131) in h, mirrored in t = 1-h:
+//   order 3: w0 = t^3/6, w1 = 2/3 + h^2 (h/2 - 1), w2 = same in t, w3 = h^3/6
+//   order 5: 120 w1 = 26 - 50h + 20h^2 + 20h^3 - 20h^4 + 5h^5 (w4 in t)
+//            120 w2 = 66 - 60h^2 + 30h^4 - 10h^5 (w3 in t)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float q5a(float s)
+{
+    return __builtin_fmaf(s, __builtin_fmaf(s, __builtin_fmaf(s, __builtin_fmaf(s,
+           __builtin_fmaf(5.0f, s, -20.0f), 20.0f), 20.0f), -50.0f), 26.0f);
+}
+__device__ __forceinline__ float q5b(float s)
+{
+    const float s2 = s * s;
+    return __builtin_fmaf(s2, __builtin_fmaf(s2, __builtin_fmaf(-10.0f, s, 30.0f), -60.0f), 66.0f);
+}
+template <int ORDER>
+__device__ __forceinline__ void wts(float h, float *w)
+{
+    if constexpr (ORDER == 1) {
+        w[0] = 1.0f - h;
+        w[1] = h;
+    } else if constexpr (ORDER == 3) {
+        const float c6 = 1.0f / 6.0f, c23 = 2.0f / 3.0f;
+        const float t = 1.0f - h, h2 = h * h, t2 = t * t;
+        w[0] = (t2 * t) * c6;
+        w[1] = __builtin_fmaf(h2, __builtin_fmaf(0.5f, h, -1.0f), c23);
+        w[2] = __builtin_fmaf(t2, __builtin_fmaf(0.5f, t, -1.0f), c23);
+        w[3] = (h2 * h) * c6;
+    } else {
+        const float c120 = 1.0f / 120.0f;
+        const float t = 1.0f - h, h2 = h * h, t2 = t * t;
+        w[0] = ((t2 * t2) * t) * c120;
+        w[1] = q5a(h) * c120;
+        w[2] = q5b(h) * c120;
+        w[3] = q5b(t) * c120;
+        w[4] = q5a(t) * c120;
+        w[5] = ((h2 * h2) * h) * c120;
+    }
+}
+
+// factorised gather  F = sum_i sx[i] * (sum_j sy[j] * b[i + j*pitch])
+template <int NW, typename P>
+__device__ __forceinline__ float gather(P b, int pitch, const float *sx, const float *sy)
+{
+    float acc = 0.0f;
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+        float r = sy[0] * b[i];
+#pragma unroll
+        for (int j = 1; j < NW; j++) r = __builtin_fmaf(sy[j], b[i + j * pitch], r);
+        acc = (i == 0) ? sx[0] * r : __builtin_fmaf(sx[i], r, acc);
+    }
+    return acc;
+}
+
+// exact round-to-nearest-even of c * 2^40 as int64 (|c| < 2^22)
+__device__ __forceinline__ long long fx40(float c)
+{
+    const float r = __builtin_rintf(c * 0x1p40f);
+    const float a = fabsf(r);
+    const float hi = truncf(a * 0x1p-32f);
+    const float lo = __builtin_fmaf(hi, -4294967296.0f, a);
+    const unsigned long long u = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+    return r < 0.0f ? -(long long)u : (long long)u;
+}
+__device__ __forceinline__ unsigned q23(float s) { return (unsigned)__builtin_rintf(s * 8388608.0f); }
+
+// periodic wrap x = fmod(x + (ng-ngb), ng) + ngb (Particle.mm:216-221); the
+// two conditional subtractions are the exact remainder for 0 <= a < 3 ng.
+__device__ __forceinline__ float wrap(float x, int ng, int ngb)
+{
+    const float L = (float)ng;
+    float a = x + (float)(ng - ngb);
+    if (a >= 0.0f && a < 3.0f * L) {
+        if (a >= L) a -= L;
+        if (a >= L) a -= L;
+    } else {
+        a = fmodf(a, L);
+    }
+    return a + (float)ngb;
+}
+
+struct PushArgs {
+    float *x, *y, *vx, *vy, *vz;
+    int64_t n, n_binned, pfirst;
+    const int32_t *tile_off;
+    int ntx, ntiles, chunk, use_tiles;
+    const float *Ex, *Ey, *Bz;
+    float bz0;
+    float constE, constB, constX, constY;
+    int ngx, ngy, ngb, nx, ny;
+    int bc0, bc1, bc2, bc3, boris;
+    unsigned long long *rho, *jx, *jy;
+    float inv_vnorm;
+    uint32_t *dead;
+    unsigned long long *counters;
+    double *ke_part;
+    double ke_coef;
+};
+
+template <int ORDER, bool UNIB, int DEP, bool PUSH>
+__global__ __launch_bounds__(GP_BLOCK) void k_push(PushArgs a)
+{
+    constexpr int NGB = ORDER / 2, NW = ORDER + 1;
+    constexpr int W = WIN + 2 * NGB + 1;              // node window edge
+    constexpr int NF = PUSH ? W * W : 1;
+    constexpr int NB = (PUSH && !UNIB) ? W * W : 1;
+    constexpr int NR = DEP >= 1 ? W * W : 1;
+    constexpr int NJ = DEP >= 2 ? W * W : 1;
+    __shared__ float sEx[NF], sEy[NF], sBz[NB];
+    __shared__ unsigned long long sRho[NR], sJx[NJ], sJy[NJ];
+    __shared__ double sKE[GP_BLOCK / 64];
+
+    const int tid = threadIdx.x, b = blockIdx.x;
+    int64_t p0, p1;
+    int tiled = 0, tx0 = 0, ty0 = 0;
+    if (a.use_tiles && b < a.ntiles) {
+        p0 = a.tile_off[b];
+        p1 = a.tile_off[b + 1];
+        if (p1 > a.n) p1 = a.n;
+        tiled = 1;
+        tx0 = a.ngb + (b % a.ntx) * GP_TILE;
+        ty0 = a.ngb + (b / a.ntx) * GP_TILE;
+    } else {
+        const int64_t eb = a.use_tiles ? (b - a.ntiles) : b;
+        p0 = a.pfirst + eb * a.chunk;
+        p1 = p0 + a.chunk;
+        if (p1 > a.n) p1 = a.n;
+    }
+    const int cx0 = tx0 - GP_DRIFT, cy0 = ty0 - GP_DRIFT;   // first window cell
+    const int X0 = cx0 - NGB, Y0 = cy0 - NGB;                // first window node
+    const int pex = a.nx + 2, pey = a.nx + 1;
+
+    if (tiled && p0 < p1) {
+        for (int e = tid; e < W * W; e += GP_BLOCK) {
+            const int gx = X0 + e % W, gy = Y0 + e / W;
+            if constexpr (PUSH) {
+                sEx[e] = (gx >= 0 && gx < a.nx + 2 && gy >= 0 && gy < a.ny + 1) ? a.Ex[gx + gy * pex] : 0.0f;
+                sEy[e] = (gx >= 0 && gx < a.nx + 1 && gy >= 0 && gy < a.ny + 2) ? a.Ey[gx + gy * pey] : 0.0f;
+                if constexpr (!UNIB)
+                    sBz[e] = (gx >= 0 && gx < a.nx + 2 && gy >= 0 && gy < a.ny + 2) ? a.Bz[gx + gy * pex] : 0.0f;
+            }
+            if constexpr (DEP >= 1) sRho[e] = 0ull;
+            if constexpr (DEP >= 2) { sJx[e] = 0ull; sJy[e] = 0ull; }
+        }
+    }
+    __syncthreads();
+
+    const float lox = (float)a.ngb, hix = (float)(a.ngb + a.ngx);
+    const float loy = (float)a.ngb, hiy = (float)(a.ngb + a.ngy);
+    double ke = 0.0;
+
+    for (int64_t p = p0 + tid; p < p1; p += GP_BLOCK) {
+        float x = a.x[p], y = a.y[p];
+        float vx = a.vx[p], vy = a.vy[p];
+        const float vz = a.vz[p];
+        // live particles lie in [ngb, ngb+ng); anything else (absorbed
+        // marker, escaped, NaN) is left untouched and not deposited
+        if (!(x >= lox && x < hix && y >= loy && y < hiy)) {
+            if (x >= 0.0f) atomicAdd(&a.counters[5], 1ull);
+            continue;
+        }
+        int flag = 0;
+        if constexpr (PUSH) {
+            const float xh = x - 0.5f, yh = y - 0.5f;
+            const float fi1 = floorf(xh), fj1 = floorf(y), fi2 = floorf(x), fj2 = floorf(yh);
+            const int i1 = (int)fi1, j1 = (int)fj1, i2 = (int)fi2, j2 = (int)fj2;
+            float sx0[NW], sy0[NW], sx1[NW], sy1[NW];
+            wts<ORDER>(xh - fi1, sx0);
+            wts<ORDER>(y - fj1, sy0);
+            wts<ORDER>(x - fi2, sx1);
+            wts<ORDER>(yh - fj2, sy1);
+            float Epx, Epy, Bpz;
+            const bool inw = tiled && (unsigned)(i2 - cx0) < (unsigned)WIN && (unsigned)(j1 - cy0) < (unsigned)WIN;
+            if (inw) {
+                Epx = gather<NW>(sEx + (i1 + 1 - NGB - X0) + (j1 - NGB - Y0) * W, W, sx0, sy0);
+                Epy = gather<NW>(sEy + (i2 - NGB - X0) + (j2 + 1 - NGB - Y0) * W, W, sx1, sy1);
+                if constexpr (UNIB) Bpz = a.bz0;
+                else Bpz = gather<NW>(sBz + (i1 + 1 - NGB - X0) + (j2 + 1 - NGB - Y0) * W, W, sx0, sy1);
+            } else {
+                Epx = gather<NW>(a.Ex + (i1 + 1 - NGB) + (int64_t)(j1 - NGB) * pex, pex, sx0, sy0);
+                Epy = gather<NW>(a.Ey + (i2 - NGB) + (int64_t)(j2 + 1 - NGB) * pey, pey, sx1, sy1);
+                if constexpr (UNIB) Bpz = a.bz0;
+                else Bpz = gather<NW>(a.Bz + (i1 + 1 - NGB) + (int64_t)(j2 + 1 - NGB) * pex, pex, sx0, sy1);
+            }
+            // Boris (Particle.mm:156-209) for the gathered components
+            // Ex, Ey, Bz; Ez = Bx = By = 0 exactly as in the reference.
+            const float ex0 = a.constE * Epx, ey0 = a.constE * Epy;
+            float etx = ex0, ety = ey0, btz = a.constB * Bpz;
+            const float btbt = btz * btz;
+            if (a.boris == 4 || a.boris == 6) {
+                const float fN1 = (a.boris == 4) ? (1.0f + btbt / 3.0f)
+                                                 : (1.0f + btbt / 3.0f + 2.0f * btbt * btbt / 15.0f);
+                etx = fN1 * etx; ety = fN1 * ety; btz = fN1 * btz;
+            }
+            const float umx = vx + etx, umy = vy + ety;
+            const float v0x = __builtin_fmaf(umy, btz, umx);
+            const float v0y = __builtin_fmaf(-umx, btz, umy);
+            const float ssz = (2.0f * btz) / (1.0f + btbt);
+            const float upx = __builtin_fmaf(v0y, ssz, umx);
+            const float upy = __builtin_fmaf(-v0x, ssz, umy);
+            vx = upx + ex0;
+            vy = upy + ey0;
+            x = __builtin_fmaf(vx, a.constX, x);
+            y = __builtin_fmaf(vy, a.constY, y);
+            // particle BCs (Particle.mm:215-269)
+            if (a.bc0 == GP_PBC_PERIODIC) x = wrap(x, a.ngx, a.ngb);
+            if (a.bc2 == GP_PBC_PERIODIC) y = wrap(y, a.ngy, a.ngb);
+            if (a.bc0 == GP_PBC_DELETE && x < lox) flag = 1;
+            if (a.bc1 == GP_PBC_DELETE && x >= hix) flag = 2;
+            if (a.bc2 == GP_PBC_DELETE && y < loy) flag = 3;
+            if (a.bc3 == GP_PBC_DELETE && y >= hiy) flag = 4;
+            if (a.bc0 == GP_PBC_REFLECT && x < lox) { x = lox + lox - x; vx = -vx; }
+            if (a.bc1 == GP_PBC_REFLECT && x >= hix) { x = hix + hix - x; vx = -vx; }
+            if (a.bc2 == GP_PBC_REFLECT && y < loy) { y = loy + loy - y; vy = -vy; }
+            if (a.bc3 == GP_PBC_REFLECT && y >= hiy) { y = hiy + hiy - y; vy = -vy; }
+            // a wrapped or reflected position can round onto the upper wall
+            if (!flag) {
+                if (x >= hix) x = __int_as_float(__float_as_int(hix) - 1);
+                if (y >= hiy) y = __int_as_float(__float_as_int(hiy) - 1);
+            }
+            a.x[p] = flag ? -(float)flag : x;
+            a.y[p] = y;
+            a.vx[p] = vx;
+            a.vy[p] = vy;
+        }
+        if (flag) {
+            atomicAdd(&a.counters[flag], 1ull);
+            const unsigned long long slot = atomicAdd(&a.counters[0], 1ull);
+            a.dead[slot] = (uint32_t)p;
+            continue;
+        }
+        if constexpr (PUSH) {
+            const float vv = vx * vx + vy * vy + vz * vz;
+            ke += a.ke_coef * (double)vv;
+        }
+        if constexpr (DEP >= 1) {
+            // charge (Particle.mm:357-361): node (i2+i-ngb, j1+j-ngb), weights
+            // of x - floor(x) and y - floor(y)
+            if (!(x >= lox && x < hix && y >= loy && y < hiy)) {
+                atomicAdd(&a.counters[5], 1ull);
+                continue;
+            }
+            const float fi2 = floorf(x), fj1 = floorf(y);
+            const int i2 = (int)fi2, j1 = (int)fj1;
+            float sx[NW], sy[NW];
+            wts<ORDER>(x - fi2, sx);
+            wts<ORDER>(y - fj1, sy);
+            unsigned qx[NW], qy[NW];
+#pragma unroll
+            for (int k = 0; k < NW; k++) { qx[k] = q23(sx[k]); qy[k] = q23(sy[k]); }
+            const bool inw = tiled && (unsigned)(i2 - cx0) < (unsigned)WIN && (unsigned)(j1 - cy0) < (unsigned)WIN;
+            if (inw) {
+                unsigned long long *r = sRho + (i2 - NGB - X0) + (j1 - NGB - Y0) * W;
+#pragma unroll
+                for (int j = 0; j < NW; j++)
+#pragma unroll
+                    for (int i = 0; i < NW; i++)
+                        atomicAdd(r + i + j * W, (unsigned long long)qx[i] * qy[j]);
+            } else {
+                unsigned long long *r = a.rho + (i2 - NGB) + (int64_t)(j1 - NGB) * pey;
+#pragma unroll
+                for (int j = 0; j < NW; j++)
+#pragma unroll
+                    for (int i = 0; i < NW; i++)
+                        atomicAdd(r + i + (int64_t)j * pey, (unsigned long long)qx[i] * qy[j]);
+            }
+            if constexpr (DEP >= 2) {
+                // current (Particle.mm:363-373)
+                const float xh = x - 0.5f, yh = y - 0.5f;
+                const float fi1 = floorf(xh), fj2 = floorf(yh);
+                const int i1 = (int)fi1, j2 = (int)fj2;
+                float sx0[NW], sy1[NW];
+                wts<ORDER>(xh - fi1, sx0);
+                wts<ORDER>(yh - fj2, sy1);
+                const float vnx = vx * a.inv_vnorm, vny = vy * a.inv_vnorm;
+                unsigned long long *jxb, *jyb;
+                int pjx, pjy;
+                if (inw) {
+                    jxb = sJx + (i1 + 1 - NGB - X0) + (j1 - NGB - Y0) * W; pjx = W;
+                    jyb = sJy + (i2 - NGB - X0) + (j2 + 1 - NGB - Y0) * W; pjy = W;
+                } else {
+                    jxb = a.jx + (i1 + 1 - NGB) + (int64_t)(j1 - NGB) * pex; pjx = pex;
+                    jyb = a.jy + (i2 - NGB) + (int64_t)(j2 + 1 - NGB) * pey; pjy = pey;
+                }
+#pragma unroll
+                for (int j = 0; j < NW; j++)
+#pragma unroll
+                    for (int i = 0; i < NW; i++) {
+                        atomicAdd(jxb + i + (int64_t)j * pjx, (unsigned long long)fx40((sx0[i] * sy[j]) * vnx));
+                        atomicAdd(jyb + i + (int64_t)j * pjy, (unsigned long long)fx40((sx[i] * sy1[j]) * vny));
+                    }
+            }
+        }
+    }
+
+    if constexpr (PUSH) {
+        // deterministic workgroup sum of the kinetic energy
+        for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off, 64);
+        if ((tid & 63) == 0) sKE[tid >> 6] = ke;
+    }
+    __syncthreads();
+    if constexpr (PUSH) {
+        if (tid == 0) {
+            double s = 0.0;
+            for (int w = 0; w < GP_BLOCK / 64; w++) s += sKE[w];
+            a.ke_part[b] = s;
+        }
+    }
+    if constexpr (DEP >= 1) {
+        if (tiled && p0 < p1) {
+            for (int e = tid; e < W * W; e += GP_BLOCK) {
+                const int gx = X0 + e % W, gy = Y0 + e / W;
+                const unsigned long long v = sRho[e];
+                if (v && gx >= 0 && gx <= a.nx && gy >= 0 && gy <= a.ny) atomicAdd(&a.rho[gx + (int64_t)gy * pey], v);
+                if constexpr (DEP >= 2) {
+                    const unsigned long long u = sJx[e], w = sJy[e];
+                    if (u && gx >= 0 && gx <= a.nx + 1 && gy >= 0 && gy <= a.ny) atomicAdd(&a.jx[gx + (int64_t)gy * pex], u);
+                    if (w && gx >= 0 && gx <= a.nx && gy >= 0 && gy <= a.ny + 1) atomicAdd(&a.jy[gx + (int64_t)gy * pey], w);
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_sum_partials(const double *part, int n, double *out)
+{
+    // single workgroup, fixed order: thread t sums t, t+256, ...; then tree
+    __shared__ double s[GP_BLOCK];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += GP_BLOCK) acc += part[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = GP_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+// kinetic energy of the live particles without a push
+__global__ __launch_bounds__(GP_BLOCK) void k_ke(const float *x, const float *vx, const float *vy, const float *vz,
+                                                int64_t n, double coef, double *part)
+{
+    __shared__ double s[GP_BLOCK / 64];
+    double ke = 0.0;
+    for (int64_t p = blockIdx.x * (int64_t)GP_BLOCK + threadIdx.x; p < n; p += (int64_t)gridDim.x * GP_BLOCK) {
+        if (!(x[p] >= 0.0f)) continue;
+        const float a = vx[p], b = vy[p], c = vz[p];
+        const float vv = a * a + b * b + c * c;
+        ke += coef * (double)vv;
+    }
+    for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = ke;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < GP_BLOCK / 64; w++) t += s[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of uint32 (3-phase, recursive over block sums)
+// ---------------------------------------------------------------------------
+constexpr int SC_ITEMS = 8, SC_TILE = GP_BLOCK * SC_ITEMS;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *tot)
+{
+    __shared__ uint32_t ws[GP_BLOCK / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    uint32_t inc = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += t;
+    }
+    if (lane == 63) ws[wv] = inc;
+    __syncthreads();
+    uint32_t base = 0, all = 0;
+    for (int w = 0; w < GP_BLOCK / 64; w++) {
+        if (w < wv) base += ws[w];
+        all += ws[w];
+    }
+    __syncthreads();
+    *tot = all;
+    return base + inc - v;
+}
+
+__global__ __launch_bounds__(GP_BLOCK) void k_scan_block(uint32_t *d, int64_t n, uint32_t *sums)
+{
+    const int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_ITEMS;
+    uint32_t v[SC_ITEMS], s = 0;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        v[k] = (base + k < n) ? d[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t off = block_excl_scan(s, &tot);
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        if (base + k < n) d[base + k] = off;
+        off += v[k];
+    }
+    if (threadIdx.x == 0 && sums) sums[blockIdx.x] = tot;
+}
+__global__ void k_scan_add(uint32_t *d, int64_t n, const uint32_t *sums)
+{
+    const int64_t i = (int64_t)blockIdx.x * SC_TILE + threadIdx.x;
+    const uint32_t s = sums[blockIdx.x];
+    for (int k = 0; k < SC_ITEMS; k++) {
+        const int64_t j = i + (int64_t)k * GP_BLOCK;
+        if (j < n) d[j] += s;
+    }
+}
+
+int scan_u32(hipStream_t s, uint32_t *d, int64_t n)
+{
+    if (n <= 0) return GP_OK;
+    const int nb = gp_cdiv(n, SC_TILE);
+    if (nb == 1) {
+        k_scan_block<<<1, GP_BLOCK, 0, s>>>(d, n, nullptr);
+        GP_HIP(hipGetLastError());
+        return GP_OK;
+    }
+    uint32_t *sums = nullptr;
+    GP_HIP(hipMallocAsync((void **)&sums, sizeof(uint32_t) * nb, s));
+    k_scan_block<<<nb, GP_BLOCK, 0, s>>>(d, n, sums);
+    GP_HIP(hipGetLastError());
+    int rc = scan_u32(s, sums, nb);
+    if (rc) return rc;
+    k_scan_add<<<nb, GP_BLOCK, 0, s>>>(d, n, sums);
+    GP_HIP(hipGetLastError());
+    GP_HIP(hipFreeAsync(sums, s));
+    return GP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort of (uint32 key, uint32 value), 8-bit digits
+// ---------------------------------------------------------------------------
+constexpr int RS_TILE = 4096;
+
+__global__ __launch_bounds__(GP_BLOCK) void k_rs_hist(const uint32_t *keys, int64_t n, int shift,
+                                                      uint32_t *hist, int nblk)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int k = threadIdx.x; k < RS_TILE; k += GP_BLOCK) {
+        const int64_t i = base + k;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(GP_BLOCK) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                                                         uint32_t *vout, int64_t n, int shift, int init_vals,
+                                                         const uint32_t *offs, int nblk)
+{
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wc[GP_BLOCK / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    run[tid] = offs[(int64_t)tid * nblk + blockIdx.x];
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int c = 0; c < RS_TILE / GP_BLOCK; c++) {
+        for (int w = 0; w < GP_BLOCK / 64; w++) wc[w][tid] = 0;
+        __syncthreads();
+        const int64_t i = base + (int64_t)c * GP_BLOCK + tid;
+        const bool valid = i < n;
+        const uint32_t k = valid ? kin[i] : 0u;
+        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t v = valid ? (init_vals ? (uint32_t)i : vin[i]) : 0u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            const bool on = (d >> bit) & 1u;
+            const unsigned long long bb = __ballot(valid && on);
+            peers &= on ? bb : ~bb;
+        }
+        const int rank = __popcll(peers & lt);
+        if (valid && rank == 0) wc[wv][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = run[d] + rank;
+            for (int w = 0; w < wv; w++) off += wc[w][d];
+            kout[off] = k;
+            vout[off] = v;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int w = 0; w < GP_BLOCK / 64; w++) add += wc[w][tid];
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// sort keys, permutation gather, tile offsets
+// ---------------------------------------------------------------------------
+__global__ void k_tile_key(const float *x, const float *y, int64_t n, int ngb, int ngx, int ngy, int ntx,
+                           uint32_t *key)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ci = (int)floorf(x[i]) - ngb, cj = (int)floorf(y[i]) - ngb;
+    ci = ci < 0 ? 0 : (ci >= ngx ? ngx - 1 : ci);
+    cj = cj < 0 ? 0 : (cj >= ngy ? ngy - 1 : cj);
+    key[i] = (uint32_t)((cj / GP_TILE) * ntx + ci / GP_TILE);
+}
+__global__ void k_permute(gp_soa in, gp_soa out, const uint32_t *perm, int64_t n)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = perm[i];
+    out.x[i] = in.x[s]; out.y[i] = in.y[s];
+    out.vx[i] = in.vx[s]; out.vy[i] = in.vy[s]; out.vz[i] = in.vz[s];
+}
+__global__ void k_tile_off(const uint32_t *key, int64_t n, int ntiles, int32_t *off)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = (int)key[i];
+    const int kp = i > 0 ? (int)key[i - 1] : -1;
+    for (int t = kp + 1; t <= k; t++) off[t] = (int32_t)i;
+    if (i == n - 1)
+        for (int t = k + 1; t <= ntiles; t++) off[t] = (int32_t)n;
+}
+
+// ---------------------------------------------------------------------------
+// compaction: holes (absorbed, index < n_new) ascending are filled by the
+// live particles of the tail [n_new, n) taken from the end, which is the
+// permutation of the reference's loop (Particle.mm:906-968).
+// ---------------------------------------------------------------------------
+__global__ void k_lower_bound(const uint32_t *a, int64_t n, uint32_t v, uint32_t *out)
+{
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (a[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    *out = (uint32_t)lo;
+}
+__global__ void k_tail_flags(const float *x, int64_t n_new, int64_t len, uint32_t *flag)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k < len) flag[k] = (x[n_new + k] >= 0.0f) ? 1u : 0u;
+}
+__global__ void k_donors(const float *x, int64_t n_new, int64_t len, const uint32_t *pos, uint32_t *donor)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k < len && x[n_new + k] >= 0.0f) donor[pos[k]] = (uint32_t)(n_new + k);
+}
+__global__ void k_fill(gp_soa s, const uint32_t *holes, const uint32_t *donor, const uint32_t *nh)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t h = *nh;
+    if (k >= h) return;
+    const uint32_t d = donor[h - 1 - k], t = holes[k];
+    s.x[t] = s.x[d]; s.y[t] = s.y[d]; s.vx[t] = s.vx[d]; s.vy[t] = s.vy[d]; s.vz[t] = s.vz[d];
+}
+
+// ---------------------------------------------------------------------------
+// seeded generation (Collision.mm:63-174, seeding :280-294)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t smix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+struct Pcg {
+    uint64_t s, inc;
+    __device__ __forceinline__ uint32_t next()
+    {
+        const uint64_t old = s;
+        s = old * 6364136223846793005ULL + inc;
+        const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((-rot) & 31u));
+    }
+    // (u >> 9) + 1/2 is exact in FP32: the result lies strictly inside (0, 1)
+    __device__ __forceinline__ float u01() { return ((float)(next() >> 9) + 0.5f) * 0x1p-23f; }
+};
+__device__ __forceinline__ Pcg seed_stream(uint64_t master, int64_t id)
+{
+    const uint64_t g = 0x9E3779B97F4A7C15ULL;
+    Pcg r;
+    r.s = smix(master + (uint64_t)(2 * id + 1) * g);
+    r.inc = smix(master + (uint64_t)(2 * id + 2) * g) | 1ULL;
+    return r;
+}
+
+struct LoadArgs {
+    gp_gen g;
+    float dx, dy, ngb;
+    uint64_t seed;
+    int64_t first_id, n, ntot;
+    gp_soa e, i;
+    int64_t ne, ni;
+    int with_ion;
+};
+
+__global__ void k_load(LoadArgs a)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    const float PIf = 3.14159265358979f;
+    const int64_t id = a.first_id + k;
+    Pcg r = seed_stream(a.seed, id);
+    const gp_gen &g = a.g;
+    const float rx = r.u01(), ry = r.u01();
+    float x, y;
+    switch (g.gen_type) {
+    default:
+    case 0: case 3:
+        x = g.xmin + rx * (g.xmax - g.xmin);
+        y = g.ymin + ry * (g.ymax - g.ymin);
+        break;
+    case 1:
+        x = (g.xmin + g.xmax) / 2.0f + (g.xmax - g.xmin) / PIf * asinf(2.0f * rx - 1.0f);
+        y = g.ymin + ry * (g.ymax - g.ymin);
+        break;
+    case 2:
+        x = g.xmin + rx * (g.xmax - g.xmin);
+        y = (g.ymin + g.ymax) / 2.0f + (g.ymax - g.ymin) / PIf * asinf(2.0f * ry - 1.0f);
+        break;
+    case 4: {
+        int ax = (int)roundf(sqrtf((float)a.ntot * (g.xmax - g.xmin) / (g.ymax - g.ymin)));
+        if (ax < 1) ax = 1;
+        int ay = (int)(a.ntot / ax);
+        if (ay < 1) ay = 1;
+        const float dq = (g.xmax - g.xmin) / (float)ax, eq = (g.ymax - g.ymin) / (float)ay;
+        x = g.xmin + dq * (0.5f + (float)(id % ax));
+        y = g.ymin + eq * (0.5f + (float)(id / ax));
+        break; }
+    case 5: {
+        const float rz = r.u01();
+        x = (g.xmin + g.xmax) / 2.0f + (g.xmax - g.xmin) * sqrtf(-logf(rx)) * cosf(2.0f * PIf * rz);
+        y = g.ymin + ry * (g.ymax - g.ymin);
+        break; }
+    case 6: {
+        const float rz = r.u01();
+        x = g.xmin + rx * (g.xmax - g.xmin);
+        y = (g.ymin + g.ymax) / 2.0f + (g.ymax - g.ymin) * sqrtf(-logf(rz)) * cosf(2.0f * PIf * ry);
+        break; }
+    case 7: {
+        const double L = (double)g.xmax - (double)g.xmin;
+        const double kk = 2.0 * 3.141592653589793 * (double)g.pert_mode / L, d = g.pert_delta;
+        const double tgt = (double)rx * L;
+        double s = tgt;
+        for (int it = 0; it < 8; it++) {
+            const double f = s + d / kk * sin(kk * s) - tgt, fp = 1.0 + d * cos(kk * s);
+            s -= f / fp;
+        }
+        x = (float)((double)g.xmin + s);
+        y = g.ymin + ry * (g.ymax - g.ymin);
+        break; }
+    }
+    float rv1 = r.u01(), rv2 = r.u01();
+    float am = g.ele_vth * sqrtf(-logf(rv1));
+    const float vx = g.ele_u[0] + am * cosf(2.0f * PIf * rv2);
+    const float vy = g.ele_u[1] + am * sinf(2.0f * PIf * rv2);
+    rv1 = r.u01(); rv2 = r.u01();
+    const float vz = g.ele_u[2] + g.ele_vth * sqrtf(-logf(rv1)) * cosf(2.0f * PIf * rv2);
+    x = x / a.dx + a.ngb;
+    y = y / a.dy + a.ngb;
+    const int64_t je = a.ne + k;
+    a.e.x[je] = x; a.e.y[je] = y; a.e.vx[je] = vx; a.e.vy[je] = vy; a.e.vz[je] = vz;
+    if (a.with_ion && g.gen_type != 3) {
+        rv1 = r.u01(); rv2 = r.u01();
+        am = g.ion_vth * sqrtf(-logf(rv1));
+        const int64_t ji = a.ni + k;
+        a.i.vx[ji] = g.ion_u[0] + am * cosf(2.0f * PIf * rv2);
+        a.i.vy[ji] = g.ion_u[1] + am * sinf(2.0f * PIf * rv2);
+        rv1 = r.u01(); rv2 = r.u01();
+        a.i.vz[ji] = g.ion_u[2] + g.ion_vth * sqrtf(-logf(rv1)) * cosf(2.0f * PIf * rv2);
+        a.i.x[ji] = x; a.i.y[ji] = y;
+    }
+}
+
+// half-Maxwellian flux through the x-min wall: vx Rayleigh (flux-weighted),
+// vy, vz Gaussian; the particle entered during the step, x = wall + vx*dt*r
+struct InjArgs {
+    gp_soa s;
+    int64_t base, n;
+    uint64_t seed, stream;
+    float vth, u0, u1, u2;
+    float dtdx, lo, ngx, ngy;
+};
+__global__ void k_inject(InjArgs a)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    const float PIf = 3.14159265358979f;
+    Pcg r = seed_stream(a.seed, (int64_t)(a.stream + (uint64_t)k));
+    const float r1 = r.u01(), r2 = r.u01(), r3 = r.u01(), r4 = r.u01(), r5 = r.u01();
+    const float vx = a.u0 + a.vth * sqrtf(-2.0f * logf(r1));
+    const float am = a.vth * sqrtf(-2.0f * logf(r2));
+    const float vy = a.u1 + am * cosf(2.0f * PIf * r3);
+    const float vz = a.u2 + am * sinf(2.0f * PIf * r3);
+    float x = a.lo + vx * a.dtdx * r4;
+    const float hi = a.lo + a.ngx;
+    if (x >= hi) x = __int_as_float(__float_as_int(hi) - 1);   // a particle faster than L/dt stays inside
+    const float y = a.lo + r5 * a.ngy;
+    const int64_t j = a.base + k;
+    a.s.x[j] = x; a.s.y[j] = y; a.s.vx[j] = vx; a.s.vy[j] = vy; a.s.vz[j] = vz;
+}
+
+// moments (Moment.mm:33-135): node weights of int(x + 1e-20)
+struct MomArgs {
+    const float *x, *y, *vx, *vy, *vz;
+    int64_t n;
+    int ngb, ngx, ngy, nx;
+    float inv_vnorm;
+    unsigned long long *acc;   // 10 grids
+    int64_t ng;
+    unsigned long long *escaped;
+};
+template <int ORDER>
+__global__ void k_moments(MomArgs a)
+{
+    constexpr int NW = ORDER + 1, NGB = ORDER / 2;
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const float x = a.x[p], y = a.y[p];
+    if (!(x >= (float)a.ngb && x < (float)(a.ngb + a.ngx) && y >= (float)a.ngb && y < (float)(a.ngb + a.ngy))) {
+        if (x >= 0.0f) atomicAdd(a.escaped, 1ull);
+        return;
+    }
+    const int i1 = (int)(x + 1e-20f), j1 = (int)(y + 1e-20f);
+    float sx[NW], sy[NW];
+    wts<ORDER>(x - (float)i1, sx);
+    wts<ORDER>(y - (float)j1, sy);
+    const float v[3] = {a.vx[p] * a.inv_vnorm, a.vy[p] * a.inv_vnorm, a.vz[p] * a.inv_vnorm};
+    unsigned qx[NW], qy[NW];
+    for (int k = 0; k < NW; k++) { qx[k] = q23(sx[k]); qy[k] = q23(sy[k]); }
+    const int pit = a.nx + 1;
+    for (int j = 0; j < NW; j++)
+        for (int i = 0; i < NW; i++) {
+            const int64_t node = (i1 + i - NGB) + (int64_t)(j1 + j - NGB) * pit;
+            const float s = sx[i] * sy[j];
+            atomicAdd(&a.acc[node], (unsigned long long)qx[i] * qy[j]);
+            atomicAdd(&a.acc[1 * a.ng + node], (unsigned long long)fx40(s * v[0]));
+            atomicAdd(&a.acc[2 * a.ng + node], (unsigned long long)fx40(s * v[1]));
+            atomicAdd(&a.acc[3 * a.ng + node], (unsigned long long)fx40(s * v[2]));
+            atomicAdd(&a.acc[4 * a.ng + node], (unsigned long long)fx40((s * v[0]) * v[0]));
+            atomicAdd(&a.acc[5 * a.ng + node], (unsigned long long)fx40((s * v[0]) * v[1]));
+            atomicAdd(&a.acc[6 * a.ng + node], (unsigned long long)fx40((s * v[0]) * v[2]));
+            atomicAdd(&a.acc[7 * a.ng + node], (unsigned long long)fx40((s * v[1]) * v[1]));
+            atomicAdd(&a.acc[8 * a.ng + node], (unsigned long long)fx40((s * v[1]) * v[2]));
+            atomicAdd(&a.acc[9 * a.ng + node], (unsigned long long)fx40((s * v[2]) * v[2]));
+        }
+}
+__global__ void k_mom_scale(const unsigned long long *acc, int64_t ng, double sn, double sv, double svv, double *out)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= 10 * ng) return;
+    const int c = (int)(k / ng);
+    const double f = c == 0 ? sn : (c <= 3 ? sv : svv);
+    out[k] = (double)(long long)acc[k] * f;
+}
+
+// ---------------------------------------------------------------------------
+template <int ORDER, bool UNIB, int DEP, bool PUSH>
+void launch_push_t(int grid, hipStream_t s, const PushArgs &a)
+{
+    k_push<ORDER, UNIB, DEP, PUSH><<<grid, GP_BLOCK, 0, s>>>(a);
+}
+template <int ORDER>
+int launch_push_o(int grid, hipStream_t s, const PushArgs &a, bool unib, int dep, bool push)
+{
+    if (push) {
+        if (unib) {
+            if (dep == 0) launch_push_t<ORDER, true, 0, true>(grid, s, a);
+            else if (dep == 1) launch_push_t<ORDER, true, 1, true>(grid, s, a);
+            else launch_push_t<ORDER, true, 2, true>(grid, s, a);
+        } else {
+            if (dep == 0) launch_push_t<ORDER, false, 0, true>(grid, s, a);
+            else if (dep == 1) launch_push_t<ORDER, false, 1, true>(grid, s, a);
+            else launch_push_t<ORDER, false, 2, true>(grid, s, a);
+        }
+    } else {
+        if (dep == 1) launch_push_t<ORDER, true, 1, false>(grid, s, a);
+        else launch_push_t<ORDER, true, 2, false>(grid, s, a);
+    }
+    GP_HIP(hipGetLastError());
+    return GP_OK;
+}
+
+int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64_t first, int64_t count)
+{
+    gp_species &S = c->sp[sid];
+    PushArgs a{};
+    a.x = S.a.x; a.y = S.a.y; a.vx = S.a.vx; a.vy = S.a.vy; a.vz = S.a.vz;
+    const int64_t end = first + count;
+    a.n = end;
+    const bool whole = (first == 0 && end == S.n);
+    a.use_tiles = whole && S.n_binned > 0 ? 1 : 0;
+    a.n_binned = a.use_tiles ? std::min(S.n_binned, S.n) : 0;
+    a.pfirst = a.use_tiles ? a.n_binned : first;
+    a.tile_off = S.tile_off;
+    a.ntx = c->ntx; a.ntiles = c->ntiles;
+    a.chunk = 16 * GP_BLOCK;
+    a.Ex = c->Ex; a.Ey = c->Ey; a.Bz = c->Bz; a.bz0 = c->bz0;
+    a.constE = (float)(0.5 * (double)S.qf * dt / (double)S.mf);             // Particle.mm:618
+    a.constB = (float)(0.5 * (double)S.qf * dt / (double)S.mf / 2.99792458e10);  // Particle.mm:619
+    a.constX = (float)(dt / (double)c->dxf);                                 // Particle.mm:620
+    a.constY = (float)(dt / (double)c->dyf);
+    a.ngx = c->ngx; a.ngy = c->ngy; a.ngb = c->ngb; a.nx = c->nx; a.ny = c->ny;
+    a.bc0 = c->cfg.pbc[0]; a.bc1 = c->cfg.pbc[1]; a.bc2 = c->cfg.pbc[2]; a.bc3 = c->cfg.pbc[3];
+    a.boris = c->cfg.boris_order;
+    a.rho = (unsigned long long *)S.rho_acc;
+    a.jx = (unsigned long long *)S.jx_acc;
+    a.jy = (unsigned long long *)S.jy_acc;
+    a.inv_vnorm = (float)S.inv_vnorm;
+    a.dead = S.dead;
+    a.counters = S.counters;
+    a.ke_part = S.ke_part;
+    a.ke_coef = 0.5 * (double)S.mf * (double)S.wf;
+    const int64_t rest = end - a.pfirst;
+    const int extra = rest > 0 ? gp_cdiv(rest, a.chunk) : 0;
+    const int grid = (a.use_tiles ? c->ntiles : 0) + extra;
+    if (grid == 0) return GP_OK;
+    if (grid > S.ke_nblk) return gp_fail(GP_ERR_STATE, "kinetic-energy partial buffer too small");
+    int rc;
+    switch (c->order) {
+    case 1: rc = launch_push_o<1>(grid, c->stream, a, c->uniform_b, dep, push); break;
+    case 3: rc = launch_push_o<3>(grid, c->stream, a, c->uniform_b, dep, push); break;
+    default: rc = launch_push_o<5>(grid, c->stream, a, c->uniform_b, dep, push); break;
+    }
+    if (rc) return rc;
+    if (push) {
+        k_sum_partials<<<1, GP_BLOCK, 0, c->stream>>>(S.ke_part, grid, S.ke_out);
+        GP_HIP(hipGetLastError());
+    }
+    return GP_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// host entry points used by gp_api.hip
+// ===========================================================================
+int gp_radix_sort_u32(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp,
+                      int64_t n, int bits, int init_vals, uint32_t **keys_out, uint32_t **vals_out)
+{
+    uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
+    if (n <= 0) { *keys_out = ki; *vals_out = vi; return GP_OK; }
+    const int nblk = gp_cdiv(n, RS_TILE);
+    uint32_t *hist = nullptr;
+    GP_HIP(hipMallocAsync((void **)&hist, sizeof(uint32_t) * 256 * (size_t)nblk, s));
+    const int passes = bits <= 0 ? 1 : (bits + 7) / 8;
+    for (int ps = 0; ps < passes; ps++) {
+        const int shift = 8 * ps;
+        k_rs_hist<<<nblk, GP_BLOCK, 0, s>>>(ki, n, shift, hist, nblk);
+        GP_HIP(hipGetLastError());
+        int rc = scan_u32(s, hist, 256 * (int64_t)nblk);
+        if (rc) return rc;
+        k_rs_scatter<<<nblk, GP_BLOCK, 0, s>>>(ki, vi, ko, vo, n, shift, (ps == 0) ? init_vals : 0, hist, nblk);
+        GP_HIP(hipGetLastError());
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    GP_HIP(hipFreeAsync(hist, s));
+    *keys_out = ki;
+    *vals_out = vi;
+    return GP_OK;
+}
+
+int gp_launch_push(gp_ctx *c, int sid, double dt, int deposit)
+{
+    gp_species &S = c->sp[sid];
+    const int dep = deposit ? (c->cfg.cur_dens ? 2 : 1) : 0;
+    if (c->timing && S.n > 0) { int rc = gp_time_mark(c, 0); if (rc) return rc; }
+    int rc = run_particle_kernel(c, sid, dt, dep, true, 0, S.n);
+    if (rc) return rc;
+    if (c->timing && S.n > 0) { rc = gp_time_mark(c, 0); if (rc) return rc; }
+    return GP_OK;
+}
+
+int gp_launch_deposit(gp_ctx *c, int sid, int64_t first, int64_t count)
+{
+    const int dep = c->cfg.cur_dens ? 2 : 1;
+    return run_particle_kernel(c, sid, 0.0, dep, false, first, count);
+}
+
+int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke)
+{
+    gp_species &S = c->sp[sid];
+    hipStream_t s = c->stream;
+    unsigned long long h[8];
+    GP_HIP(hipMemcpyAsync(h, S.counters, sizeof(h), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    if (h[5]) return gp_fail(GP_ERR_STATE, "particles outside the domain with no boundary condition: " + std::to_string(h[5]));
+    const int64_t nd = (int64_t)h[0];
+    for (int w = 0; w < 4; w++) counts[w] = (int64_t)h[1 + w];
+    if (ke) {
+        GP_HIP(hipMemcpyAsync(ke, S.ke_out, sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    if (nd > 0) {
+        if (nd > S.n) return gp_fail(GP_ERR_STATE, "absorbed count exceeds particle count");
+        const int64_t n_new = S.n - nd;
+        // sort the absorbed indices ascending
+        int bits = 1;
+        while (bits < 32 && ((int64_t)1 << bits) < S.n) bits++;
+        uint32_t *ks = nullptr, *vs = nullptr;
+        int rc = gp_radix_sort_u32(s, S.dead, S.val, S.dead_tmp, S.val2, nd, bits, 1, &ks, &vs);
+        if (rc) return rc;
+        uint32_t *nh = S.key2;         // device scalar: number of holes
+        uint32_t *flags = S.key;       // tail flags -> positions (nd)
+        uint32_t *donor = S.key2 + 1;  // donors (<= nd)
+        k_lower_bound<<<1, 1, 0, s>>>(ks, nd, (uint32_t)n_new, nh);
+        GP_HIP(hipGetLastError());
+        const int g = gp_cdiv(nd, GP_BLOCK);
+        k_tail_flags<<<g, GP_BLOCK, 0, s>>>(S.a.x, n_new, nd, flags);
+        GP_HIP(hipGetLastError());
+        rc = scan_u32(s, flags, nd);
+        if (rc) return rc;
+        k_donors<<<g, GP_BLOCK, 0, s>>>(S.a.x, n_new, nd, flags, donor);
+        GP_HIP(hipGetLastError());
+        k_fill<<<g, GP_BLOCK, 0, s>>>(S.a, ks, donor, nh);
+        GP_HIP(hipGetLastError());
+        S.n = n_new;
+        if (S.n_binned > S.n) S.n_binned = S.n;
+    }
+    GP_HIP(hipMemsetAsync(S.counters, 0, sizeof(unsigned long long) * 8, s));
+    GP_HIP(hipStreamSynchronize(s));
+    return GP_OK;
+}
+
+int gp_launch_sort(gp_ctx *c, int sid)
+{
+    gp_species &S = c->sp[sid];
+    hipStream_t s = c->stream;
+    const int64_t n = S.n;
+    if (n == 0) {
+        GP_HIP(hipMemsetAsync(S.tile_off, 0, sizeof(int32_t) * (c->ntiles + 1), s));
+        S.n_binned = 0;
+        return GP_OK;
+    }
+    if (c->timing) { int rc = gp_time_mark(c, 1); if (rc) return rc; }
+    const int g = gp_cdiv(n, GP_BLOCK);
+    k_tile_key<<<g, GP_BLOCK, 0, s>>>(S.a.x, S.a.y, n, c->ngb, c->ngx, c->ngy, c->ntx, S.key);
+    GP_HIP(hipGetLastError());
+    uint32_t *ks = nullptr, *vs = nullptr;
+    int rc = gp_radix_sort_u32(s, S.key, S.val, S.key2, S.val2, n, c->tile_bits, 1, &ks, &vs);
+    if (rc) return rc;
+    k_permute<<<g, GP_BLOCK, 0, s>>>(S.a, S.b, vs, n);
+    GP_HIP(hipGetLastError());
+    std::swap(S.a, S.b);
+    k_tile_off<<<g, GP_BLOCK, 0, s>>>(ks, n, c->ntiles, S.tile_off);
+    GP_HIP(hipGetLastError());
+    S.n_binned = n;
+    if (c->timing) { int rc = gp_time_mark(c, 1); if (rc) return rc; }
+    return GP_OK;
+}
+
+int gp_launch_load(gp_ctx *c, int se, int si, const gp_gen *g, uint64_t seed, int64_t first_id, int64_t n)
+{
+    if (n <= 0) return GP_OK;
+    gp_species &E = c->sp[se];
+    LoadArgs a{};
+    a.g = *g;
+    a.dx = c->dxf; a.dy = c->dyf; a.ngb = (float)c->ngb;
+    a.seed = seed; a.first_id = first_id; a.n = n; a.ntot = first_id + n;
+    a.e = E.a; a.ne = E.n;
+    a.with_ion = si >= 0;
+    if (si >= 0) { a.i = c->sp[si].a; a.ni = c->sp[si].n; }
+    if (E.n + n > E.cap || (si >= 0 && c->sp[si].n + n > c->sp[si].cap))
+        return gp_fail(GP_ERR_CAPACITY, "pNum > pNumMax in particle generation");
+    k_load<<<gp_cdiv(n, GP_BLOCK), GP_BLOCK, 0, c->stream>>>(a);
+    GP_HIP(hipGetLastError());
+    E.n += n;
+    if (si >= 0 && g->gen_type != 3) c->sp[si].n += n;
+    return GP_OK;
+}
+
+int gp_launch_inject_wall(gp_ctx *c, int sid, int64_t n, float vth, const float u[3], double dt,
+                          uint64_t seed, uint64_t stream, int deposit)
+{
+    if (n <= 0) return GP_OK;
+    gp_species &S = c->sp[sid];
+    if (S.n + n > S.cap) return gp_fail(GP_ERR_CAPACITY, "pNum > pNumMax in wall injection");
+    InjArgs a{};
+    a.s = S.a; a.base = S.n; a.n = n; a.seed = seed; a.stream = stream;
+    a.vth = vth; a.u0 = u[0]; a.u1 = u[1]; a.u2 = u[2];
+    a.dtdx = (float)(dt / (double)c->dxf);
+    a.lo = (float)c->ngb;
+    a.ngx = (float)c->ngx;
+    a.ngy = (float)c->ngy;
+    k_inject<<<gp_cdiv(n, GP_BLOCK), GP_BLOCK, 0, c->stream>>>(a);
+    GP_HIP(hipGetLastError());
+    const int64_t first = S.n;
+    S.n += n;
+    if (deposit) return gp_launch_deposit(c, sid, first, n);
+    return GP_OK;
+}
+
+int gp_launch_moments(gp_ctx *c, int sid, double *host10)
+{
+    gp_species &S = c->sp[sid];
+    hipStream_t s = c->stream;
+    const int64_t ng = (int64_t)(c->nx + 1) * (c->ny + 1);
+    unsigned long long *acc = nullptr;
+    double *out = nullptr;
+    GP_HIP(hipMallocAsync((void **)&acc, sizeof(unsigned long long) * 10 * ng, s));
+    GP_HIP(hipMallocAsync((void **)&out, sizeof(double) * 10 * ng, s));
+    GP_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * 10 * ng, s));
+    MomArgs a{};
+    a.x = S.a.x; a.y = S.a.y; a.vx = S.a.vx; a.vy = S.a.vy; a.vz = S.a.vz;
+    a.n = S.n; a.ngb = c->ngb; a.ngx = c->ngx; a.ngy = c->ngy; a.nx = c->nx;
+    a.inv_vnorm = (float)S.inv_vnorm; a.acc = acc; a.ng = ng; a.escaped = S.counters + 5;
+    if (S.n > 0) {
+        const int g = gp_cdiv(S.n, GP_BLOCK);
+        if (c->order == 1) k_moments<1><<<g, GP_BLOCK, 0, s>>>(a);
+        else if (c->order == 3) k_moments<3><<<g, GP_BLOCK, 0, s>>>(a);
+        else k_moments<5><<<g, GP_BLOCK, 0, s>>>(a);
+        GP_HIP(hipGetLastError());
+    }
+    // Moment.mm:242 scale = w/(dx*dy) on n; velocity sums are unscaled
+    const double sn = ldexp(1.0, -GP_RHO_FRAC) * ((double)S.wf / ((double)c->dxf * (double)c->dyf));
+    const double sv = ldexp(1.0, -GP_CUR_FRAC) * S.vnorm;
+    const double svv = ldexp(1.0, -GP_CUR_FRAC) * S.vnorm * S.vnorm;
+    k_mom_scale<<<gp_cdiv(10 * ng, GP_BLOCK), GP_BLOCK, 0, s>>>(acc, ng, sn, sv, svv, out);
+    GP_HIP(hipGetLastError());
+    GP_HIP(hipMemcpyAsync(host10, out, sizeof(double) * 10 * ng, hipMemcpyDeviceToHost, s));
+    GP_HIP(hipFreeAsync(acc, s));
+    GP_HIP(hipFreeAsync(out, s));
+    GP_HIP(hipStreamSynchronize(s));
+    return GP_OK;
+}
+
+int gp_launch_ke(gp_ctx *c, int sid)
+{
+    gp_species &S = c->sp[sid];
+    const int g = std::min(1024, std::max(1, gp_cdiv(S.n, GP_BLOCK)));
+    k_ke<<<g, GP_BLOCK, 0, c->stream>>>(S.a.x, S.a.vx, S.a.vy, S.a.vz, S.n, 0.5 * (double)S.mf * (double)S.wf, S.ke_part);
+    GP_HIP(hipGetLastError());
+    k_sum_partials<<<1, GP_BLOCK, 0, c->stream>>>(S.ke_part, g, S.ke_out);
+    GP_HIP(hipGetLastError());
+    return GP_OK;
+}

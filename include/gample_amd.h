@@ -1,0 +1,238 @@
+/*
+ * gample_amd.h -- C-ABI of the MI355X particle-in-cell step.
+ *
+ * One shared library (libgample_amd.so) built for gfx950.  All device memory
+ * (particles, grids, solver hierarchy, network weights) is owned by a
+ * context; the host holds opaque handles and receives copies on request.
+ * Every call returns an int status (GP_OK = 0; GP_NOT_CONVERGED = 1 is a
+ * non-fatal solver status; negative values are errors whose text is given by
+ * gp_last_error()).  No C++ exception crosses this boundary.  Calls on one
+ * context are ordered on the context's HIP stream and are not re-entrant;
+ * separate contexts are independent.
+ *
+ * Each entry point names the reference interface it replaces (paths are in
+ * the reference tree, tnaoki0630/gample).
+ */
+#ifndef GAMPLE_AMD_H
+#define GAMPLE_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GP_OK               0
+#define GP_NOT_CONVERGED    1
+#define GP_ERR_ARG        (-1)
+#define GP_ERR_HIP        (-2)
+#define GP_ERR_NOMEM      (-3)
+#define GP_ERR_CAPACITY   (-4)
+#define GP_ERR_STATE      (-5)
+#define GP_ERR_UNSUPPORTED (-6)
+#define GP_ERR_NONFINITE  (-7)
+
+/* particle boundary codes (Particle.mm:461-509 function constants 0-3) */
+#define GP_PBC_NONE     (-1)
+#define GP_PBC_PERIODIC   0
+#define GP_PBC_DELETE     1
+#define GP_PBC_REFLECT    2
+/* field boundary codes (EMField.mm:200-233) */
+#define GP_FBC_PERIODIC   0
+#define GP_FBC_DIRICHLET  1
+#define GP_FBC_NEUMANN    2
+
+typedef struct gp_ctx gp_ctx;
+
+/* Grid, weighting and boundary set-up.  Replaces the parsed
+ * ParamForField / BoundaryCondition* structs (Init.h:9-99) as consumed by
+ * -[EMField initWithDevice:withParam:withLogger:] (EMField.mm:105-656) and
+ * -[Particle initWithDevice:...] (Particle.mm:407-611). */
+typedef struct {
+    int    ngx, ngy;          /* interior cells (NumberOfGridX/Y) */
+    double dx, dy;            /* cell size, cm (GridSizeOfX/Y) */
+    int    weight_order;      /* 1, 3 or 5 (WeightingOrder) */
+    int    boris_order;       /* 2, 4 or 6 (BorisOrder) */
+    int    pbc[4];            /* particle BC Xmin, Xmax, Ymin, Ymax */
+    int    fbc[4];            /* field BC Xmin, Xmax, Ymin, Ymax */
+    double fbc_value[4];      /* Dirichlet potential, statV */
+    double e0[3];             /* uniform initial E, statV/cm (AmplitudeOfE) */
+    double b0[3];             /* uniform initial B, G (AmplitudeOfB, static) */
+    int    cur_dens;          /* 1: deposit Jx, Jy as well (FlagForCurDens / EMField=2) */
+    int    em_model;          /* 1 electrostatic, 2 Darwin post-solve */
+    int    device;            /* HIP device ordinal */
+} gp_config;
+
+/* Species constants.  Replaces ParamForParticle (Init.h:40-54) and the
+ * Particle object's m, q, w members (Particle.mm:420-424). */
+typedef struct {
+    double q, m, w;           /* statC, g, macro-particle weight */
+    int64_t capacity;         /* MaximumParticleNumber */
+    double vnorm;             /* velocity scale of the current fixed-point
+                                 format, cm/s (rounded up to a power of 2);
+                                 0 = choose from vth_hint */
+    double vth_hint;          /* thermal speed sqrt(kT/m), cm/s */
+} gp_species_desc;
+
+/* Seeded particle generation.  Replaces generationParams and the
+ * artificialIonization kernel's initial load (Collision.h:25-41,
+ * Collision.mm:63-174, 280-336).  gen_type: 0 uniform, 1 Xsinusoidal,
+ * 2 Ysinusoidal, 4 quietStart, 5 Xgaussian, 6 Ygaussian, 7 cosine
+ * perturbation 1 + pert_delta*cos(2 pi pert_mode (x-xmin)/(xmax-xmin)). */
+typedef struct {
+    int   gen_type;
+    float xmin, xmax, ymin, ymax;     /* cm */
+    float ele_u[3], ele_vth;          /* drift, sqrt(2kT/m) as the reference */
+    float ion_u[3], ion_vth;
+    float pert_delta; int pert_mode;
+} gp_gen;
+
+/* Solver statistics.  Replaces the (iters, error) pair logged by
+ * solvePoisson (EMField.mm:866, 1119-1127). */
+typedef struct {
+    int    iterations;        /* inner preconditioned-CG iterations */
+    int    refinements;       /* FP64 outer corrections */
+    double relres;            /* ||b - A phi||_2 / ||b||_2 in FP64 */
+    double seconds;           /* device time of the solve */
+    int    used_nn;           /* 1 if the network warm start / FPCG ran */
+    int    fell_back;         /* 1 if NN-FPCG fell back to MG-CG */
+} gp_solve_stats;
+
+/* per-step diagnostics: the logged scalars of reduce (Particle.mm:990-1001)
+ * and solvePoisson (EMField.mm:1119-1127) */
+typedef struct {
+    double kinetic[8];        /* per species, erg, FP64 */
+    double we_ref;            /* sum(E^2) dx dy / 2 over whole arrays (reference form) */
+    double we_cgs;            /* E^2/(8 pi) over unique points, erg/cm */
+    double joule;             /* sum(E.J) dt dx dy */
+    int64_t count[8];         /* live particles per species */
+    int64_t absorbed[8][4];   /* particles absorbed per species per wall this step */
+} gp_diag;
+
+/* field identifiers for gp_field_get / gp_field_set */
+enum {
+    GP_F_RHO = 0,    /* double (nx+1)(ny+1), folded total charge density */
+    GP_F_EX,         /* float (nx+2)(ny+1) */
+    GP_F_EY,         /* float (nx+1)(ny+2) */
+    GP_F_EZ,         /* float (nx+1)(ny+1) */
+    GP_F_BX,         /* float (nx+1)(ny+2) */
+    GP_F_BY,         /* float (nx+2)(ny+1) */
+    GP_F_BZ,         /* float (nx+2)(ny+2) */
+    GP_F_PHI,        /* double (nx+3)(ny+3) */
+    GP_F_JX,         /* double (nx+2)(ny+1) total current (not folded) */
+    GP_F_JY,         /* double (nx+1)(ny+2) */
+    GP_F_DEL_EX,     /* float, Darwin transverse correction */
+    GP_F_DEL_EY,
+    GP_F_DEL_BZ,
+    GP_F_SOL,        /* double Poisson unknowns (nkx+1)(nky+1) */
+    GP_F_COUNT
+};
+/* per-species accumulator identifiers for gp_species_grid_get */
+enum { GP_G_RHO_ACC = 0, GP_G_JX_ACC, GP_G_JY_ACC };
+
+/* ---- context ---------------------------------------------------------- */
+/* Replaces EMField init (EMField.mm:105-656): allocates fields, builds the
+ * Poisson/Darwin solver hierarchy. */
+int  gp_ctx_create(const gp_config *cfg, gp_ctx **out);
+int  gp_ctx_destroy(gp_ctx *ctx);
+const char *gp_last_error(void);
+int  gp_sync(gp_ctx *ctx);
+/* grid geometry: nx = ngx+2ngb, ny, nkx, nky, ikmin, jkmin */
+int  gp_dims(const gp_ctx *ctx, int out[6]);
+
+/* ---- species ---------------------------------------------------------- */
+/* Replaces -[Particle initWithDevice:withParam:specimen:withLogger:]
+ * (Particle.mm:407-611): device SoA store of capacity particles. */
+int  gp_species_create(gp_ctx *ctx, const gp_species_desc *d, int *sid);
+/* host SoA -> device (replaces the shared-memory particleBuffer writes,
+ * Particle.mm:548-597, and FileIO loadProgress, FileIO.mm:295-352) */
+int  gp_species_upload(gp_ctx *ctx, int sid, int64_t n, const float *x, const float *y,
+                       const float *vx, const float *vy, const float *vz);
+/* device -> host copy (replaces reads of particleBuffer contents) */
+int  gp_species_download(gp_ctx *ctx, int sid, int64_t cap, int64_t *n, float *x, float *y,
+                         float *vx, float *vy, float *vz);
+int  gp_species_count(gp_ctx *ctx, int sid, int64_t *n);
+/* per-species int64 accumulators (deposit output before scaling) */
+int  gp_species_grid_get(gp_ctx *ctx, int sid, int which, int64_t *host, int64_t count);
+
+/* ---- generation ------------------------------------------------------- */
+/* Replaces the initial artificialIonization dispatch of
+ * -[Collision initWithDevice:...] (Collision.mm:280-336): n electron/ion
+ * pairs appended with PCG32 streams seeded by splitmix64(master_seed) in
+ * particle-id order starting at first_id.  sid_ion < 0: electrons only. */
+int  gp_load_pair(gp_ctx *ctx, int sid_ele, int sid_ion, const gp_gen *g, uint64_t master_seed,
+                  int64_t first_id, int64_t n);
+/* Half-Maxwellian flux injection through the x-min wall (no reference
+ * counterpart; the reference's nearest analogue is the hollow-cathode line
+ * source, Collision.mm:568-678): n particles with vx from the flux-weighted
+ * Rayleigh law, vy, vz Gaussian, x = wall + vx*dt*r.  deposit != 0 also
+ * deposits them into the current accumulators. */
+int  gp_inject_wall(gp_ctx *ctx, int sid, int64_t n, float vth, const float u[3], double dt,
+                    uint64_t seed, uint64_t stream, int deposit);
+
+/* ---- step pieces (reference order) ------------------------------------ */
+/* -[EMField resetChargeDensity] (EMField.mm:1363-1378) */
+int  gp_reset_grids(gp_ctx *ctx);
+/* -[Particle integrateChargeDensity:withMoment:withLogger:]
+ * (Particle.mm:1004-1040 + kernel :281-389): int64 fixed-point deposit of
+ * every live particle of sid into its accumulators (rho, and J if enabled). */
+int  gp_deposit(gp_ctx *ctx, int sid);
+/* -[EMField solvePoisson:withLogger:] (EMField.mm:745-1136): scale and sum
+ * the species accumulators, fold ghosts, solve to tol (relative residual in
+ * FP64), reconstruct phi and E = -grad phi, energies.  Clears the
+ * accumulators afterwards.  solver: 0 MG-CG, 1 NN warm start + FPCG. */
+int  gp_solve_fields(gp_ctx *ctx, double dt, double tol, int maxiter, int solver,
+                     gp_solve_stats *st, gp_diag *diag);
+/* -[EMField solveVectorPotential:withLogger:] (EMField.mm:1138-1361) */
+int  gp_solve_darwin(gp_ctx *ctx, double dt, double tol, int maxiter, int solver,
+                     gp_solve_stats st[3]);
+/* -[Particle update:withEMField:withMom:withLogger:] (Particle.mm:615-672 +
+ * kernel :65-278).  deposit != 0 fuses the next step's deposit of the
+ * pushed particles into the same kernel (Particle.mm:281-389). */
+int  gp_push(gp_ctx *ctx, int sid, double dt, int deposit);
+/* -[Particle reduce:] (Particle.mm:893-1002): compacts absorbed particles
+ * (holes filled from the tail), per-wall counts, FP64 kinetic energy. */
+int  gp_reduce(gp_ctx *ctx, int sid, int64_t counts[4], double *kinetic);
+/* cell/tile sort of the SoA store by a GPU LSD radix sort (no reference
+ * counterpart: the reference keeps insertion order) */
+int  gp_sort(gp_ctx *ctx, int sid);
+/* -[Moment integrateMoments:withEMField:withLogger:] (Moment.mm:206-270 +
+ * kernel :33-135): n, n*v (3), n*vv (6) node moments in FP64,
+ * 10*(nx+1)*(ny+1) doubles, unnormalised as the kernel leaves them. */
+int  gp_moments(gp_ctx *ctx, int sid, double *host10);
+
+/* ---- whole step ------------------------------------------------------- */
+/* One cycle of the main loop (main.mm:154-254) for every species, with the
+ * deposit fused into the push: fields from the current accumulators, push +
+ * deposit, reduce, and a cell sort every sort_interval cycles (0 = never). */
+int  gp_step(gp_ctx *ctx, double dt, double tol, int maxiter, int solver, int sort_interval,
+             int cycle, gp_solve_stats *st, gp_diag *diag);
+
+/* ---- fields ----------------------------------------------------------- */
+int  gp_field_get(gp_ctx *ctx, int which, void *host, int64_t bytes);
+int  gp_field_set(gp_ctx *ctx, int which, const void *host, int64_t bytes);
+/* diagnostics of the current E and J (EMField.mm:1096-1116) */
+int  gp_field_energy(gp_ctx *ctx, double dt, double *we_ref, double *we_cgs, double *joule);
+
+/* ---- timing ----------------------------------------------------------- */
+/* HIP-event timing of the push kernel on the context stream: enable = 1
+ * starts recording, and gp_kernel_time returns total ms and launch count
+ * since enabling.  which: 0 push(+deposit), 1 sort, 2 field solve. */
+int  gp_timing(gp_ctx *ctx, int enable);
+int  gp_kernel_time(gp_ctx *ctx, int which, double *total_ms, int64_t *launches);
+
+/* ---- learned preconditioner ------------------------------------------ */
+/* Build and train the convolutional warm start / preconditioner on (rho,
+ * phi) pairs held by the caller (host arrays, nsamp x (nkx+1)(nky+1)
+ * doubles), BF16 MFMA kernels with FP32 accumulate, Adam.  No reference
+ * counterpart (the reference solves with AMGCL only, EMField.mm:866). */
+int  gp_nn_init(gp_ctx *ctx, int base_channels, uint64_t seed);
+int  gp_nn_train(gp_ctx *ctx, const double *rho, const double *phi, int nsamp, int epochs,
+                 int batch, float lr, float *loss_out);
+int  gp_nn_infer(gp_ctx *ctx, const double *rhs, double *out);
+int  gp_nn_save(gp_ctx *ctx, const char *path);
+int  gp_nn_load(gp_ctx *ctx, const char *path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

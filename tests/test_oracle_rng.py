@@ -1,0 +1,88 @@
+"""Pins of the oracle's generators (src/Collision.h:11-16,
+src/Collision.mm:21-30, 63-174, 280-336) against published known-answer
+vectors (tests/golden/rng_kat.json, see make_kat.py) and distribution
+properties of the seeded loader."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+KAT = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "rng_kat.json")))
+
+
+def test_splitmix64_kat(orc):
+    s = C.c_uint64(1234567)
+    got = [orc.lib().gpo_splitmix64(C.byref(s)) for _ in range(5)]
+    assert got == KAT["splitmix64_seed_1234567"]
+    s = C.c_uint64(0)
+    assert orc.lib().gpo_splitmix64(C.byref(s)) == KAT["splitmix64_seed_0_first"]
+
+
+def test_pcg32_kat(orc):
+    # pcg32_srandom_r(42, 54): state = 0, inc = 54*2+1, step, state += 42, step
+    inc = (54 << 1) | 1
+    st = C.c_uint64(0)
+    orc.lib().gpo_pcg32(C.byref(st), inc)
+    st.value = (st.value + 42) & ((1 << 64) - 1)
+    orc.lib().gpo_pcg32(C.byref(st), inc)
+    got = [orc.lib().gpo_pcg32(C.byref(st), inc) for _ in range(6)]
+    assert got == KAT["pcg32_srandom_42_54"]
+
+
+def test_rand01_open_interval(orc):
+    assert orc.lib().gpo_rand01(0) == 2.0 ** -24                 # (0 + 1/2) * 2^-23
+    assert orc.lib().gpo_rand01(0xFFFFFFFF) == 1.0 - 2.0 ** -24  # never rounds to 1
+
+
+def _gen(orc, gt=0, vth_e=1e8, vth_i=1e5, L=1.0):
+    g = orc.Gen()
+    g.gen_type = gt
+    g.xmin, g.xmax, g.ymin, g.ymax = 0.0, L, 0.0, L
+    g.ele_u[:] = [0.0, 0.0, 0.0]
+    g.ion_u[:] = [0.0, 0.0, 0.0]
+    g.ele_vth, g.ion_vth = vth_e, vth_i
+    g.dx = g.dy = L / 64
+    g.ngb = 1.0
+    g.pert_delta, g.pert_mode = 0.0, 0
+    return g
+
+
+def test_loader_uniform_maxwellian(orc):
+    n = 200000
+    g = _gen(orc)
+    e, i = orc.load_pair(g, 0, 0, n)
+    for a in (e[0], e[1]):
+        assert a.min() >= 1.0 and a.max() < 65.0
+        h = np.histogram(a, bins=16, range=(1, 65))[0]
+        assert np.all(np.abs(h / (n / 16) - 1) < 0.03)
+    np.testing.assert_array_equal(e[0], i[0])      # ions co-located (Collision.mm:150-151)
+    np.testing.assert_array_equal(e[1], i[1])
+    # Box-Muller with sqrt(-ln r): per-component variance vth^2/2
+    for v, vth in ((e[2], 1e8), (e[3], 1e8), (e[4], 1e8), (i[2], 1e5)):
+        assert abs(np.mean(v)) < 0.01 * vth
+        assert abs(np.var(v.astype(np.float64)) / (vth * vth / 2) - 1) < 0.02
+
+
+def test_loader_counter_based(orc):
+    """Particle id k draws from its own stream, so a slice [a, b) generated
+    alone equals the same slice of a full generation."""
+    g = _gen(orc)
+    full, _ = orc.load_pair(g, 7, 0, 5000)
+    part, _ = orc.load_pair(g, 7, 1200, 800)
+    for a, b in zip(full, part):
+        np.testing.assert_array_equal(a[1200:2000], b)
+
+
+def test_loader_cosine_perturbation(orc):
+    """gen_type 7: density 1 + delta cos(2 pi m (x - xmin)/L)."""
+    n = 400000
+    g = _gen(orc, gt=7)
+    g.pert_delta, g.pert_mode = 0.2, 3
+    e, _ = orc.load_pair(g, 3, 0, n, ion=False)
+    xs = (e[0] - 1.0) / 64.0
+    h, edges = np.histogram(xs, bins=48, range=(0, 1))
+    c = 0.5 * (edges[1:] + edges[:-1])
+    expect = n / 48 * (1 + 0.2 * np.cos(2 * np.pi * 3 * c))
+    assert np.max(np.abs(h - expect) / expect) < 0.05
