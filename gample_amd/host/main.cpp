@@ -1,0 +1,173 @@
+// main.cpp -- `gample -i condition.json [-o tag] [--unfused] [--dump-config]`
+//
+// The driver loop of the reference (src/main.mm:28-263): parse the
+// condition file, build the modules, load particles, initial deposit and
+// Poisson solve, then for each cycle: deposit -> Poisson [-> Darwin] ->
+// push -> reduce, with the Kahan-compensated time (main.mm:150-163).  By
+// default the deposit of the next cycle is fused into the push kernel
+// (gp_step); --unfused runs the pieces in the reference's order.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "Constant.h"
+#include "Init.h"
+#include "Modules.h"
+
+using namespace gample;
+
+static std::map<std::string, std::string> parseArgs(int argc, char **argv)   // main.mm:15-26
+{
+    std::map<std::string, std::string> o;
+    for (int i = 1; i < argc; i++) {
+        std::string k = argv[i];
+        if (k == "--unfused" || k == "--dump-config") { o[k] = "1"; continue; }
+        if (k[0] == '-' && i + 1 < argc) { o[k] = argv[i + 1]; i++; }
+    }
+    return o;
+}
+
+static void dump(const Init &in)
+{
+    const auto &F = in.paramForField;
+    std::printf("{\"Particle\":%d,\"EMField\":%d,\"Start\":%d,\"End\":%d,\"TimeStep\":%.17g,"
+                "\"pNumMax\":%lld,\"maxiter\":%d,\"tolerance\":%.17g,\"ngx\":%d,\"ngy\":%d,\"ngb\":%d,"
+                "\"dx\":%.17g,\"dy\":%.17g,\"weightOrder\":%d,\"BorisOrder\":%d,\"checkCurDens\":%d,\"species\":[",
+                in.flagForEquation.Particle, in.flagForEquation.EMField, in.paramForTimeIntegration.Start,
+                in.paramForTimeIntegration.End, in.paramForTimeIntegration.TimeStep, in.paramForComputing.pNumMax,
+                in.paramForComputing.maxiter, in.paramForComputing.tolerance, F.ngx, F.ngy, F.ngb, F.dx, F.dy,
+                F.weightOrder, F.BorisOrder, F.checkCurDens);
+    for (size_t s = 0; s < in.paramForParticle.size(); s++) {
+        const auto &p = in.paramForParticle[s];
+        std::printf("%s{\"name\":\"%s\",\"pNum\":%lld,\"q\":%.17g,\"m\":%.17g,\"w\":%.17g,\"genType\":\"%s\","
+                    "\"genT\":%.17g}", s ? "," : "", p.pName.c_str(), p.pNum, p.q, p.m, p.w, p.genType.c_str(), p.genT);
+    }
+    std::printf("],\"pbc\":[");
+    for (size_t i = 0; i < in.particleBoundaries.size(); i++)
+        std::printf("%s[\"%s\",\"%s\"]", i ? "," : "", in.particleBoundaries[i].position.c_str(),
+                    in.particleBoundaries[i].type.c_str());
+    std::printf("],\"fbc\":[");
+    for (size_t i = 0; i < in.fieldBoundaries.size(); i++)
+        std::printf("%s[\"%s\",\"%s\",%.17g]", i ? "," : "", in.fieldBoundaries[i].position.c_str(),
+                    in.fieldBoundaries[i].type.c_str(), in.fieldBoundaries[i].val);
+    std::printf("],\"sources\":%zu}\n", in.particleSources.size());
+}
+
+int main(int argc, char **argv)
+{
+    auto args = parseArgs(argc, argv);
+    if (!args.count("-i")) {
+        std::fprintf(stderr, "Usage: %s -i inputfile.json [-o tag] [--unfused]\n", argv[0]);
+        return 1;
+    }
+    Init init;
+    try {
+        init.parseInputFile(args["-i"]);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "[Fatal] parseInputFile Failed: %s\n", e.what());
+        return -1;
+    }
+    if (args.count("--dump-config")) { dump(init); return 0; }
+    std::string tag;
+    if (args.count("-o")) tag = args["-o"];
+    else {
+        char b[32];
+        std::time_t t = std::time(nullptr);
+        std::strftime(b, sizeof b, "%Y%m%d%H%M%S", std::localtime(&t));
+        tag = b;
+    }
+    const auto &T = init.paramForTimeIntegration;
+    if (T.Start != 0) {
+        std::fprintf(stderr, "[Fatal] restart (Start != 0) is not supported by this build\n");
+        return 1;
+    }
+    if (!init.particleSources.empty())
+        std::fprintf(stderr, "[warn] SourceForParticle entries (artificial ionization / hollow cathode) are "
+                             "not run by this build\n");
+    const bool fused = !args.count("--unfused");
+    try {
+        Logger logger(T.ProjectName + "_" + tag, true);
+        EMField fld(init, 0);
+        std::vector<Particle> ptcl;
+        ptcl.reserve(init.flagForEquation.Particle);
+        for (int s = 0; s < init.flagForEquation.Particle; s++) ptcl.emplace_back(fld, init, s);
+        Collision col(fld, init, ptcl);
+        (void)col;
+        Moment mom(fld);
+        (void)mom;
+        // initial field (main.mm:137-141)
+        fld.resetChargeDensity();
+        for (auto &p : ptcl) p.integrateChargeDensity();
+        gp_solve_stats st0 = fld.solvePoisson(0.0, nullptr);
+        std::printf("initial solve: %d iterations, relres %.3e\n", st0.iterations, st0.relres);
+        if (fused) {   // accumulators for cycle 1 (the solve consumed them)
+            fld.resetChargeDensity();
+            for (auto &p : ptcl) p.integrateChargeDensity();
+        }
+        const double dt = T.TimeStep;
+        double time = T.Start * dt, comp = 0.0;
+        const int sort_every = init.paramForComputing.sortInterval;
+        auto t_start = std::chrono::steady_clock::now();
+        for (int cycle = T.Start + 1; cycle <= T.End; cycle++) {
+            const double y = dt - comp, t = time + y;
+            comp = (t - time) - y;
+            time = t;
+            const bool logc = T.LogOutput > 0 && cycle % T.LogOutput == 0;
+            auto c0 = std::chrono::steady_clock::now();
+            gp_solve_stats st;
+            gp_diag dg;
+            std::memset(&dg, 0, sizeof dg);
+            std::vector<std::map<std::string, std::string>> flow;
+            if (fused) {
+                check(gp_step(fld.ctx, dt, fld.tol, fld.maxiter, fld.solver, sort_every, cycle, &st, logc ? &dg : nullptr),
+                      "gp_step");
+            } else {
+                fld.resetChargeDensity();
+                for (auto &p : ptcl) p.integrateChargeDensity();
+                st = fld.solvePoisson(dt, logc ? &dg : nullptr);
+                if (init.flagForEquation.EMField == 2) {
+                    gp_solve_stats ds[3];
+                    fld.solveVectorPotential(dt, ds);
+                }
+                for (auto &p : ptcl) {
+                    p.update(dt, false);
+                    flow.push_back(p.reduce());
+                }
+                if (sort_every > 0 && cycle % sort_every == 0)
+                    for (auto &p : ptcl) check(gp_sort(fld.ctx, p.sid), "sort");
+            }
+            auto c1 = std::chrono::steady_clock::now();
+            if (logc) {
+                double phi, ex, ey;
+                fld.probe(phi, ex, ey);
+                logger.probe(cycle, time, phi, ex, ey);
+                char b[64];
+                std::map<std::string, std::string> row;
+                row["cycle"] = std::to_string(cycle);
+                std::snprintf(b, sizeof b, "%.9e", time); row["time"] = b;
+                row["iteration"] = std::to_string(st.iterations);
+                std::snprintf(b, sizeof b, "%.3e", st.relres); row["error"] = b;
+                std::snprintf(b, sizeof b, "%.6e", dg.we_ref * ergtoev); row["totalElectricEnergy"] = b;
+                std::snprintf(b, sizeof b, "%.9e", dg.we_cgs); row["fieldEnergyCGS"] = b;
+                std::snprintf(b, sizeof b, "%.9e", dg.kinetic[0] + dg.kinetic[1]); row["kineticEnergy"] = b;
+                row["elapsed_us"] = std::to_string(
+                    std::chrono::duration_cast<std::chrono::microseconds>(c1 - c0).count());
+                for (size_t s = 0; s < ptcl.size(); s++)
+                    row["pNum_" + ptcl[s].pName] = std::to_string(ptcl[s].count());
+                logger.row(row);
+                std::printf("Frame %d completed\n", cycle);
+            }
+        }
+        auto t_end = std::chrono::steady_clock::now();
+        std::printf("calculation end. total elapsed time: %.3f sec.\n",
+                    std::chrono::duration<double>(t_end - t_start).count());
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "[Fatal] %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}
