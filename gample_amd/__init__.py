@@ -26,7 +26,8 @@ GP_NOT_CONVERGED = 1
 PBC = {"none": -1, "periodic": 0, "Delete": 1, "Reflect": 2}
 FBC = {"periodic": 0, "Dirichlet": 1, "Neumann": 2}
 
-F_RHO, F_EX, F_EY, F_EZ, F_BX, F_BY, F_BZ, F_PHI, F_JX, F_JY, F_DEL_EX, F_DEL_EY, F_DEL_BZ, F_SOL = range(14)
+(F_RHO, F_EX, F_EY, F_EZ, F_BX, F_BY, F_BZ, F_PHI, F_JX, F_JY, F_DEL_EX, F_DEL_EY, F_DEL_BZ, F_SOL, F_CHI, F_AX,
+ F_AY) = range(17)
 G_RHO_ACC, G_JX_ACC, G_JY_ACC = range(3)
 
 
@@ -286,7 +287,15 @@ class Context:
                 F_JX: ((ny + 1, nx + 2), np.float64), F_JY: ((ny + 2, nx + 1), np.float64),
                 F_DEL_EX: ((ny + 1, nx + 2), np.float32), F_DEL_EY: ((ny + 2, nx + 1), np.float32),
                 F_DEL_BZ: ((ny + 2, nx + 2), np.float32),
-                F_SOL: ((self.nky + 1, self.nkx + 1), np.float64)}[which]
+                F_SOL: ((self.nky + 1, self.nkx + 1), np.float64)}.get(which) or self._darwin_shape(which)
+
+    def _darwin_shape(self, which):
+        perx = self.cfg.fbc[0] == 0
+        pery = self.cfg.fbc[2] == 0
+        mx, my = self.cfg.ngx - (1 if perx else 0) + 1, self.cfg.ngy - (1 if pery else 0) + 1
+        if which in (F_CHI, F_AX, F_AY):
+            return ((my, mx), np.float64)
+        raise KeyError(which)
 
     def field(self, which):
         shape, dt = self.field_shape(which)

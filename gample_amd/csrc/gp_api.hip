@@ -422,8 +422,12 @@ int gp_field_get(gp_ctx *c, int which, void *host, int64_t bytes)
     case GP_F_DEL_EY: src = c->dEy; need = (nx + 1) * (ny + 2) * 4; break;
     case GP_F_DEL_BZ: src = c->dBz; need = (nx + 2) * (ny + 2) * 4; break;
     case GP_F_SOL: src = c->ps.x64; need = (int64_t)c->ps.n * 8; break;
+    case GP_F_CHI: src = c->dsol_chi; need = (int64_t)c->ds.n * 8; break;
+    case GP_F_AX: src = c->dsol_ax; need = (int64_t)c->ds.n * 8; break;
+    case GP_F_AY: src = c->dsol_ay; need = (int64_t)c->ds.n * 8; break;
     default: return gp_fail(GP_ERR_ARG, "unknown field id");
     }
+    if (!src) return gp_fail(GP_ERR_STATE, "field not allocated (Darwin fields need em_model = 2)");
     if (bytes != need) return gp_fail(GP_ERR_ARG, "field size mismatch: expected " + std::to_string(need) + " bytes");
     GP_HIP(hipMemcpyAsync(host, src, need, hipMemcpyDeviceToHost, c->stream));
     GP_HIP(hipStreamSynchronize(c->stream));

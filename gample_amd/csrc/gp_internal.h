@@ -19,7 +19,7 @@
 #include <vector>
 #include "../../include/gample_amd.h"
 
-#define GP_TILE 8            // tile edge, cells
+#define GP_TILE 16           // tile edge, cells
 #define GP_DRIFT 8           // cells a particle may leave its tile and still use LDS
 #define GP_BLOCK 256         // threads per workgroup of the particle kernels
 #define GP_MAX_SPECIES 8
@@ -61,11 +61,18 @@ struct gp_species {
     uint32_t *key = nullptr, *key2 = nullptr, *val = nullptr, *val2 = nullptr;
 };
 
+// multigrid axis types
+#define GP_AX_PER 0
+#define GP_AX_DIR 1
+#define GP_AX_NEU 2
+
 struct gp_mg_level {
     int mx, my;          // unknowns per axis
-    int perx, pery;      // periodic flags
+    int perx, pery;      // axis types GP_AX_* of x and y
     double hx2i, hy2i;   // 1/hx^2, 1/hy^2 at this level
-    float *u = nullptr, *f = nullptr, *r = nullptr;
+    float *u = nullptr;  // correction (output of the level)
+    float *f = nullptr;  // rhs
+    float *r = nullptr;  // pre-smoothed correction (down leg -> up leg)
 };
 
 struct gp_solver {
@@ -76,11 +83,13 @@ struct gp_solver {
     // FP64 fine vectors
     double *b64 = nullptr, *x64 = nullptr, *r64 = nullptr, *rhs_bc = nullptr;
     // FP32 PCG vectors
-    float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *e = nullptr;
+    float *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *q2 = nullptr, *e = nullptr;
+    int bottom = 0;            // first level solved inside one workgroup
     double *red = nullptr;     // reduction partials
     double *scal = nullptr;    // device scalars
     double *hscal = nullptr;   // pinned host mirror
     int red_blocks = 0;
+    int last_it = 4;           // inner iterations of the previous solve
 };
 
 struct gp_nn;   // gp_nn.hip
@@ -133,6 +142,14 @@ int gp_solve_fields_impl(gp_ctx *c, double dt, double tol, int maxiter, int solv
                          gp_solve_stats *st, gp_diag *diag);
 int gp_solve_darwin_impl(gp_ctx *c, double dt, double tol, int maxiter, int solver, gp_solve_stats st[3]);
 int gp_energy_impl(gp_ctx *c, double dt, double *we_ref, double *we_cgs, double *joule);
+
+// ---- elliptic solver (gp_mg.hip) ----
+int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double cy);
+void gp_mg_free(gp_solver &s);
+int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int singular, gp_solve_stats *stats,
+                 int (*precorrect)(void *, float *, const float *), void *pc_ctx);
+int gp_mg_sub_mean(hipStream_t st, gp_solver &s, double *a);
+int gp_mg_apply(hipStream_t st, gp_solver &s, const float *in, float *out);
 
 // ---- learned preconditioner (gp_nn.hip) ----
 int gp_nn_create(gp_ctx *c, int base_ch, uint64_t seed);

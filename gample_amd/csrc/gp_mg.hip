@@ -1,0 +1,911 @@
+// gp_mg.hip -- geometric multigrid preconditioned CG with FP64 refinement.
+//
+// Replaces the AMGCL solves of EMField.mm:866, 1199, 1237, 1275.  The system
+// is S x = b with S = D (-A): A is the reference's 5-point operator
+// (EMField.mm:236-405 Poisson, 483-623 Darwin) and D halves the rows of
+// Neumann boundary nodes (making the ghost-mirror rows symmetric).
+//
+// V-cycle (symmetric, so it is a valid CG preconditioner):
+//   pre-smoothing  one red-black Gauss-Seidel sweep (red, then black) from 0
+//   restriction    R = P^T / 4 (full weighting) of the residual
+//   coarse level   recursion; rediscretised operator (h doubles per level)
+//   prolongation   bilinear P, added
+//   post-smoothing one sweep in the adjoint order (black, then red)
+// Each grid level costs two launches: k_mg_down (red, black, residual,
+// restriction on a 32x32 tile whose 3-wide halo is recomputed locally) and
+// k_mg_up (prolongation, black, red with a 2-wide halo).  Recomputing the
+// halo reproduces global red-black Gauss-Seidel exactly.  All levels with at
+// most 4096 unknowns run inside one workgroup in LDS (k_mg_bottom).
+//
+// Outer loop in FP64: r = b - S x, stop at ||r||/||b|| <= tol; inner
+// correction in FP32 by MG-PCG (dot products accumulated in FP64), x += e.
+#include "gp_internal.h"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+namespace {
+
+constexpr int TB = 32;                 // tile of unknowns per workgroup
+constexpr int HD = 3, HU = 2;          // halos of the down / up kernels
+constexpr int WD = TB + 2 * HD, WU = TB + 2 * HU;
+constexpr int BOT_MAX = 4096;          // unknowns of the largest LDS level
+constexpr int BOT_LEVELS = 12;
+constexpr int RB = 256;                // workgroup of the reduction kernels
+
+struct Lev {
+    int mx, my;     // unknowns
+    int tx, ty;     // axis types GP_AX_*
+    int Mx, My;     // intervals
+    float cx, cy;   // 1/h^2
+    int rb;         // red-black colouring valid (no odd periodic axis)
+};
+
+// wrap for an index at most one period outside [0, m)
+__device__ __forceinline__ int wrp(int g, int m) { return g < 0 ? g + m : (g >= m ? g - m : g); }
+// wrap of a tile-window index: a window reaches up to TB + 2 HD past the
+// tile, more than one period when the periodic extent is small
+__device__ __forceinline__ int wwrap(int g, int m)
+{
+    const int w = wrp(g, m);
+    return (w >= 0 && w < m) ? w : ((g % m) + m) % m;
+}
+__device__ __forceinline__ float wgt(int t, int i, int m) { return (t == GP_AX_NEU && (i == 0 || i == m - 1)) ? 0.5f : 1.0f; }
+__device__ __forceinline__ bool in_axis(int t, int g, int m) { return t == GP_AX_PER || (g >= 0 && g < m); }
+// 1/w of the row scaling D (w = 1/2 on Neumann boundary nodes): 1, 2 or 4
+__device__ __forceinline__ float iweight(const Lev &L, int gi, int gj)
+{
+    float a = (L.tx == GP_AX_NEU && (gi == 0 || gi == L.mx - 1)) ? 2.0f : 1.0f;
+    float b = (L.ty == GP_AX_NEU && (gj == 0 || gj == L.my - 1)) ? 2.0f : 1.0f;
+    return a * b;
+}
+// red/black colour; on a periodic axis the extent is even (checked when the
+// hierarchy is built), so the parity of an unwrapped index is that of the
+// wrapped one
+__device__ __forceinline__ int colour(int gi, int gj) { return (gi + gj) & 1; }
+
+// cx (uL + uR) + cy (uD + uU) at window point k = li + lj*W whose global
+// (unwrapped) coordinates are gi, gj: Dirichlet walls are 0, Neumann
+// boundaries mirror, periodic axes read the window's wrapped copy
+__device__ __forceinline__ float nsum(const float *w, int W, const Lev &L, int k, int gi, int gj)
+{
+    float uL, uR, uD, uU;
+    if (L.tx == GP_AX_PER || gi > 0) uL = w[k - 1]; else uL = (L.tx == GP_AX_NEU) ? w[k + 1] : 0.0f;
+    if (L.tx == GP_AX_PER || gi < L.mx - 1) uR = w[k + 1]; else uR = (L.tx == GP_AX_NEU) ? w[k - 1] : 0.0f;
+    if (L.ty == GP_AX_PER || gj > 0) uD = w[k - W]; else uD = (L.ty == GP_AX_NEU) ? w[k + W] : 0.0f;
+    if (L.ty == GP_AX_PER || gj < L.my - 1) uU = w[k + W]; else uU = (L.ty == GP_AX_NEU) ? w[k - W] : 0.0f;
+    return L.cx * (uL + uR) + L.cy * (uD + uU);
+}
+
+// fine unknown of fine node n (-1 for a Dirichlet wall or outside a Neumann axis)
+__device__ __forceinline__ int f_unk(int t, int n, int M)
+{
+    if (t == GP_AX_DIR) return (n <= 0 || n >= M) ? -1 : n - 1;
+    if (t == GP_AX_PER) return n;          // caller wraps
+    return (n < 0 || n > M) ? -1 : n;
+}
+__device__ __forceinline__ int c_node(int t, int I) { return t == GP_AX_DIR ? I + 1 : I; }
+__device__ __forceinline__ int c_unk(int t, int N, int M)
+{
+    if (t == GP_AX_DIR) return (N <= 0 || N >= M) ? -1 : N - 1;
+    if (t == GP_AX_PER) return wrp(N, M);
+    return N;
+}
+// bilinear prolongation of the coarse correction at fine unknown (i, j)
+__device__ __forceinline__ float prolong(const float *uc, const Lev &F, const Lev &C, int i, int j)
+{
+    const int nx = F.tx == GP_AX_DIR ? i + 1 : i, ny = F.ty == GP_AX_DIR ? j + 1 : j;
+    const int ax0 = nx >> 1, ax1 = (nx + 1) >> 1, ay0 = ny >> 1, ay1 = (ny + 1) >> 1;
+    const int I0 = c_unk(C.tx, ax0, C.Mx), I1 = c_unk(C.tx, ax1, C.Mx);
+    const int J0 = c_unk(C.ty, ay0, C.My), J1 = c_unk(C.ty, ay1, C.My);
+    const float a00 = (I0 >= 0 && J0 >= 0) ? uc[I0 + J0 * C.mx] : 0.0f;
+    const float a10 = (I1 >= 0 && J0 >= 0) ? uc[I1 + J0 * C.mx] : 0.0f;
+    const float a01 = (I0 >= 0 && J1 >= 0) ? uc[I0 + J1 * C.mx] : 0.0f;
+    const float a11 = (I1 >= 0 && J1 >= 0) ? uc[I1 + J1 * C.mx] : 0.0f;
+    // even index: ax0 == ax1 and the two halves add to the coarse value
+    return 0.25f * ((a00 + a10) + (a01 + a11));
+}
+
+// every kernel of an iteration returns at once when the device-side
+// convergence flag is set (the host enqueues iterations in batches)
+__device__ __forceinline__ bool done(const double *scal);
+
+// ---------------------------------------------------------------------------
+// grid level, down leg: u = pre-smoothed from 0; fc = R (f - S u)
+// blockDim (32, 8)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float *fc, Lev L, Lev C, const double *scal)
+{
+    __shared__ float su[WD * WD], sf[WD * WD], sr[(TB + 2) * (TB + 2)];
+    if (done(scal)) return;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int ti0 = blockIdx.x * TB, tj0 = blockIdx.y * TB;
+    const int gi0 = ti0 - HD, gj0 = tj0 - HD;
+    const float invd = 1.0f / (2.0f * (L.cx + L.cy)), d = 2.0f * (L.cx + L.cy);
+    for (int lj = ty; lj < WD; lj += 8) {
+        const int gj = gj0 + lj;
+        const bool vj = in_axis(L.ty, gj, L.my);
+        const int wj = wwrap(gj, L.my);
+        for (int li = tx; li < WD; li += 32) {
+            const int gi = gi0 + li, e = li + lj * WD;
+            const bool v = vj && in_axis(L.tx, gi, L.mx);
+            const float ff = v ? f[wwrap(gi, L.mx) + wj * L.mx] : 0.0f;
+            sf[e] = ff;
+            // red from zero (neighbours are black = 0); black starts at 0
+            su[e] = (v && colour(gi, gj) == 0) ? ff * iweight(L, gi, gj) * invd : 0.0f;
+        }
+    }
+    __syncthreads();
+    // black on the tile + 2
+    for (int lj = 1 + ty; lj < WD - 1; lj += 8) {
+        const int gj = gj0 + lj;
+        if (!in_axis(L.ty, gj, L.my)) continue;
+        for (int li = 1 + tx; li < WD - 1; li += 32) {
+            const int gi = gi0 + li, k = li + lj * WD;
+            if (in_axis(L.tx, gi, L.mx) && colour(gi, gj) == 1)
+                su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WD, L, k, gi, gj)) * invd;
+        }
+    }
+    __syncthreads();
+    // residual on the tile + 1
+    for (int lj = 2 + ty; lj < WD - 2; lj += 8) {
+        const int gj = gj0 + lj;
+        const bool vj = in_axis(L.ty, gj, L.my);
+        for (int li = 2 + tx; li < WD - 2; li += 32) {
+            const int gi = gi0 + li, k = li + lj * WD;
+            float r = 0.0f;
+            if (vj && in_axis(L.tx, gi, L.mx))
+                r = sf[k] - (d * su[k] - nsum(su, WD, L, k, gi, gj)) / iweight(L, gi, gj);
+            sr[(li - 2) + (lj - 2) * (TB + 2)] = r;
+        }
+    }
+    __syncthreads();
+    // pre-smoothed u of the tile
+    for (int lj = ty; lj < TB; lj += 8) {
+        const int gj = tj0 + lj;
+        if (gj >= L.my) break;
+        const int gi = ti0 + tx;
+        if (gi < L.mx) u[gi + gj * L.mx] = su[(HD + tx) + (HD + lj) * WD];
+    }
+    // restriction for the coarse unknowns centred in this tile
+    const int ci0 = ti0 > 0 ? ti0 / 2 - 1 : 0, cj0 = tj0 > 0 ? tj0 / 2 - 1 : 0;
+    constexpr int CW = TB / 2 + 2;
+    for (int cjj = ty; cjj < CW; cjj += 8) {
+        const int J = cj0 + cjj;
+        if (J >= C.my) break;
+        const int nyc = 2 * c_node(C.ty, J);
+        const int fv = f_unk(L.ty, nyc, L.My);
+        if (fv < tj0 || fv >= tj0 + TB) continue;
+        for (int cii = tx; cii < CW; cii += 32) {
+            const int I = ci0 + cii;
+            if (I >= C.mx) continue;
+            const int nxc = 2 * c_node(C.tx, I);
+            const int fu = f_unk(L.tx, nxc, L.Mx);
+            if (fu < ti0 || fu >= ti0 + TB) continue;
+            float acc = 0.0f;
+#pragma unroll
+            for (int b = -1; b <= 1; b++) {
+                if (L.ty != GP_AX_PER && f_unk(L.ty, nyc + b, L.My) < 0) continue;
+                const float wy = b == 0 ? 0.5f : 0.25f;
+#pragma unroll
+                for (int a = -1; a <= 1; a++) {
+                    if (L.tx != GP_AX_PER && f_unk(L.tx, nxc + a, L.Mx) < 0) continue;
+                    acc += (a == 0 ? 0.5f : 0.25f) * wy * sr[(fu + a - ti0 + 1) + (fv + b - tj0 + 1) * (TB + 2)];
+                }
+            }
+            fc[I + J * C.mx] = acc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// grid level, up leg: u = upre + P uc; black; red.  Optional partial of f.u
+// (the CG's r.z at the finest level).  blockDim (32, 8)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mg_up(const float *upre, const float *f, const float *uc, float *u, Lev L, Lev C,
+                                               double *part, const double *scal)
+{
+    __shared__ float su[WU * WU], sf[WU * WU];
+    __shared__ double red[4];
+    if (done(scal)) return;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + 32 * ty;
+    const int ti0 = blockIdx.x * TB, tj0 = blockIdx.y * TB;
+    const int gi0 = ti0 - HU, gj0 = tj0 - HU;
+    const float invd = 1.0f / (2.0f * (L.cx + L.cy));
+    for (int lj = ty; lj < WU; lj += 8) {
+        const int gj = gj0 + lj;
+        const bool vj = in_axis(L.ty, gj, L.my);
+        const int wj = wwrap(gj, L.my);
+        for (int li = tx; li < WU; li += 32) {
+            const int gi = gi0 + li, e = li + lj * WU;
+            float uu = 0.0f, ff = 0.0f;
+            if (vj && in_axis(L.tx, gi, L.mx)) {
+                const int wi = wwrap(gi, L.mx), k = wi + wj * L.mx;
+                uu = upre[k] + prolong(uc, L, C, wi, wj);
+                ff = f[k];
+            }
+            su[e] = uu;
+            sf[e] = ff;
+        }
+    }
+    __syncthreads();
+    for (int lj = 1 + ty; lj < WU - 1; lj += 8) {
+        const int gj = gj0 + lj;
+        if (!in_axis(L.ty, gj, L.my)) continue;
+        for (int li = 1 + tx; li < WU - 1; li += 32) {
+            const int gi = gi0 + li, k = li + lj * WU;
+            if (in_axis(L.tx, gi, L.mx) && colour(gi, gj) == 1)
+                su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WU, L, k, gi, gj)) * invd;
+        }
+    }
+    __syncthreads();
+    for (int lj = HU + ty; lj < HU + TB; lj += 8) {
+        const int gj = gj0 + lj;
+        if (!in_axis(L.ty, gj, L.my)) continue;
+        const int li = HU + tx, gi = gi0 + li, k = li + lj * WU;
+        if (in_axis(L.tx, gi, L.mx) && colour(gi, gj) == 0)
+            su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WU, L, k, gi, gj)) * invd;
+    }
+    __syncthreads();
+    double acc = 0.0;
+    for (int lj = ty; lj < TB; lj += 8) {
+        const int gj = tj0 + lj, gi = ti0 + tx;
+        if (gj < L.my && gi < L.mx) {
+            const int k = (HU + tx) + (HU + lj) * WU;
+            u[gi + gj * L.mx] = su[k];
+            acc += (double)sf[k] * (double)su[k];
+        }
+    }
+    if (part) {
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = acc;
+        __syncthreads();
+        if (tid == 0) part[blockIdx.x + blockIdx.y * gridDim.x] = red[0] + red[1] + red[2] + red[3];
+    }
+}
+
+// one red-black sweep on a grid level (coarsest fallback when a level cannot
+// be coarsened below BOT_MAX): first colour on tile+1, second on the tile
+__global__ __launch_bounds__(256) void k_rb_sweep(const float *uin, const float *f, float *u, Lev L, int first,
+                                                  const double *scal)
+{
+    __shared__ float su[WU * WU], sf[WU * WU];
+    if (done(scal)) return;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int ti0 = blockIdx.x * TB, tj0 = blockIdx.y * TB;
+    const int gi0 = ti0 - HU, gj0 = tj0 - HU;
+    const float invd = 1.0f / (2.0f * (L.cx + L.cy));
+    for (int lj = ty; lj < WU; lj += 8) {
+        const int gj = gj0 + lj;
+        const bool vj = in_axis(L.ty, gj, L.my);
+        const int wj = wwrap(gj, L.my);
+        for (int li = tx; li < WU; li += 32) {
+            const int gi = gi0 + li, e = li + lj * WU;
+            const bool v = vj && in_axis(L.tx, gi, L.mx);
+            const int k = v ? wwrap(gi, L.mx) + wj * L.mx : 0;
+            su[e] = v ? uin[k] : 0.0f;
+            sf[e] = v ? f[k] : 0.0f;
+        }
+    }
+    __syncthreads();
+    for (int lj = 1 + ty; lj < WU - 1; lj += 8) {
+        const int gj = gj0 + lj;
+        if (!in_axis(L.ty, gj, L.my)) continue;
+        for (int li = 1 + tx; li < WU - 1; li += 32) {
+            const int gi = gi0 + li, k = li + lj * WU;
+            if (in_axis(L.tx, gi, L.mx) && colour(gi, gj) == first)
+                su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WU, L, k, gi, gj)) * invd;
+        }
+    }
+    __syncthreads();
+    for (int lj = HU + ty; lj < HU + TB; lj += 8) {
+        const int gj = gj0 + lj;
+        if (!in_axis(L.ty, gj, L.my)) continue;
+        const int li = HU + tx, gi = gi0 + li, k = li + lj * WU;
+        if (in_axis(L.tx, gi, L.mx) && colour(gi, gj) == 1 - first)
+            su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WU, L, k, gi, gj)) * invd;
+    }
+    __syncthreads();
+    for (int lj = ty; lj < TB; lj += 8) {
+        const int gj = tj0 + lj, gi = ti0 + tx;
+        if (gj < L.my && gi < L.mx) u[gi + gj * L.mx] = su[(HU + tx) + (HU + lj) * WU];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// bottom: the whole V-cycle of the small levels inside one workgroup
+// (blockDim 32 x 32), every level resident in LDS
+// ---------------------------------------------------------------------------
+struct Bottom {
+    Lev lv[BOT_LEVELS];
+    int off[BOT_LEVELS];   // LDS offset of level l (u at off, f at off + n)
+    int nl;
+    int singular;
+    int sweeps;            // symmetric sweep pairs on the coarsest level
+};
+
+__device__ __forceinline__ float gs_point(const float *u, const float *f, const Lev &L, int i, int j, float invd)
+{
+    const int k = i + j * L.mx;
+    float uL, uR, uD, uU;
+    uL = i > 0 ? u[k - 1] : (L.tx == GP_AX_PER ? u[k + L.mx - 1] : (L.tx == GP_AX_NEU ? u[k + 1] : 0.0f));
+    uR = i < L.mx - 1 ? u[k + 1] : (L.tx == GP_AX_PER ? u[k - (L.mx - 1)] : (L.tx == GP_AX_NEU ? u[k - 1] : 0.0f));
+    uD = j > 0 ? u[k - L.mx] : (L.ty == GP_AX_PER ? u[k + (L.my - 1) * L.mx] : (L.ty == GP_AX_NEU ? u[k + L.mx] : 0.0f));
+    uU = j < L.my - 1 ? u[k + L.mx] : (L.ty == GP_AX_PER ? u[k - (L.my - 1) * L.mx] : (L.ty == GP_AX_NEU ? u[k - L.mx] : 0.0f));
+    return (f[k] * iweight(L, i, j) + L.cx * (uL + uR) + L.cy * (uD + uU)) * invd;
+}
+// r = w d (g - u) with g the Gauss-Seidel value, w = 1/iweight
+__device__ __forceinline__ float resid_point(const float *u, const float *f, const Lev &L, int i, int j, float invd)
+{
+    const float g = gs_point(u, f, L, i, j, invd);
+    return (g - u[i + j * L.mx]) / (invd * iweight(L, i, j));
+}
+#define BOT_LOOP(L) \
+    for (int j = threadIdx.y; j < (L).my; j += 32) \
+        for (int i = threadIdx.x; i < (L).mx; i += 32)
+
+// every level has a valid red-black colouring (gp_mg_build stops before an
+// odd periodic extent)
+__device__ void bot_sweep(float *u, const float *f, const Lev &L, int color)
+{
+    const float invd = 1.0f / (2.0f * (L.cx + L.cy));
+    BOT_LOOP(L) if (((i + j) & 1) == color) u[i + j * L.mx] = gs_point(u, f, L, i, j, invd);
+    __syncthreads();
+}
+__device__ void bot_project(float *a, const Lev &L, float *scratch)
+{
+    const int tid = threadIdx.x + 32 * threadIdx.y;
+    float s = 0.0f;
+    BOT_LOOP(L) s += a[i + j * L.mx];
+    scratch[tid] = s;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (tid < w) scratch[tid] += scratch[tid + w];
+        __syncthreads();
+    }
+    const float m = scratch[0] / (float)(L.mx * L.my);
+    __syncthreads();
+    BOT_LOOP(L) a[i + j * L.mx] -= m;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uout, Bottom B, double *part,
+                                                    const double *scal)
+{
+    extern __shared__ float lds[];
+    __shared__ float scratch[1024];
+    __shared__ double dred[16];
+    if (done(scal)) return;
+    const int tid = threadIdx.x + 32 * threadIdx.y;
+    {
+        const Lev &L = B.lv[0];
+        const int n = L.mx * L.my;
+        float *f = lds + B.off[0] + n;
+        for (int k = tid; k < n; k += 1024) f[k] = fin[k];
+    }
+    __syncthreads();
+    for (int l = 0; l + 1 < B.nl; l++) {
+        const Lev &L = B.lv[l], &C = B.lv[l + 1];
+        const int n = L.mx * L.my, nc = C.mx * C.my;
+        float *u = lds + B.off[l], *f = u + n;
+        float *fc = lds + B.off[l + 1] + nc;
+        const float invd = 1.0f / (2.0f * (L.cx + L.cy));
+        // red from zero, black zero
+        BOT_LOOP(L) u[i + j * L.mx] = 0.0f;
+        __syncthreads();
+        bot_sweep(u, f, L, 0);
+        bot_sweep(u, f, L, 1);
+        BOT_LOOP(C) {
+            const int cxn = 2 * c_node(C.tx, i), cyn = 2 * c_node(C.ty, j);
+            float acc = 0.0f;
+            for (int b = -1; b <= 1; b++) {
+                int jj = L.ty == GP_AX_PER ? wrp(cyn + b, L.my) : f_unk(L.ty, cyn + b, L.My);
+                if (jj < 0) continue;
+                const float wy = b == 0 ? 0.5f : 0.25f;
+                for (int a = -1; a <= 1; a++) {
+                    int ii = L.tx == GP_AX_PER ? wrp(cxn + a, L.mx) : f_unk(L.tx, cxn + a, L.Mx);
+                    if (ii < 0) continue;
+                    acc += (a == 0 ? 0.5f : 0.25f) * wy * resid_point(u, f, L, ii, jj, invd);
+                }
+            }
+            fc[i + j * C.mx] = acc;
+        }
+        __syncthreads();
+    }
+    {
+        const Lev &L = B.lv[B.nl - 1];
+        const int n = L.mx * L.my;
+        float *u = lds + B.off[B.nl - 1], *f = u + n;
+        if (B.singular) bot_project(f, L, scratch);
+        BOT_LOOP(L) u[i + j * L.mx] = 0.0f;
+        __syncthreads();
+        for (int s = 0; s < B.sweeps; s++) { bot_sweep(u, f, L, 0); bot_sweep(u, f, L, 1); }
+        for (int s = 0; s < B.sweeps; s++) { bot_sweep(u, f, L, 1); bot_sweep(u, f, L, 0); }
+        if (B.singular) bot_project(u, L, scratch);
+    }
+    for (int l = B.nl - 2; l >= 0; l--) {
+        const Lev &L = B.lv[l], &C = B.lv[l + 1];
+        const int n = L.mx * L.my;
+        float *u = lds + B.off[l], *f = u + n;
+        const float *uc = lds + B.off[l + 1];
+        BOT_LOOP(L) u[i + j * L.mx] += prolong(uc, L, C, i, j);
+        __syncthreads();
+        bot_sweep(u, f, L, 1);
+        bot_sweep(u, f, L, 0);
+    }
+    {
+        const Lev &L = B.lv[0];
+        const int n = L.mx * L.my;
+        const float *u = lds + B.off[0], *f = u + n;
+        double acc = 0.0;
+        for (int k = tid; k < n; k += 1024) {
+            uout[k] = u[k];
+            acc += (double)f[k] * (double)u[k];
+        }
+        if (part) {
+            for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+            if ((tid & 63) == 0) dred[tid >> 6] = acc;
+            __syncthreads();
+            if (tid == 0) {
+                double t = 0.0;
+                for (int w = 0; w < 16; w++) t += dred[w];
+                part[0] = t;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CG vector kernels (FP32 data, FP64 accumulation)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < RB / 64; w++) s += sh[w];
+    return s;
+}
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_TMP, S_BB, S_MEAN, S_DONE, S_TGT, S_IT, S_MAXIT, S_NS };
+__device__ __forceinline__ bool done(const double *scal) { return scal && scal[S_DONE] != 0.0; }
+// final reduction of partials into scal[slot]; op 1: alpha = rz / sum; op 2:
+// beta = sum / rz, rz = sum; op 3: residual norm of a CG iteration -- count
+// it and raise the convergence flag at the target or the iteration limit.
+// gate != 0: skip once the flag is up.
+__global__ void k_final(const double *part, int nb, double *scal, int slot, int op, int gate = 0)
+{
+    __shared__ double sh[RB];
+    if (gate && done(scal)) return;
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < nb; k += RB) acc += part[k];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = RB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double s = sh[0];
+        scal[slot] = s;
+        if (op == 1) scal[S_ALPHA] = s != 0.0 ? scal[S_RZ] / s : 0.0;
+        if (op == 2) { scal[S_BETA] = scal[S_RZ] != 0.0 ? s / scal[S_RZ] : 0.0; scal[S_RZ] = s; }
+        if (op == 3) {
+            scal[S_IT] += 1.0;
+            if (!(sqrt(s) > scal[S_TGT]) || scal[S_IT] >= scal[S_MAXIT]) scal[S_DONE] = 1.0;
+        }
+    }
+}
+// start of an inner solve: target = tol * ||r0||, flag up at once if r0 = 0
+__global__ void k_start(double *scal, double inner_tol, int maxit)
+{
+    scal[S_TGT] = inner_tol * sqrt(scal[S_RR]);
+    scal[S_DONE] = scal[S_RR] > 0.0 ? 0.0 : 1.0;
+    scal[S_IT] = 0.0;
+    scal[S_MAXIT] = (double)maxit;
+}
+// p_new = z + beta p_old (beta from scal), q = S p_new, partial p_new.q
+__global__ __launch_bounds__(RB) void k_pupdate_spmv(const float *z, const float *pold, float *pnew, float *q, Lev L,
+                                                    const double *scal, double *part)
+{
+    __shared__ double sh[RB / 64];
+    if (done(scal)) return;
+    const float beta = (float)scal[S_BETA];
+    const int n = L.mx * L.my;
+    double acc = 0.0;
+    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
+        const int i = k % L.mx, j = k / L.mx;
+        auto P = [&](int kk) { return __builtin_fmaf(beta, pold[kk], z[kk]); };
+        const float c = P(k);
+        float uL, uR, uD, uU;
+        uL = i > 0 ? P(k - 1) : (L.tx == GP_AX_PER ? P(k + L.mx - 1) : (L.tx == GP_AX_NEU ? P(k + 1) : 0.0f));
+        uR = i < L.mx - 1 ? P(k + 1) : (L.tx == GP_AX_PER ? P(k - (L.mx - 1)) : (L.tx == GP_AX_NEU ? P(k - 1) : 0.0f));
+        uD = j > 0 ? P(k - L.mx) : (L.ty == GP_AX_PER ? P(k + (L.my - 1) * L.mx) : (L.ty == GP_AX_NEU ? P(k + L.mx) : 0.0f));
+        uU = j < L.my - 1 ? P(k + L.mx) : (L.ty == GP_AX_PER ? P(k - (L.my - 1) * L.mx) : (L.ty == GP_AX_NEU ? P(k - L.mx) : 0.0f));
+        const float w = wgt(L.tx, i, L.mx) * wgt(L.ty, j, L.my);
+        const float v = w * (L.cx * (2.0f * c - uL - uR) + L.cy * (2.0f * c - uD - uU));
+        pnew[k] = c;
+        q[k] = v;
+        acc += (double)c * (double)v;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// e += alpha p; r -= alpha q; partial r.r
+__global__ __launch_bounds__(RB) void k_update_xr(float *x, float *r, const float *p, const float *q, int n,
+                                                 const double *scal, double *part)
+{
+    __shared__ double sh[RB / 64];
+    if (done(scal)) return;
+    const float alpha = (float)scal[S_ALPHA];
+    double acc = 0.0;
+    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
+        x[k] = __builtin_fmaf(alpha, p[k], x[k]);
+        const float rv = __builtin_fmaf(-alpha, q[k], r[k]);
+        r[k] = rv;
+        acc += (double)rv * (double)rv;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ __launch_bounds__(RB) void k_dot(const float *a, const float *b, int n, double *part)
+{
+    __shared__ double sh[RB / 64];
+    double acc = 0.0;
+    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) acc += (double)a[k] * (double)b[k];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ void k_set_scalar(double *scal, int slot, double v) { scal[slot] = v; }
+__global__ void k_zero_f(float *d, int n)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) d[k] = 0.0f;
+}
+
+// FP64 refinement kernels
+__global__ __launch_bounds__(RB) void k_resid64(const double *x, const double *b, double *r, Lev L, double cx,
+                                               double cy, double *part_rr, double *part_bb)
+{
+    __shared__ double sh[RB / 64];
+    const int n = L.mx * L.my;
+    double arr = 0.0, abb = 0.0;
+    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
+        const int i = k % L.mx, j = k / L.mx;
+        const double c = x[k];
+        double xl, xr, yl, yr;
+        if (i > 0) xl = x[k - 1]; else xl = L.tx == GP_AX_PER ? x[k + L.mx - 1] : (L.tx == GP_AX_NEU ? x[k + 1] : 0.0);
+        if (i < L.mx - 1) xr = x[k + 1]; else xr = L.tx == GP_AX_PER ? x[k - (L.mx - 1)] : (L.tx == GP_AX_NEU ? x[k - 1] : 0.0);
+        if (j > 0) yl = x[k - L.mx]; else yl = L.ty == GP_AX_PER ? x[k + (L.my - 1) * L.mx] : (L.ty == GP_AX_NEU ? x[k + L.mx] : 0.0);
+        if (j < L.my - 1) yr = x[k + L.mx]; else yr = L.ty == GP_AX_PER ? x[k - (L.my - 1) * L.mx] : (L.ty == GP_AX_NEU ? x[k - L.mx] : 0.0);
+        const double w = (double)wgt(L.tx, i, L.mx) * (double)wgt(L.ty, j, L.my);
+        const double sx = w * (cx * (2.0 * c - xl - xr) + cy * (2.0 * c - yl - yr));
+        const double rv = b[k] - sx;
+        r[k] = rv;
+        arr += rv * rv;
+        abb += b[k] * b[k];
+    }
+    double s = block_sum(arr, sh);
+    if (threadIdx.x == 0) part_rr[blockIdx.x] = s;
+    __syncthreads();
+    s = block_sum(abb, sh);
+    if (threadIdx.x == 0) part_bb[blockIdx.x] = s;
+}
+__global__ void k_to_f32(const double *r, float *rf, int n, double scale)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) rf[k] = (float)(r[k] * scale);
+}
+__global__ void k_axpy64(double *x, const float *e, int n, double scale)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) x[k] += (double)e[k] * scale;
+}
+__global__ __launch_bounds__(RB) void k_sum64(const double *a, int n, double *part)
+{
+    __shared__ double sh[RB / 64];
+    double acc = 0.0;
+    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) acc += a[k];
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ void k_sub_mean64(double *a, int n, const double *scal, int slot)
+{
+    const double m = scal[slot] / (double)n;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) a[k] -= m;
+}
+
+Lev mklev(const gp_mg_level &l)
+{
+    Lev L;
+    L.mx = l.mx; L.my = l.my;
+    L.tx = l.perx; L.ty = l.pery;
+    L.Mx = L.tx == GP_AX_DIR ? l.mx + 1 : (L.tx == GP_AX_NEU ? l.mx - 1 : l.mx);
+    L.My = L.ty == GP_AX_DIR ? l.my + 1 : (L.ty == GP_AX_NEU ? l.my - 1 : l.my);
+    L.cx = (float)l.hx2i; L.cy = (float)l.hy2i;
+    L.rb = !((L.tx == GP_AX_PER && (l.mx & 1)) || (L.ty == GP_AX_PER && (l.my & 1)));
+    return L;
+}
+inline dim3 tiles(const Lev &L) { return dim3((L.mx + TB - 1) / TB, (L.my + TB - 1) / TB); }
+
+// z = M r (s.lv[0].f aliases r, s.lv[0].u aliases z); part: partial r.z.
+// scal (may be null) carries the convergence flag that makes every kernel a
+// no-op once the CG has converged.
+int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double *scal)
+{
+    const int nl = (int)s.lv.size();
+    const int lb = s.bottom;   // first level handled by k_mg_bottom (nl if none)
+    const dim3 blk(32, 8);
+    for (int l = 0; l < std::min(lb, nl - 1); l++) {
+        const Lev L = mklev(s.lv[l]), C = mklev(s.lv[l + 1]);
+        k_mg_down<<<tiles(L), blk, 0, st>>>(s.lv[l].f, s.lv[l].r, s.lv[l + 1].f, L, C, scal);
+    }
+    GP_HIP(hipGetLastError());
+    *nparts = 0;
+    if (lb < nl) {
+        Bottom B{};
+        int off = 0;
+        B.nl = nl - lb;
+        for (int l = lb; l < nl; l++) {
+            B.lv[l - lb] = mklev(s.lv[l]);
+            B.off[l - lb] = off;
+            off += 2 * s.lv[l].mx * s.lv[l].my;
+        }
+        const Lev &Lc = B.lv[B.nl - 1];
+        B.singular = s.singular;
+        B.sweeps = std::max(2, std::max(Lc.mx, Lc.my) / 2);
+        const bool top = lb == 0;
+        k_mg_bottom<<<1, dim3(32, 32), off * sizeof(float), st>>>(s.lv[lb].f, s.lv[lb].u, B, top ? part : nullptr,
+                                                                  scal);
+        GP_HIP(hipGetLastError());
+        if (top) *nparts = 1;
+    } else {
+        // the coarsest grid level could not be coarsened into the LDS range:
+        // symmetric red-black sweeps with grid kernels, ping-pong buffers
+        // (sweep k reads a, writes b; an even count ends in u)
+        const Lev L = mklev(s.lv[nl - 1]);
+        const int sweeps = std::min(400, std::max(4, 2 * std::max(L.mx, L.my)));
+        float *a = s.lv[nl - 1].u, *b = s.lv[nl - 1].r;
+        k_zero_f<<<256, 256, 0, st>>>(a, L.mx * L.my);
+        for (int k = 0; k < 2 * sweeps; k++) {
+            k_rb_sweep<<<tiles(L), blk, 0, st>>>(a, s.lv[nl - 1].f, b, L, k < sweeps ? 0 : 1, scal);
+            std::swap(a, b);
+        }
+        GP_HIP(hipGetLastError());
+        if (nl == 1) {
+            const int n = L.mx * L.my;
+            k_dot<<<std::min(1024, (n + RB - 1) / RB), RB, 0, st>>>(s.lv[0].f, s.lv[0].u, n, part);
+            *nparts = std::min(1024, (n + RB - 1) / RB);
+        }
+    }
+    for (int l = std::min(lb, nl - 1) - 1; l >= 0; l--) {
+        const Lev L = mklev(s.lv[l]), C = mklev(s.lv[l + 1]);
+        const dim3 g = tiles(L);
+        k_mg_up<<<g, blk, 0, st>>>(s.lv[l].r, s.lv[l].f, s.lv[l + 1].u, s.lv[l].u, L, C, l == 0 ? part : nullptr,
+                                   scal);
+        if (l == 0) *nparts = (int)(g.x * g.y);
+    }
+    GP_HIP(hipGetLastError());
+    return GP_OK;
+}
+
+// FP32 PCG on S e = r (s.r holds r and is overwritten); e in s.e.  The
+// iterations are enqueued in batches; a device-side flag (set by the
+// residual reduction at the target or the iteration limit) turns the rest of
+// a batch into no-ops, so the host synchronises once per batch.
+int pcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int *iters)
+{
+    const Lev L0 = mklev(s.lv[0]);
+    const int n = s.n, nb = s.red_blocks;
+    const int ge = std::min(1024, gp_cdiv(n, 256));
+    float *p = s.p, *p2 = s.q2;
+    k_zero_f<<<ge, 256, 0, st>>>(s.e, n);
+    k_zero_f<<<ge, 256, 0, st>>>(p, n);
+    k_set_scalar<<<1, 1, 0, st>>>(s.scal, S_BETA, 0.0);
+    k_set_scalar<<<1, 1, 0, st>>>(s.scal, S_DONE, 0.0);
+    double *red_pq = s.red, *red_rr = s.red + 1024, *red_rz = s.red + 2048;
+    int np = 0;
+    int rc = vcycle(st, s, red_rz, &np, nullptr);          // z = M r, partial r.z
+    if (rc) return rc;
+    k_final<<<1, RB, 0, st>>>(red_rz, np, s.scal, S_RZ, 0);
+    k_dot<<<nb, RB, 0, st>>>(s.r, s.r, n, red_rr);
+    k_final<<<1, RB, 0, st>>>(red_rr, nb, s.scal, S_RR, 0);
+    k_start<<<1, 1, 0, st>>>(s.scal, inner_tol, maxit);
+    GP_HIP(hipGetLastError());
+    int enq = 0;
+    // first batch: the previous solve's iteration count (one more if it
+    // ran short), then single iterations
+    for (int batch = std::max(2, s.last_it);; batch = 1) {
+        for (int b = 0; b < batch && enq < maxit; b++, enq++) {
+            k_pupdate_spmv<<<nb, RB, 0, st>>>(s.z, p, p2, s.q, L0, s.scal, red_pq);
+            std::swap(p, p2);
+            k_final<<<1, RB, 0, st>>>(red_pq, nb, s.scal, S_PQ, 1, 1);
+            k_update_xr<<<nb, RB, 0, st>>>(s.e, s.r, p, s.q, n, s.scal, red_rr);
+            k_final<<<1, RB, 0, st>>>(red_rr, nb, s.scal, S_RR, 3, 1);
+            rc = vcycle(st, s, red_rz, &np, s.scal);
+            if (rc) return rc;
+            k_final<<<1, RB, 0, st>>>(red_rz, np, s.scal, S_TMP, 2, 1);
+        }
+        GP_HIP(hipGetLastError());
+        GP_HIP(hipMemcpyAsync(s.hscal, s.scal, S_NS * sizeof(double), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipStreamSynchronize(st));
+        if (!(s.hscal[S_RR] == s.hscal[S_RR])) return gp_fail(GP_ERR_NONFINITE, "non-finite residual in the elliptic solve");
+        if (s.hscal[S_DONE] != 0.0 || enq >= maxit) break;
+    }
+    *iters = (int)s.hscal[S_IT];
+    s.last_it = *iters + (*iters >= enq ? 1 : 0);
+    return GP_OK;
+}
+
+}  // namespace
+
+int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double cy)
+{
+    s.lv.clear();
+    gp_mg_level l;
+    l.mx = mx; l.my = my; l.perx = tx; l.pery = ty; l.hx2i = cx; l.hy2i = cy;
+    s.lv.push_back(l);
+    auto M = [](int t, int m) { return t == GP_AX_DIR ? m + 1 : (t == GP_AX_NEU ? m - 1 : m); };
+    for (;;) {
+        const gp_mg_level &f = s.lv.back();
+        if ((int64_t)f.mx * f.my <= 16) break;
+        const int Mx = M(tx, f.mx), My = M(ty, f.my);
+        if ((Mx & 1) || (My & 1) || Mx < 4 || My < 4) break;
+        if ((int)s.lv.size() >= 24) break;
+        gp_mg_level c;
+        const int Mxc = Mx / 2, Myc = My / 2;
+        if ((tx == GP_AX_PER && (Mxc & 1)) || (ty == GP_AX_PER && (Myc & 1))) break;   // keep red-black valid
+        c.mx = tx == GP_AX_DIR ? Mxc - 1 : (tx == GP_AX_NEU ? Mxc + 1 : Mxc);
+        c.my = ty == GP_AX_DIR ? Myc - 1 : (ty == GP_AX_NEU ? Myc + 1 : Myc);
+        c.perx = tx; c.pery = ty;
+        c.hx2i = f.hx2i / 4.0; c.hy2i = f.hy2i / 4.0;
+        s.lv.push_back(c);
+    }
+    // the first level small enough for the LDS bottom solver
+    const int nl = (int)s.lv.size();
+    s.bottom = nl;
+    for (int k = 0; k < nl; k++)
+        if ((int64_t)s.lv[k].mx * s.lv[k].my <= BOT_MAX) { s.bottom = k; break; }
+    if (s.bottom < nl && nl - s.bottom > BOT_LEVELS) return gp_fail(GP_ERR_UNSUPPORTED, "too many multigrid levels");
+    const int n = mx * my;
+    s.n = n;
+    if (!mklev(s.lv[0]).rb)
+        return gp_fail(GP_ERR_UNSUPPORTED, "a periodic axis needs an even number of cells (red-black smoothing)");
+    if ((int64_t)s.lv[nl - 1].mx * s.lv[nl - 1].my > BOT_MAX && nl - 1 < s.bottom)
+        ;   // coarsest level solved by grid-kernel sweeps (slow, but valid)
+    for (int k = 0; k < nl; k++) {
+        const size_t m = (size_t)s.lv[k].mx * s.lv[k].my;
+        if (k > 0) {
+            GP_HIP(hipMalloc((void **)&s.lv[k].u, m * sizeof(float)));
+            GP_HIP(hipMalloc((void **)&s.lv[k].f, m * sizeof(float)));
+        }
+        GP_HIP(hipMalloc((void **)&s.lv[k].r, m * sizeof(float)));
+    }
+    GP_HIP(hipMalloc((void **)&s.b64, n * sizeof(double)));
+    GP_HIP(hipMalloc((void **)&s.x64, n * sizeof(double)));
+    GP_HIP(hipMalloc((void **)&s.r64, n * sizeof(double)));
+    GP_HIP(hipMemset(s.x64, 0, n * sizeof(double)));
+    GP_HIP(hipMalloc((void **)&s.r, n * sizeof(float)));
+    GP_HIP(hipMalloc((void **)&s.z, n * sizeof(float)));
+    GP_HIP(hipMalloc((void **)&s.p, n * sizeof(float)));
+    GP_HIP(hipMalloc((void **)&s.q, n * sizeof(float)));
+    GP_HIP(hipMalloc((void **)&s.q2, n * sizeof(float)));
+    GP_HIP(hipMalloc((void **)&s.e, n * sizeof(float)));
+    s.lv[0].f = s.r;     // the finest level's rhs is the CG residual
+    s.lv[0].u = s.z;     // and its output the preconditioned residual
+    s.red_blocks = std::min(1024, gp_cdiv(n, RB));
+    const int ntile0 = ((mx + TB - 1) / TB) * ((my + TB - 1) / TB);
+    GP_HIP(hipMalloc((void **)&s.red, (2048 + std::max(ntile0, 1024) + 64) * sizeof(double)));
+    GP_HIP(hipMalloc((void **)&s.scal, S_NS * sizeof(double)));
+    GP_HIP(hipHostMalloc((void **)&s.hscal, S_NS * sizeof(double), hipHostMallocDefault));
+    return GP_OK;
+}
+
+void gp_mg_free(gp_solver &s)
+{
+    for (size_t k = 0; k < s.lv.size(); k++) {
+        if (k > 0) { (void)hipFree(s.lv[k].u); (void)hipFree(s.lv[k].f); }
+        (void)hipFree(s.lv[k].r);
+    }
+    s.lv.clear();
+    void *ps[] = {s.b64, s.x64, s.r64, s.rhs_bc, s.r, s.z, s.p, s.q, s.q2, s.e, s.red, s.scal};
+    for (void *p : ps) if (p) (void)hipFree(p);
+    if (s.hscal) (void)hipHostFree(s.hscal);
+    s = gp_solver();
+}
+
+// Solve S x = b64 (already in s.b64) starting from s.x64, FP64 refinement.
+// precorrect, when given, supplies the first correction (a learned warm
+// start) in place of MG-PCG; the FP64 loop then continues with MG-PCG.
+int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int singular, gp_solve_stats *stats,
+                 int (*precorrect)(void *, float *, const float *), void *pc_ctx)
+{
+    s.singular = singular;
+    const Lev L0 = mklev(s.lv[0]);
+    const int n = s.n, nb = s.red_blocks;
+    const int ge = std::min(1024, gp_cdiv(n, 256));
+    const double cx = s.lv[0].hx2i, cy = s.lv[0].hy2i;
+    auto t0 = std::chrono::steady_clock::now();
+    if (singular) {
+        k_sum64<<<nb, RB, 0, st>>>(s.b64, n, s.red);
+        k_final<<<1, RB, 0, st>>>(s.red, nb, s.scal, S_MEAN, 0);
+        k_sub_mean64<<<ge, 256, 0, st>>>(s.b64, n, s.scal, S_MEAN);
+        GP_HIP(hipGetLastError());
+    }
+    int total_it = 0, outer = 0;
+    double rel = 0.0;
+    int status = GP_OK;
+    for (;;) {
+        k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb);
+        k_final<<<1, RB, 0, st>>>(s.red, nb, s.scal, S_RR, 0);
+        k_final<<<1, RB, 0, st>>>(s.red + nb, nb, s.scal, S_BB, 0);
+        GP_HIP(hipGetLastError());
+        GP_HIP(hipMemcpyAsync(s.hscal, s.scal, S_NS * sizeof(double), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipStreamSynchronize(st));
+        const double rr = s.hscal[S_RR], bb = s.hscal[S_BB];
+        rel = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+        if (!(rel == rel)) return gp_fail(GP_ERR_NONFINITE, "non-finite residual in the FP64 refinement");
+        if (rel <= tol || bb == 0.0) break;
+        if (total_it >= maxiter || outer >= 20) { status = GP_NOT_CONVERGED; break; }
+        const double rms = std::sqrt(rr / n);
+        const double sc = rms > 0 ? 1.0 / rms : 1.0;
+        k_to_f32<<<ge, 256, 0, st>>>(s.r64, s.r, n, sc);
+        GP_HIP(hipGetLastError());
+        if (precorrect) {
+            int rc = precorrect(pc_ctx, s.e, s.r);
+            if (rc) return rc;
+            k_axpy64<<<ge, 256, 0, st>>>(s.x64, s.e, n, 1.0 / sc);
+            GP_HIP(hipGetLastError());
+            precorrect = nullptr;
+            outer++;
+            continue;
+        }
+        // ask the FP32 solve for just the reduction still needed (FP32 cannot
+        // usefully go below ~1e-6 of its own starting residual)
+        const double inner = std::max(2e-6, std::min(1e-2, 0.3 * tol / rel));
+        int it = 0;
+        int rc = pcg32(st, s, inner, std::max(1, maxiter - total_it), &it);
+        if (rc) return rc;
+        total_it += it;
+        k_axpy64<<<ge, 256, 0, st>>>(s.x64, s.e, n, 1.0 / sc);
+        GP_HIP(hipGetLastError());
+        outer++;
+    }
+    if (singular) {
+        k_sum64<<<nb, RB, 0, st>>>(s.x64, n, s.red);
+        k_final<<<1, RB, 0, st>>>(s.red, nb, s.scal, S_MEAN, 0);
+        k_sub_mean64<<<ge, 256, 0, st>>>(s.x64, n, s.scal, S_MEAN);
+        GP_HIP(hipGetLastError());
+    }
+    GP_HIP(hipStreamSynchronize(st));
+    auto t1 = std::chrono::steady_clock::now();
+    if (stats) {
+        stats->iterations = total_it;
+        stats->refinements = outer;
+        stats->relres = rel;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+    }
+    return status;
+}
+
+// a -= plain mean of a (n = the solver's unknown count)
+int gp_mg_sub_mean(hipStream_t st, gp_solver &s, double *a)
+{
+    const int n = s.n, nb = s.red_blocks;
+    k_sum64<<<nb, RB, 0, st>>>(a, n, s.red);
+    k_final<<<1, RB, 0, st>>>(s.red, nb, s.scal, S_MEAN, 0);
+    k_sub_mean64<<<std::min(1024, gp_cdiv(n, 256)), 256, 0, st>>>(a, n, s.scal, S_MEAN);
+    GP_HIP(hipGetLastError());
+    return GP_OK;
+}
+
+// apply one V-cycle to an FP32 vector (used by the learned preconditioner's
+// flexible CG): out = M in
+int gp_mg_apply(hipStream_t st, gp_solver &s, const float *in, float *out)
+{
+    const int n = s.n;
+    GP_HIP(hipMemcpyAsync(s.r, in, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    int np = 0;
+    int rc = vcycle(st, s, s.red + 2048, &np, nullptr);
+    if (rc) return rc;
+    GP_HIP(hipMemcpyAsync(out, s.z, n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return GP_OK;
+}

@@ -91,7 +91,10 @@ __device__ __forceinline__ long long fx40(float c)
     const unsigned long long u = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
     return r < 0.0f ? -(long long)u : (long long)u;
 }
-__device__ __forceinline__ unsigned q23(float s) { return (unsigned)__builtin_rintf(s * 8388608.0f); }
+// per-axis weight in [0, 1] as an unsigned with 23 fractional bits; the value
+// fits 24 bits, and the mask lets the compiler use the full-rate 24-bit
+// multiplies (v_mul_u32_u24 / v_mul_hi_u32_u24) for the 46-bit product
+__device__ __forceinline__ unsigned q23(float s) { return (unsigned)__builtin_rintf(s * 8388608.0f) & 0xFFFFFFu; }
 
 // periodic wrap x = fmod(x + (ng-ngb), ng) + ngb (Particle.mm:216-221); the
 // two conditional subtractions are the exact remainder for 0 <= a < 3 ng.
@@ -178,10 +181,21 @@ __global__ __launch_bounds__(GP_BLOCK) void k_push(PushArgs a)
     const float loy = (float)a.ngb, hiy = (float)(a.ngb + a.ngy);
     double ke = 0.0;
 
+    // one particle ahead: the next particle's loads are in flight while the
+    // current one is processed
+    float nx_ = 0.f, ny_ = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
+    if (p0 + tid < p1) {
+        const int64_t q = p0 + tid;
+        nx_ = a.x[q]; ny_ = a.y[q]; nvx = a.vx[q]; nvy = a.vy[q]; nvz = a.vz[q];
+    }
     for (int64_t p = p0 + tid; p < p1; p += GP_BLOCK) {
-        float x = a.x[p], y = a.y[p];
-        float vx = a.vx[p], vy = a.vy[p];
-        const float vz = a.vz[p];
+        float x = nx_, y = ny_;
+        float vx = nvx, vy = nvy;
+        const float vz = nvz;
+        if (p + GP_BLOCK < p1) {
+            const int64_t q = p + GP_BLOCK;
+            nx_ = a.x[q]; ny_ = a.y[q]; nvx = a.vx[q]; nvy = a.vy[q]; nvz = a.vz[q];
+        }
         // live particles lie in [ngb, ngb+ng); anything else (absorbed
         // marker, escaped, NaN) is left untouched and not deposited
         if (!(x >= lox && x < hix && y >= loy && y < hiy)) {

@@ -124,6 +124,9 @@ enum {
     GP_F_DEL_EY,
     GP_F_DEL_BZ,
     GP_F_SOL,        /* double Poisson unknowns (nkx+1)(nky+1) */
+    GP_F_CHI,        /* double Darwin potentials on the node grid (nkxv+1)(nkyv+1), */
+    GP_F_AX,         /*   nkxv = ngx - [periodic x] (EMField.mm:486-495) */
+    GP_F_AY,
     GP_F_COUNT
 };
 /* per-species accumulator identifiers for gp_species_grid_get */
