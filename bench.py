@@ -129,15 +129,29 @@ def cpu_baseline(args, P):
         return out
 
     sp = one_step(sp)                     # warm-up
+    steps = 0
     t0 = time.perf_counter()
-    for _ in range(args.cpu_steps):
+    while steps < args.cpu_steps or time.perf_counter() - t0 < args.cpu_seconds:
         sp = one_step(sp)
+        steps += 1
     t1 = time.perf_counter()
-    rate = 2 * n * args.cpu_steps / (t1 - t0)
+    rate = 2 * n * steps / (t1 - t0)
     return dict(value=rate, unit="particle-steps/s", cores=threads, kind="port",
                 sample=f"oracle push_fast + deposit_fx (C/OpenMP, -O2 x86-64-v3) of 2 x {n:,} particles on the "
-                       f"{ngx}x{ngx} 3rd-order grid, {args.cpu_steps} timed steps after 1 warm-up; Poisson not "
-                       f"included (CPU MG is not built)")
+                       f"{ngx}x{ngx} 3rd-order grid, {steps} timed steps ({t1 - t0:.1f} s) after 1 warm-up; "
+                       f"Poisson not included")
+
+
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size: reads and writes mib MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copy calibration: {mib} MiB read + {mib} MiB written per launch", file=sys.stderr, flush=True)
 
 
 def traffic_from_profiles():
@@ -158,15 +172,21 @@ def main():
     ap.add_argument("--ngx", type=int, default=1024)
     ap.add_argument("--ppc", type=int, default=128)
     ap.add_argument("--order", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (3 times), the reference kernel for scaling "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a --pmc run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU (particle-sharded, int64 "
                                    "all-reduce of the charge grid) design is described in DESIGN.md",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_calib:
+        copy_calibration(args.copy_calib)
     r = gpu_run(args)
     ms_step = 1e3 * r["seconds"] / args.steps
     value = r["n_total"] * args.steps / r["seconds"]

@@ -19,7 +19,8 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgample_amd.so")
+# GAMPLE_AMD_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("GAMPLE_AMD_LIB") or os.path.join(_HERE, "libgample_amd.so")
 
 GP_OK = 0
 GP_NOT_CONVERGED = 1

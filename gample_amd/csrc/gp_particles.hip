@@ -23,6 +23,12 @@
 namespace {
 
 constexpr int WIN = GP_TILE + 2 * GP_DRIFT;   // cell window of a tile
+// waves per SIMD the push kernel is register-budgeted for
+#ifdef PUSH_WAVES
+#define PUSH_OCC __attribute__((amdgpu_waves_per_eu(PUSH_WAVES, PUSH_WAVES)))
+#else
+#define PUSH_OCC
+#endif
 
 // ---------------------------------------------------------------------------
 // B-spline weights.  Horner forms of the reference polynomials
@@ -91,10 +97,17 @@ __device__ __forceinline__ long long fx40(float c)
     const unsigned long long u = ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
     return r < 0.0f ? -(long long)u : (long long)u;
 }
-// per-axis weight in [0, 1] as an unsigned with 23 fractional bits; the value
-// fits 24 bits, and the mask lets the compiler use the full-rate 24-bit
-// multiplies (v_mul_u32_u24 / v_mul_hi_u32_u24) for the 46-bit product
-__device__ __forceinline__ unsigned q23(float s) { return (unsigned)__builtin_rintf(s * 8388608.0f) & 0xFFFFFFu; }
+// per-axis weight s in [0, 1] as an unsigned with 23 fractional bits,
+// rint(s * 2^23).  One fma: s * 2^23 + 2^23 lies in [2^23, 2^24], where the
+// float spacing is 1, so the single rounding of the fma is the round-to-
+// nearest-even of s * 2^23, and the low 24 bits of the sum's encoding are that
+// integer (2^24 itself encodes as 0x4B800000, whose low bits give 2^23).  The
+// 24-bit value lets the compiler use the full-rate 24-bit multiplies
+// (v_mul_u32_u24 / v_mul_hi_u32_u24) for the 46-bit product.
+__device__ __forceinline__ unsigned q23(float s)
+{
+    return __float_as_uint(__builtin_fmaf(s, 8388608.0f, 8388608.0f)) & 0xFFFFFFu;
+}
 
 // periodic wrap x = fmod(x + (ng-ngb), ng) + ngb (Particle.mm:216-221); the
 // two conditional subtractions are the exact remainder for 0 <= a < 3 ng.
@@ -130,7 +143,7 @@ struct PushArgs {
 };
 
 template <int ORDER, bool UNIB, int DEP, bool PUSH>
-__global__ __launch_bounds__(GP_BLOCK) void k_push(PushArgs a)
+__global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
 {
     constexpr int NGB = ORDER / 2, NW = ORDER + 1;
     constexpr int W = WIN + 2 * NGB + 1;              // node window edge
@@ -181,20 +194,32 @@ __global__ __launch_bounds__(GP_BLOCK) void k_push(PushArgs a)
     const float loy = (float)a.ngb, hiy = (float)(a.ngb + a.ngy);
     double ke = 0.0;
 
+    // The segment is walked in sub-segments of at most 2^28 particles, so that
+    // a particle's address is a uniform base plus a 32-bit byte offset (the
+    // scalar-base + vector-offset form of the global loads and stores).
+    constexpr int64_t SUB = int64_t(1) << 28;
+    for (int64_t s0 = p0; s0 < p1; s0 += SUB) {
+    const uint32_t cnt = (uint32_t)(p1 - s0 < SUB ? p1 - s0 : SUB);
+    float *const bx = a.x + s0, *const by = a.y + s0, *const bvx = a.vx + s0, *const bvy = a.vy + s0;
+    const float *const bvz = a.vz + s0;
+    auto ld = [](const float *b, uint32_t o) { return *(const float *)((const char *)b + o); };
+    auto st = [](float *b, uint32_t o, float v) { *(float *)((char *)b + o) = v; };
     // one particle ahead: the next particle's loads are in flight while the
     // current one is processed
     float nx_ = 0.f, ny_ = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
-    if (p0 + tid < p1) {
-        const int64_t q = p0 + tid;
-        nx_ = a.x[q]; ny_ = a.y[q]; nvx = a.vx[q]; nvy = a.vy[q]; nvz = a.vz[q];
+    if ((uint32_t)tid < cnt) {
+        const uint32_t o = 4u * tid;
+        nx_ = ld(bx, o); ny_ = ld(by, o); nvx = ld(bvx, o); nvy = ld(bvy, o); nvz = ld(bvz, o);
     }
-    for (int64_t p = p0 + tid; p < p1; p += GP_BLOCK) {
+    for (uint32_t k = tid; k < cnt; k += GP_BLOCK) {
+        const int64_t p = s0 + k;
+        const uint32_t ob = 4u * k;
         float x = nx_, y = ny_;
         float vx = nvx, vy = nvy;
         const float vz = nvz;
-        if (p + GP_BLOCK < p1) {
-            const int64_t q = p + GP_BLOCK;
-            nx_ = a.x[q]; ny_ = a.y[q]; nvx = a.vx[q]; nvy = a.vy[q]; nvz = a.vz[q];
+        if (k + GP_BLOCK < cnt) {
+            const uint32_t o = ob + 4u * GP_BLOCK;
+            nx_ = ld(bx, o); ny_ = ld(by, o); nvx = ld(bvx, o); nvy = ld(bvy, o); nvz = ld(bvz, o);
         }
         // live particles lie in [ngb, ngb+ng); anything else (absorbed
         // marker, escaped, NaN) is left untouched and not deposited
@@ -261,10 +286,10 @@ __global__ __launch_bounds__(GP_BLOCK) void k_push(PushArgs a)
                 if (x >= hix) x = __int_as_float(__float_as_int(hix) - 1);
                 if (y >= hiy) y = __int_as_float(__float_as_int(hiy) - 1);
             }
-            a.x[p] = flag ? -(float)flag : x;
-            a.y[p] = y;
-            a.vx[p] = vx;
-            a.vy[p] = vy;
+            st(bx, ob, flag ? -(float)flag : x);
+            st(by, ob, y);
+            st(bvx, ob, vx);
+            st(bvy, ob, vy);
         }
         if (flag) {
             atomicAdd(&a.counters[flag], 1ull);
@@ -334,6 +359,7 @@ __global__ __launch_bounds__(GP_BLOCK) void k_push(PushArgs a)
                     }
             }
         }
+    }
     }
 
     if constexpr (PUSH) {

@@ -244,6 +244,8 @@ def _poisson_case(gp, orc, order, fbc, ngx, ngy, seed=0, tol=1e-8):
     (3, ("Dirichlet",) * 4, 64, 32),
     (3, ("Dirichlet", "Dirichlet", "periodic", "periodic"), 512, 512),
     (1, ("periodic",) * 4, 256, 256),
+    (3, ("Dirichlet",) * 4, 96, 80),        # intervals not powers of two
+    (1, ("periodic",) * 4, 16, 512),        # grid level narrower than a tile window
 ])
 def test_poisson_against_oracle(gp, orc, order, fbc, ngx, ngy):
     say(f"[poisson] order={order} fbc={fbc} {ngx}x{ngy}")
