@@ -91,9 +91,54 @@ def gpu_run(args):
     ctx.timing(False)
     _, dg, _ = ctx.solve_fields(dt, tol=tol, diag=True)
     nn = ctx.count(se) + ctx.count(si)
+    nnrep = None if args.no_nn else nn_poisson(ctx, args, dt, tol, cyc)
     ctx.close()
     return dict(n_total=nn, seconds=t1 - t0, push_ms=push_ms, push_n=push_n, sort_ms=sort_ms, sort_n=sort_n,
-                solve_ms=solve_ms, solve_n=solve_n, iters=iters, P=P, n_species=n)
+                solve_ms=solve_ms, solve_n=solve_n, iters=iters, P=P, n_species=n, nn=nnrep)
+
+
+def nn_poisson(ctx, args, dt, tol, cyc):
+    """Learned Poisson warm start / preconditioner on the workload's own grid:
+    train on (b, x) pairs from further steps of the run plus random power-law
+    right-hand sides, then solve held-out fields of the run cold (x0 = 0) with
+    MG-CG and with each network mode, and warm (previous solution) with
+    MG-CG as the time loop does."""
+    from gample_amd.nn_data import add_random_pairs
+    t0 = time.perf_counter()
+    ctx.nn_init(32, seed=1)
+    for _ in range(args.nn_sim_pairs):
+        cyc += 1
+        ctx.step(dt, tol=tol, maxiter=400, sort_interval=20, cycle=cyc)
+        ctx.nn_add_pair()
+    add_random_pairs(ctx, args.nn_random_pairs, np.random.default_rng(7), True, True, tol=tol)
+    t1 = time.perf_counter()
+    loss0 = ctx.nn_train(epochs=1, batch=4, lr=0.0)
+    epochs, loss = 0, loss0
+    while epochs < args.nn_max_epochs and time.perf_counter() - t1 < args.nn_train_seconds:
+        loss = ctx.nn_train(epochs=1, batch=4, lr=1e-3)
+        epochs += 1
+    t2 = time.perf_counter()
+    import gample_amd as gp
+    rows = {k: [] for k in ("mg_cold", "mg_warm", "nn_warm_fcg", "nn_warm_mg", "nn_fcg")}
+    for _ in range(args.nn_eval_fields):
+        xprev = ctx.field(gp.F_SOL).ravel().copy()
+        cyc += 1
+        ctx.step(dt, tol=tol, maxiter=400, sort_interval=20, cycle=cyc)
+        b = ctx.field(gp.F_RHS).ravel().copy()
+        for key, solver, x0 in (("mg_cold", 0, None), ("mg_warm", 0, xprev), ("nn_warm_fcg", 1, None),
+                                ("nn_warm_mg", 2, None), ("nn_fcg", 3, None)):
+            _, st, _ = ctx.solve_rhs(b, x0=x0, tol=tol, maxiter=400, solver=solver)
+            rows[key].append((st.iterations, 1e3 * st.seconds, st.relres, st.fell_back))
+    out = {"train_pairs": args.nn_sim_pairs + args.nn_random_pairs, "harvest_s": t1 - t0, "train_s": t2 - t1,
+           "epochs": epochs, "loss_untrained": loss0, "loss_trained": loss,
+           "network": "4-level conv encoder-decoder 32-64-128-128, 3x3, BF16 MFMA / FP32 accumulate",
+           "fields": args.nn_eval_fields, "tol": tol}
+    for k, v in rows.items():
+        a = np.array(v, dtype=float)
+        out[k] = {"iterations": float(a[:, 0].mean()), "ms": float(a[:, 1].mean()),
+                  "max_relres": float(a[:, 2].max()), "fell_back": int(a[:, 3].sum())}
+    out["nn_warm_fcg_over_mg_cold_time"] = out["nn_warm_fcg"]["ms"] / out["mg_cold"]["ms"]
+    return out
 
 
 def cpu_baseline(args, P):
@@ -167,8 +212,8 @@ def traffic_from_profiles():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--ngx", type=int, default=1024)
     ap.add_argument("--ppc", type=int, default=128)
     ap.add_argument("--order", type=int, default=3)
@@ -176,6 +221,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-nn", action="store_true", help="skip the learned Poisson preconditioner report")
+    ap.add_argument("--nn-sim-pairs", type=int, default=16)
+    ap.add_argument("--nn-random-pairs", type=int, default=48)
+    ap.add_argument("--nn-train-seconds", type=float, default=30.0)
+    ap.add_argument("--nn-max-epochs", type=int, default=60)
+    ap.add_argument("--nn-eval-fields", type=int, default=3)
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (3 times), the reference kernel for scaling "
                          "the FETCH_SIZE / WRITE_SIZE counters of a --pmc run")
@@ -212,7 +263,7 @@ def main():
         "breakdown_ms_per_step": {"push_deposit": r["push_ms"] / args.steps, "sort": r["sort_ms"] / args.steps,
                                   "field_solve": r["solve_ms"] / args.steps},
         "poisson": {"mg_cg_iterations_mean": float(np.mean(r["iters"])), "ms_per_solve":
-                    r["solve_ms"] / max(1, r["solve_n"]), "nn_pcg": None},
+                    r["solve_ms"] / max(1, r["solve_n"]), "nn_pcg": r["nn"]},
     }
     if not args.no_cpu:
         cb = cpu_baseline(args, r["P"])

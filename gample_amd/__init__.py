@@ -28,7 +28,9 @@ PBC = {"none": -1, "periodic": 0, "Delete": 1, "Reflect": 2}
 FBC = {"periodic": 0, "Dirichlet": 1, "Neumann": 2}
 
 (F_RHO, F_EX, F_EY, F_EZ, F_BX, F_BY, F_BZ, F_PHI, F_JX, F_JY, F_DEL_EX, F_DEL_EY, F_DEL_BZ, F_SOL, F_CHI, F_AX,
- F_AY) = range(17)
+ F_AY, F_RHS) = range(18)
+# elliptic solver choices of solve_fields / solve_darwin / step
+SOLVER_MG, SOLVER_NN, SOLVER_NN_WARM_MG, SOLVER_NN_FCG = 0, 1, 2, 3
 G_RHO_ACC, G_JX_ACC, G_JY_ACC = range(3)
 
 
@@ -87,7 +89,8 @@ EXPORTS = [
     "gp_species_grid_get", "gp_load_pair", "gp_inject_wall", "gp_reset_grids", "gp_deposit",
     "gp_solve_fields", "gp_solve_darwin", "gp_push", "gp_reduce", "gp_sort", "gp_moments",
     "gp_step", "gp_field_get", "gp_field_set", "gp_field_energy", "gp_timing", "gp_kernel_time",
-    "gp_nn_init", "gp_nn_train", "gp_nn_infer", "gp_nn_save", "gp_nn_load",
+    "gp_nn_init", "gp_nn_train", "gp_nn_infer", "gp_nn_save", "gp_nn_load", "gp_nn_add_pair", "gp_nn_params",
+    "gp_nn_dims", "gp_nn_forward", "gp_nn_grad", "gp_solve_rhs", "gp_nn_config",
 ]
 
 _lib: Optional[C.CDLL] = None
@@ -134,6 +137,13 @@ def lib() -> C.CDLL:
         L.gp_nn_infer.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.gp_nn_save.argtypes = [P, C.c_char_p]
         L.gp_nn_load.argtypes = [P, C.c_char_p]
+        L.gp_nn_add_pair.argtypes = [P]
+        L.gp_nn_config.argtypes = [P, I, I, I]
+        L.gp_nn_params.argtypes = [P, fp, fp, C.POINTER(C.c_int64)]
+        L.gp_nn_dims.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.gp_nn_forward.argtypes = [P, fp, I, fp]
+        L.gp_nn_grad.argtypes = [P, fp, fp, I, I, fp, fp]
+        L.gp_solve_rhs.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), D, I, I, C.POINTER(SolveStats)]
         _lib = L
     return _lib
 
@@ -246,6 +256,16 @@ class Context:
                                           C.byref(dg) if dg is not None else None), allow_nc=True)
         return (st, dg, rc) if diag else (st, rc)
 
+    def solve_rhs(self, b, x0=None, tol=1e-8, maxiter=400, solver=0):
+        """S x = b on the Poisson grid for a given b; returns (x, stats, rc)"""
+        b = np.ascontiguousarray(b, np.float64).ravel()
+        x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64).ravel()
+        st = SolveStats()
+        rc = _check(lib().gp_solve_rhs(self.h, b.ctypes.data_as(C.POINTER(C.c_double)),
+                                       x.ctypes.data_as(C.POINTER(C.c_double)), tol, maxiter, solver, C.byref(st)),
+                    allow_nc=True)
+        return x, st, rc
+
     def solve_darwin(self, dt, tol=1e-8, maxiter=400, solver=0):
         st = (SolveStats * 3)()
         rc = _check(lib().gp_solve_darwin(self.h, dt, tol, maxiter, solver, st), allow_nc=True)
@@ -288,7 +308,8 @@ class Context:
                 F_JX: ((ny + 1, nx + 2), np.float64), F_JY: ((ny + 2, nx + 1), np.float64),
                 F_DEL_EX: ((ny + 1, nx + 2), np.float32), F_DEL_EY: ((ny + 2, nx + 1), np.float32),
                 F_DEL_BZ: ((ny + 2, nx + 2), np.float32),
-                F_SOL: ((self.nky + 1, self.nkx + 1), np.float64)}.get(which) or self._darwin_shape(which)
+                F_SOL: ((self.nky + 1, self.nkx + 1), np.float64),
+                F_RHS: ((self.nky + 1, self.nkx + 1), np.float64)}.get(which) or self._darwin_shape(which)
 
     def _darwin_shape(self, which):
         perx = self.cfg.fbc[0] == 0
@@ -313,6 +334,72 @@ class Context:
         a, b, c = C.c_double(), C.c_double(), C.c_double()
         _check(lib().gp_field_energy(self.h, dt, C.byref(a), C.byref(b), C.byref(c)))
         return a.value, b.value, c.value
+
+    # learned preconditioner -------------------------------------------
+    def nn_init(self, base_channels=32, seed=1):
+        _check(lib().gp_nn_init(self.h, base_channels, seed))
+
+    def nn_config(self, loss_kind=1, smooth=2, fcg_cap=0):
+        _check(lib().gp_nn_config(self.h, loss_kind, smooth, fcg_cap))
+
+    def nn_add_pair(self):
+        _check(lib().gp_nn_add_pair(self.h))
+
+    def nn_train(self, epochs, batch=4, lr=1e-3, rho=None, phi=None):
+        loss = C.c_float()
+        if rho is None:
+            _check(lib().gp_nn_train(self.h, None, None, 0, epochs, batch, lr, C.byref(loss)))
+        else:
+            rho = np.ascontiguousarray(rho, np.float64)
+            phi = np.ascontiguousarray(phi, np.float64)
+            n = rho.shape[0]
+            _check(lib().gp_nn_train(self.h, rho.ctypes.data_as(C.POINTER(C.c_double)),
+                                     phi.ctypes.data_as(C.POINTER(C.c_double)), n, epochs, batch, lr, C.byref(loss)))
+        return loss.value
+
+    def nn_params(self, set_to=None):
+        cnt = C.c_int64()
+        _check(lib().gp_nn_params(self.h, None, None, C.byref(cnt)))
+        out = np.empty(cnt.value, np.float32)
+        if set_to is not None:
+            v, vp = _f32(set_to)
+            assert v.size == cnt.value
+            _check(lib().gp_nn_params(self.h, None, vp, None))
+        _check(lib().gp_nn_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float)), None, None))
+        return out
+
+    def nn_dims(self):
+        H, W = C.c_int(), C.c_int()
+        _check(lib().gp_nn_dims(self.h, C.byref(H), C.byref(W)))
+        return H.value, W.value
+
+    def nn_forward(self, x):
+        x, xp = _f32(x)
+        out = np.empty_like(x)
+        _check(lib().gp_nn_forward(self.h, xp, x.shape[0], out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def nn_grad(self, x, t, kind=0):
+        (x, xp), (t, tp) = _f32(x), _f32(t)
+        cnt = C.c_int64()
+        _check(lib().gp_nn_params(self.h, None, None, C.byref(cnt)))
+        g = np.empty(cnt.value, np.float32)
+        loss = C.c_float()
+        _check(lib().gp_nn_grad(self.h, xp, tp, x.shape[0], kind, C.byref(loss), g.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss.value, g
+
+    def nn_infer(self, rhs):
+        rhs = np.ascontiguousarray(rhs, np.float64).ravel()
+        out = np.empty_like(rhs)
+        _check(lib().gp_nn_infer(self.h, rhs.ctypes.data_as(C.POINTER(C.c_double)),
+                                 out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def nn_save(self, path):
+        _check(lib().gp_nn_save(self.h, path.encode()))
+
+    def nn_load(self, path):
+        _check(lib().gp_nn_load(self.h, path.encode()))
 
     # timing ------------------------------------------------------------
     def timing(self, enable=True):

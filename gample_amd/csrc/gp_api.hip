@@ -321,6 +321,14 @@ int gp_solve_fields(gp_ctx *c, double dt, double tol, int maxiter, int solver, g
     })
 }
 
+int gp_solve_rhs(gp_ctx *c, const double *b, double *x, double tol, int maxiter, int solver, gp_solve_stats *st)
+{
+    GP_CHECK_CTX(c);
+    if (!b || !x) return gp_fail(GP_ERR_ARG, "null vector");
+    if (!(tol > 0) || maxiter < 1) return gp_fail(GP_ERR_ARG, "tol must be > 0 and maxiter >= 1");
+    GP_TRY({ return gp_solve_rhs_impl(c, b, x, tol, maxiter, solver, st); })
+}
+
 int gp_solve_darwin(gp_ctx *c, double dt, double tol, int maxiter, int solver, gp_solve_stats st[3])
 {
     GP_CHECK_CTX(c);
@@ -425,6 +433,7 @@ int gp_field_get(gp_ctx *c, int which, void *host, int64_t bytes)
     case GP_F_CHI: src = c->dsol_chi; need = (int64_t)c->ds.n * 8; break;
     case GP_F_AX: src = c->dsol_ax; need = (int64_t)c->ds.n * 8; break;
     case GP_F_AY: src = c->dsol_ay; need = (int64_t)c->ds.n * 8; break;
+    case GP_F_RHS: src = c->ps.b64; need = (int64_t)c->ps.n * 8; break;
     default: return gp_fail(GP_ERR_ARG, "unknown field id");
     }
     if (!src) return gp_fail(GP_ERR_STATE, "field not allocated (Darwin fields need em_model = 2)");
@@ -509,6 +518,44 @@ int gp_nn_load(gp_ctx *c, const char *path)
 {
     GP_CHECK_CTX(c);
     GP_TRY({ return gp_nn_load_impl(c, path); })
+}
+int gp_nn_add_pair(gp_ctx *c)
+{
+    GP_CHECK_CTX(c);
+    if (!c->nn) return gp_fail(GP_ERR_STATE, "gp_nn_init first");
+    if (!c->have_solution) return gp_fail(GP_ERR_STATE, "no Poisson solution yet");
+    GP_TRY({ return gp_nn_add_pair_dev(c, c->ps.b64, c->ps.x64); })
+}
+int gp_nn_config(gp_ctx *c, int loss_kind, int smooth_sweeps, int fcg_cap)
+{
+    GP_CHECK_CTX(c);
+    GP_TRY({ return gp_nn_config_impl(c, loss_kind, smooth_sweeps, fcg_cap); })
+}
+int gp_nn_params(gp_ctx *c, float *get, const float *set, int64_t *count)
+{
+    GP_CHECK_CTX(c);
+    GP_TRY({ return gp_nn_params_impl(c, get, set, count); })
+}
+int gp_nn_dims(gp_ctx *c, int *H, int *W)
+{
+    GP_CHECK_CTX(c);
+    if (!H || !W) return gp_fail(GP_ERR_ARG, "null output");
+    const int mx = c->ps.lv[0].mx, my = c->ps.lv[0].my;
+    *W = (mx + 7) / 8 * 8;
+    *H = (my + 7) / 8 * 8;
+    return GP_OK;
+}
+int gp_nn_forward(gp_ctx *c, const float *in, int B, float *out)
+{
+    GP_CHECK_CTX(c);
+    if (!in || !out || B < 1) return gp_fail(GP_ERR_ARG, "bad forward arguments");
+    GP_TRY({ return gp_nn_forward_impl(c, in, B, out); })
+}
+int gp_nn_grad(gp_ctx *c, const float *in, const float *tgt, int B, int kind, float *loss, float *grad)
+{
+    GP_CHECK_CTX(c);
+    if (!in || !tgt || B < 1 || kind < 0 || kind > 1) return gp_fail(GP_ERR_ARG, "bad gradient arguments");
+    GP_TRY({ return gp_nn_grad_impl(c, in, tgt, B, kind, loss, grad); })
 }
 
 }  // extern "C"

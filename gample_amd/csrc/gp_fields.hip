@@ -453,7 +453,20 @@ int gp_energy_impl(gp_ctx *c, double dt, double *we_ref, double *we_cgs, double 
     return GP_OK;
 }
 
-int gp_nn_precorrect_cb(void *ctx, float *e, const float *r);
+// solver: 0 MG-PCG; 1 network warm start + network flexible CG (falling
+// back to MG-PCG); 2 network warm start + MG-PCG; 3 network flexible CG
+// from the current iterate
+int make_hooks(gp_ctx *c, int solver, gp_nn_bind *bind, gp_hooks *hk)
+{
+    if (solver < 0 || solver > 3) return gp_fail(GP_ERR_ARG, "solver must be 0..3");
+    if (solver == 0) return GP_OK;
+    if (!c->nn) return gp_fail(GP_ERR_STATE, "network solver requested before gp_nn_init/gp_nn_load");
+    hk->ctx = bind;
+    if (solver == 1 || solver == 2) hk->warm = gp_nn_precorrect_cb;
+    if (solver == 1 || solver == 3) hk->precond = gp_nn_precond_cb;
+    hk->cap = gp_nn_fcg_cap(c);
+    return GP_OK;
+}
 
 int gp_solve_fields_impl(gp_ctx *c, double dt, double tol, int maxiter, int solver, gp_solve_stats *st,
                          gp_diag *diag)
@@ -466,15 +479,13 @@ int gp_solve_fields_impl(gp_ctx *c, double dt, double tol, int maxiter, int solv
     k_rhs<<<gp_cdiv(mx * my, 256), 256, 0, sm>>>(s.rhs_bc, c->rho, s.b64, mx, my, s.ikmin, s.jkmin, c->nx + 1);
     GP_HIP(hipGetLastError());
     gp_solve_stats local{};
-    int (*pc)(void *, float *, const float *) = nullptr;
-    if (solver == 1) {
-        if (!c->nn) return gp_fail(GP_ERR_STATE, "NN solver requested before gp_nn_init/gp_nn_load");
-        pc = gp_nn_precorrect_cb;
-        local.used_nn = 1;
-    }
-    const int status = gp_solve_spd(sm, s, tol, maxiter, s.singular, &local, pc, c);
+    gp_hooks hk;
+    gp_nn_bind bind{c, &s};
+    int rc2 = make_hooks(c, solver, &bind, &hk);
+    if (rc2) return rc2;
+    const int status = gp_solve_spd(sm, s, tol, maxiter, s.singular, &local, solver ? &hk : nullptr);
     if (status < 0) return status;
-    if (st) { int un = local.used_nn; *st = local; st->used_nn = un; }
+    if (st) *st = local;
     RecArgs ra;
     ra.ngb = c->ngb; ra.ngx = c->ngx; ra.ngy = c->ngy; ra.nx = c->nx; ra.ny = c->ny; ra.nkx = s.nkx; ra.nky = s.nky;
     for (int w = 0; w < 4; w++) { ra.b[w] = c->cfg.fbc[w]; ra.v[w] = c->cfg.fbc_value[w]; }
@@ -489,6 +500,34 @@ int gp_solve_fields_impl(gp_ctx *c, double dt, double tol, int maxiter, int solv
     return status;
 }
 
+// S x = b on the Poisson hierarchy for a caller-given b (host FP64), x in/out
+// as the initial guess; the context's phi and E are not touched
+int gp_solve_rhs_impl(gp_ctx *c, const double *b, double *x, double tol, int maxiter, int solver, gp_solve_stats *st)
+{
+    hipStream_t sm = c->stream;
+    gp_solver &s = c->ps;
+    const int64_t m = s.n;
+    // keep the context's own solution (the warm start of its next step)
+    double *keep = nullptr;
+    GP_HIP(hipMallocAsync((void **)&keep, m * 8, sm));
+    GP_HIP(hipMemcpyAsync(keep, s.x64, m * 8, hipMemcpyDeviceToDevice, sm));
+    GP_HIP(hipMemcpyAsync(s.b64, b, m * 8, hipMemcpyHostToDevice, sm));
+    GP_HIP(hipMemcpyAsync(s.x64, x, m * 8, hipMemcpyHostToDevice, sm));
+    gp_solve_stats local{};
+    gp_hooks hk;
+    gp_nn_bind bind{c, &s};
+    int rc = make_hooks(c, solver, &bind, &hk);
+    if (rc) return rc;
+    const int status = gp_solve_spd(sm, s, tol, maxiter, s.singular, &local, solver ? &hk : nullptr);
+    if (status < 0) return status;
+    GP_HIP(hipMemcpyAsync(x, s.x64, m * 8, hipMemcpyDeviceToHost, sm));
+    GP_HIP(hipMemcpyAsync(s.x64, keep, m * 8, hipMemcpyDeviceToDevice, sm));
+    GP_HIP(hipFreeAsync(keep, sm));
+    GP_HIP(hipStreamSynchronize(sm));
+    if (st) *st = local;
+    return status;
+}
+
 int gp_solve_darwin_impl(gp_ctx *c, double dt, double tol, int maxiter, int solver, gp_solve_stats st[3])
 {
     if (c->cfg.em_model != 2) return gp_fail(GP_ERR_STATE, "Darwin solve needs em_model = 2");
@@ -500,8 +539,11 @@ int gp_solve_darwin_impl(gp_ctx *c, double dt, double tol, int maxiter, int solv
     a.perx = c->cfg.fbc[0] == GP_FBC_PERIODIC; a.pery = c->cfg.fbc[2] == GP_FBC_PERIODIC;
     a.dx = c->dxf; a.dy = c->dyf;
     const int m = (d.nkx + 1) * (d.nky + 1);
-    int (*pc)(void *, float *, const float *) = nullptr;
-    (void)solver;
+    gp_hooks hk;
+    gp_nn_bind bind{c, &d};
+    int rc3 = make_hooks(c, solver, &bind, &hk);
+    if (rc3) return rc3;
+    const gp_hooks *hp = solver ? &hk : nullptr;
     // chi
     k_darwin_chi_rhs<<<gp_cdiv(m, 256), 256, 0, sm>>>(c->jx, c->jy, d.b64, a);
     GP_HIP(hipGetLastError());
@@ -509,7 +551,7 @@ int gp_solve_darwin_impl(gp_ctx *c, double dt, double tol, int maxiter, int solv
     if (rc) return rc;
     k_darwin_weight<<<gp_cdiv(m, 256), 256, 0, sm>>>(d.b64, a);
     k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(d.x64, c->dsol_chi, m);
-    int s0 = gp_solve_spd(sm, d, tol, maxiter, 1, &st[0], pc, c);
+    int s0 = gp_solve_spd(sm, d, tol, maxiter, 1, &st[0], hp);
     if (s0 < 0) return s0;
     k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(c->dsol_chi, d.x64, m);
     // Ax, Ay rhs from chi; solve each
@@ -523,12 +565,12 @@ int gp_solve_darwin_impl(gp_ctx *c, double dt, double tol, int maxiter, int solv
     k_darwin_weight<<<gp_cdiv(m, 256), 256, 0, sm>>>(by, a);
     k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(d.b64, bx, m);
     k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(d.x64, c->dsol_ax, m);
-    int s1 = gp_solve_spd(sm, d, tol, maxiter, 1, &st[1], pc, c);
+    int s1 = gp_solve_spd(sm, d, tol, maxiter, 1, &st[1], hp);
     if (s1 < 0) return s1;
     k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(c->dsol_ax, d.x64, m);
     k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(d.b64, by, m);
     k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(d.x64, c->dsol_ay, m);
-    int s2 = gp_solve_spd(sm, d, tol, maxiter, 1, &st[2], pc, c);
+    int s2 = gp_solve_spd(sm, d, tol, maxiter, 1, &st[2], hp);
     if (s2 < 0) return s2;
     k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(c->dsol_ay, d.x64, m);
     GP_HIP(hipFreeAsync(bx, sm));

@@ -90,6 +90,7 @@ struct gp_solver {
     double *hscal = nullptr;   // pinned host mirror
     int red_blocks = 0;
     int last_it = 4;           // inner iterations of the previous solve
+    int mg_ref_it = 0;         // iterations of the last plain MG-PCG solve
 };
 
 struct gp_nn;   // gp_nn.hip
@@ -146,8 +147,22 @@ int gp_energy_impl(gp_ctx *c, double dt, double *we_ref, double *we_cgs, double 
 // ---- elliptic solver (gp_mg.hip) ----
 int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double cy);
 void gp_mg_free(gp_solver &s);
+// learned hooks of an elliptic solve (null members: plain MG-PCG)
+struct gp_hooks {
+    int (*warm)(void *, float *e, const float *r) = nullptr;      // first correction
+    int (*precond)(void *, float *z, const float *r) = nullptr;   // flexible-CG preconditioner
+    void *ctx = nullptr;
+    int cap = 0;   // flexible-CG iterations before the fallback (0: last plain MG-PCG count)
+};
 int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int singular, gp_solve_stats *stats,
-                 int (*precorrect)(void *, float *, const float *), void *pc_ctx);
+                 const gp_hooks *hk);
+int gp_mg_scale(hipStream_t st, gp_solver &s, float *e, const float *r);
+int gp_mg_smooth(hipStream_t st, gp_solver &s, float *z, const float *r, int nsweeps);
+// context of the learned hooks: the network and the hierarchy it serves
+struct gp_nn_bind {
+    gp_ctx *c;
+    gp_solver *s;
+};
 int gp_mg_sub_mean(hipStream_t st, gp_solver &s, double *a);
 int gp_mg_apply(hipStream_t st, gp_solver &s, const float *in, float *out);
 
@@ -158,6 +173,15 @@ int gp_nn_train_impl(gp_ctx *c, const double *rho, const double *phi, int nsamp,
                      float lr, float *loss_out);
 int gp_nn_apply(gp_ctx *c, gp_solver *s, const float *in_dev, float *out_dev);
 int gp_nn_infer_impl(gp_ctx *c, const double *rhs, double *out);
+int gp_nn_add_pair_dev(gp_ctx *c, const double *b, const double *x);
+int gp_nn_grad_impl(gp_ctx *c, const float *in, const float *tgt, int B, int kind, float *loss, float *grad);
+int gp_nn_forward_impl(gp_ctx *c, const float *in, int B, float *out);
+int gp_nn_params_impl(gp_ctx *c, float *get, const float *set, int64_t *count);
+int gp_solve_rhs_impl(gp_ctx *c, const double *b, double *x, double tol, int maxiter, int solver, gp_solve_stats *st);
+int gp_nn_precorrect_cb(void *ctx, float *e, const float *r);
+int gp_nn_precond_cb(void *ctx, float *z, const float *r);
+int gp_nn_config_impl(gp_ctx *c, int loss_kind, int smooth, int fcg_cap);
+int gp_nn_fcg_cap(const gp_ctx *c);
 int gp_nn_save_impl(gp_ctx *c, const char *path);
 int gp_nn_load_impl(gp_ctx *c, const char *path);
 

@@ -562,6 +562,50 @@ __global__ void k_zero_f(float *d, int n)
 {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) d[k] = 0.0f;
 }
+// q = S x, partial x.q
+__global__ __launch_bounds__(RB) void k_spmv_dot(const float *x, float *q, Lev L, double *part)
+{
+    __shared__ double sh[RB / 64];
+    const int n = L.mx * L.my;
+    double acc = 0.0;
+    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
+        const int i = k % L.mx, j = k / L.mx;
+        const float c = x[k];
+        float uL, uR, uD, uU;
+        uL = i > 0 ? x[k - 1] : (L.tx == GP_AX_PER ? x[k + L.mx - 1] : (L.tx == GP_AX_NEU ? x[k + 1] : 0.0f));
+        uR = i < L.mx - 1 ? x[k + 1] : (L.tx == GP_AX_PER ? x[k - (L.mx - 1)] : (L.tx == GP_AX_NEU ? x[k - 1] : 0.0f));
+        uD = j > 0 ? x[k - L.mx] : (L.ty == GP_AX_PER ? x[k + (L.my - 1) * L.mx] : (L.ty == GP_AX_NEU ? x[k + L.mx] : 0.0f));
+        uU = j < L.my - 1 ? x[k + L.mx] : (L.ty == GP_AX_PER ? x[k - (L.my - 1) * L.mx] : (L.ty == GP_AX_NEU ? x[k - L.mx] : 0.0f));
+        const float w = wgt(L.tx, i, L.mx) * wgt(L.ty, j, L.my);
+        const float v = w * (L.cx * (2.0f * c - uL - uR) + L.cy * (2.0f * c - uD - uU));
+        q[k] = v;
+        acc += (double)c * (double)v;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// p = z + beta p (beta = scal[S_BETA])
+__global__ void k_xpby(const float *z, float *p, int n, const double *scal)
+{
+    const float beta = (float)scal[S_BETA];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        p[k] = __builtin_fmaf(beta, p[k], z[k]);
+}
+// flexible-CG coefficients (Polak-Ribiere form, r_new - r_old = -alpha q):
+// beta = -alpha (z.q) / rz_old, rz = rz_new; the sums are in S_TMP (z.r)
+// and S_BB (z.q)
+__global__ void k_fcg_beta(double *scal)
+{
+    scal[S_BETA] = scal[S_RZ] != 0.0 ? -scal[S_ALPHA] * scal[S_BB] / scal[S_RZ] : 0.0;
+    scal[S_RZ] = scal[S_TMP];
+}
+// e *= (e.r) / (e.S e): S_TMP = e.r, S_PQ = e.S e
+__global__ void k_scale_opt(float *e, int n, const double *scal)
+{
+    const double d = scal[S_PQ];
+    const float a = d > 0.0 ? (float)(scal[S_TMP] / d) : 0.0f;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) e[k] *= a;
+}
 
 // FP64 refinement kernels
 __global__ __launch_bounds__(RB) void k_resid64(const double *x, const double *b, double *r, Lev L, double cx,
@@ -735,7 +779,90 @@ int pcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int *iters)
     return GP_OK;
 }
 
+// FP32 flexible CG on S e = r with a caller-supplied (possibly nonlinear)
+// preconditioner z = M(r); e in s.e, r in s.r (overwritten).  Stops at
+// ||r|| <= inner_tol ||r0|| (converged = 1) or after maxit iterations.
+int fcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int (*M)(void *, float *, const float *), void *mctx,
+          int *iters, int *converged)
+{
+    const Lev L0 = mklev(s.lv[0]);
+    const int n = s.n, nb = s.red_blocks;
+    const int ge = std::min(1024, gp_cdiv(n, 256));
+    double *red_a = s.red, *red_b = s.red + 1024;
+    // k_update_xr skips its work once the convergence flag is up; the flag
+    // belongs to pcg32's batched iterations and may still be set
+    k_set_scalar<<<1, 1, 0, st>>>(s.scal, S_DONE, 0.0);
+    k_zero_f<<<ge, 256, 0, st>>>(s.e, n);
+    k_dot<<<nb, RB, 0, st>>>(s.r, s.r, n, red_a);
+    k_final<<<1, RB, 0, st>>>(red_a, nb, s.scal, S_RR, 0);
+    GP_HIP(hipGetLastError());
+    GP_HIP(hipMemcpyAsync(s.hscal, s.scal, S_NS * sizeof(double), hipMemcpyDeviceToHost, st));
+    GP_HIP(hipStreamSynchronize(st));
+    const double target = inner_tol * std::sqrt(s.hscal[S_RR]);
+    *iters = 0;
+    *converged = s.hscal[S_RR] > 0.0 ? 0 : 1;
+    if (*converged) return GP_OK;
+    int rc = M(mctx, s.z, s.r);
+    if (rc) return rc;
+    GP_HIP(hipMemcpyAsync(s.p, s.z, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    k_dot<<<nb, RB, 0, st>>>(s.z, s.r, n, red_a);
+    k_final<<<1, RB, 0, st>>>(red_a, nb, s.scal, S_RZ, 0);
+    for (int it = 1; it <= maxit; it++) {
+        k_spmv_dot<<<nb, RB, 0, st>>>(s.p, s.q, L0, red_a);
+        k_final<<<1, RB, 0, st>>>(red_a, nb, s.scal, S_PQ, 1);                 // alpha = rz / pq
+        k_update_xr<<<nb, RB, 0, st>>>(s.e, s.r, s.p, s.q, n, s.scal, red_b);   // e += a p, r -= a q
+        k_final<<<1, RB, 0, st>>>(red_b, nb, s.scal, S_RR, 0);
+        GP_HIP(hipGetLastError());
+        GP_HIP(hipMemcpyAsync(s.hscal, s.scal, S_NS * sizeof(double), hipMemcpyDeviceToHost, st));
+        GP_HIP(hipStreamSynchronize(st));
+        *iters = it;
+        const double rr = s.hscal[S_RR];
+        if (!(rr == rr)) return gp_fail(GP_ERR_NONFINITE, "non-finite residual in the flexible CG");
+        if (std::sqrt(rr) <= target) { *converged = 1; break; }
+        if (it == maxit) break;
+        if ((rc = M(mctx, s.z, s.r))) return rc;
+        k_dot<<<nb, RB, 0, st>>>(s.z, s.r, n, red_a);
+        k_final<<<1, RB, 0, st>>>(red_a, nb, s.scal, S_TMP, 0);
+        k_dot<<<nb, RB, 0, st>>>(s.z, s.q, n, red_b);
+        k_final<<<1, RB, 0, st>>>(red_b, nb, s.scal, S_BB, 0);
+        k_fcg_beta<<<1, 1, 0, st>>>(s.scal);
+        k_xpby<<<ge, 256, 0, st>>>(s.z, s.p, n, s.scal);
+        GP_HIP(hipGetLastError());
+    }
+    return GP_OK;
+}
+
 }  // namespace
+
+// nsweeps red-black Gauss-Seidel sweeps (odd: red then black; the count is
+// rounded up to even so the result lands back in z) on S z = r, level 0
+int gp_mg_smooth(hipStream_t st, gp_solver &s, float *z, const float *r, int nsweeps)
+{
+    const Lev L = mklev(s.lv[0]);
+    float *a = z, *b = s.lv[0].r;
+    const int ns = (nsweeps + 1) / 2 * 2;
+    for (int k = 0; k < ns; k++) {
+        k_rb_sweep<<<tiles(L), dim3(32, 8), 0, st>>>(a, r, b, L, k & 1, nullptr);
+        std::swap(a, b);
+    }
+    GP_HIP(hipGetLastError());
+    return GP_OK;
+}
+
+// e *= (e.r) / (e.S e): the S-norm optimal multiple of a correction e
+int gp_mg_scale(hipStream_t st, gp_solver &s, float *e, const float *r)
+{
+    const Lev L0 = mklev(s.lv[0]);
+    const int n = s.n, nb = s.red_blocks;
+    // s.q2 is scratch here: the flexible CG keeps S p in s.q across this call
+    k_spmv_dot<<<nb, RB, 0, st>>>(e, s.q2, L0, s.red);
+    k_final<<<1, RB, 0, st>>>(s.red, nb, s.scal, S_PQ, 0);
+    k_dot<<<nb, RB, 0, st>>>(e, r, n, s.red + 1024);
+    k_final<<<1, RB, 0, st>>>(s.red + 1024, nb, s.scal, S_TMP, 0);
+    k_scale_opt<<<std::min(1024, gp_cdiv(n, 256)), 256, 0, st>>>(e, n, s.scal);
+    GP_HIP(hipGetLastError());
+    return GP_OK;
+}
 
 int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double cy)
 {
@@ -813,10 +940,12 @@ void gp_mg_free(gp_solver &s)
 }
 
 // Solve S x = b64 (already in s.b64) starting from s.x64, FP64 refinement.
-// precorrect, when given, supplies the first correction (a learned warm
-// start) in place of MG-PCG; the FP64 loop then continues with MG-PCG.
+// Hooks (may be null): hk->warm supplies the first correction (a learned
+// warm start) in place of an inner solve; hk->precond makes the inner solves
+// flexible CG with that preconditioner, which falls back to MG-PCG once its
+// iterations exceed those of the last plain MG-PCG solve on this hierarchy.
 int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int singular, gp_solve_stats *stats,
-                 int (*precorrect)(void *, float *, const float *), void *pc_ctx)
+                 const gp_hooks *hk)
 {
     s.singular = singular;
     const Lev L0 = mklev(s.lv[0]);
@@ -830,7 +959,10 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
         k_sub_mean64<<<ge, 256, 0, st>>>(s.b64, n, s.scal, S_MEAN);
         GP_HIP(hipGetLastError());
     }
-    int total_it = 0, outer = 0;
+    int total_it = 0, outer = 0, nn_it = 0, fell_back = 0;
+    const bool warm = hk && hk->warm, flex = hk && hk->precond;
+    bool warm_pending = warm;
+    const int cap = (hk && hk->cap > 0) ? hk->cap : (s.mg_ref_it > 0 ? s.mg_ref_it : 100);
     double rel = 0.0;
     int status = GP_OK;
     for (;;) {
@@ -844,17 +976,17 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
         rel = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
         if (!(rel == rel)) return gp_fail(GP_ERR_NONFINITE, "non-finite residual in the FP64 refinement");
         if (rel <= tol || bb == 0.0) break;
-        if (total_it >= maxiter || outer >= 20) { status = GP_NOT_CONVERGED; break; }
+        if (total_it >= maxiter || outer >= 40) { status = GP_NOT_CONVERGED; break; }
         const double rms = std::sqrt(rr / n);
         const double sc = rms > 0 ? 1.0 / rms : 1.0;
         k_to_f32<<<ge, 256, 0, st>>>(s.r64, s.r, n, sc);
         GP_HIP(hipGetLastError());
-        if (precorrect) {
-            int rc = precorrect(pc_ctx, s.e, s.r);
+        if (warm_pending) {
+            int rc = hk->warm(hk->ctx, s.e, s.r);
             if (rc) return rc;
             k_axpy64<<<ge, 256, 0, st>>>(s.x64, s.e, n, 1.0 / sc);
             GP_HIP(hipGetLastError());
-            precorrect = nullptr;
+            warm_pending = false;
             outer++;
             continue;
         }
@@ -862,8 +994,17 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
         // usefully go below ~1e-6 of its own starting residual)
         const double inner = std::max(2e-6, std::min(1e-2, 0.3 * tol / rel));
         int it = 0;
-        int rc = pcg32(st, s, inner, std::max(1, maxiter - total_it), &it);
-        if (rc) return rc;
+        if (flex && !fell_back) {
+            int conv = 0;
+            const int lim = std::max(1, std::min(cap - nn_it, maxiter - total_it));
+            int rc = fcg32(st, s, inner, lim, hk->precond, hk->ctx, &it, &conv);
+            if (rc) return rc;
+            nn_it += it;
+            if (!conv && nn_it >= cap) fell_back = 1;
+        } else {
+            int rc = pcg32(st, s, inner, std::max(1, maxiter - total_it), &it);
+            if (rc) return rc;
+        }
         total_it += it;
         k_axpy64<<<ge, 256, 0, st>>>(s.x64, s.e, n, 1.0 / sc);
         GP_HIP(hipGetLastError());
@@ -877,11 +1018,14 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
     }
     GP_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
+    if (!warm && !flex && status == GP_OK) s.mg_ref_it = std::max(1, total_it);
     if (stats) {
         stats->iterations = total_it;
         stats->refinements = outer;
         stats->relres = rel;
         stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+        stats->used_nn = (warm || flex) ? 1 : 0;
+        stats->fell_back = fell_back;
     }
     return status;
 }

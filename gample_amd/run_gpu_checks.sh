@@ -22,10 +22,10 @@ if [ "$what" = all ] || [ "$what" = smoke ]; then
   step smoke 180 python -u -c "import __graft_entry__ as g; g.smoke()"
 fi
 if [ "$what" = all ] || [ "$what" = bench ]; then
-  step bench 400 python -u bench.py --steps 20 --warmup 5
+  step bench 500 python -u bench.py
 fi
 if [ "$what" = all ] || [ "$what" = prof ]; then
   export TMPDIR=/tmp
-  step prof 400 rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof -o bench_kt -- python -u bench.py --steps 5 --warmup 2 --no-cpu
+  step prof 400 rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof -o bench_kt -- python -u bench.py --no-cpu --no-nn
 fi
 exit 0

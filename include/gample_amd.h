@@ -127,6 +127,7 @@ enum {
     GP_F_CHI,        /* double Darwin potentials on the node grid (nkxv+1)(nkyv+1), */
     GP_F_AX,         /*   nkxv = ngx - [periodic x] (EMField.mm:486-495) */
     GP_F_AY,
+    GP_F_RHS,        /* double Poisson right-hand side b of the SPD system (same grid as GP_F_SOL) */
     GP_F_COUNT
 };
 /* per-species accumulator identifiers for gp_species_grid_get */
@@ -186,6 +187,13 @@ int  gp_deposit(gp_ctx *ctx, int sid);
 int  gp_solve_fields(gp_ctx *ctx, double dt, double tol, int maxiter, int solver,
                      gp_solve_stats *st, gp_diag *diag);
 /* -[EMField solveVectorPotential:withLogger:] (EMField.mm:1138-1361) */
+/* Solve the Poisson system S x = b for a caller-given right-hand side b
+ * (host FP64, (nkx+1)(nky+1) unknowns as GP_F_SOL), x in/out as the initial
+ * guess, with the same solver choices as gp_solve_fields.  The context's own
+ * potential, fields and warm start are left unchanged.  No reference
+ * counterpart (the reference solves only its own rhs, EMField.mm:853-866). */
+int  gp_solve_rhs(gp_ctx *ctx, const double *b, double *x, double tol, int maxiter, int solver,
+                  gp_solve_stats *st);
 int  gp_solve_darwin(gp_ctx *ctx, double dt, double tol, int maxiter, int solver,
                      gp_solve_stats st[3]);
 /* -[Particle update:withEMField:withMom:withLogger:] (Particle.mm:615-672 +
@@ -234,6 +242,28 @@ int  gp_nn_train(gp_ctx *ctx, const double *rho, const double *phi, int nsamp, i
 int  gp_nn_infer(gp_ctx *ctx, const double *rhs, double *out);
 int  gp_nn_save(gp_ctx *ctx, const char *path);
 int  gp_nn_load(gp_ctx *ctx, const char *path);
+/* Training loss (0 mean square, 1 relative energy norm: the default), the
+ * red-black sweeps that follow the network inside the flexible-CG
+ * preconditioner (default 2), and the flexible-CG iterations allowed before
+ * it falls back to MG-PCG (0, the default: as many as the last plain MG-PCG
+ * solve on the same hierarchy needed). */
+int  gp_nn_config(gp_ctx *ctx, int loss_kind, int smooth_sweeps, int fcg_cap);
+/* Append the Poisson solver's current (b, x) pair (the last solve's
+ * right-hand side and solution, device resident) to the training set. */
+int  gp_nn_add_pair(gp_ctx *ctx);
+/* Flat FP32 parameter vector (get and/or set; count returns its length).
+ * Order per layer (in, W1..W6, out): weights [co][3x3 tap][ci], then bias. */
+int  gp_nn_params(gp_ctx *ctx, float *get, const float *set, int64_t *count);
+/* Network image geometry for the Poisson grid: H, W (padded unknown grid). */
+int  gp_nn_dims(gp_ctx *ctx, int *H, int *W);
+/* One forward pass on B host images (B x H x W FP32) -> outputs. */
+int  gp_nn_forward(gp_ctx *ctx, const float *in, int B, float *out);
+/* One forward + backward against targets, no update: the loss and the flat
+ * FP32 gradient (gp_nn_params order).  kind 0: mean square over the image;
+ * kind 1 (the training loss): energy norm (y - t)' S1 (y - t) per image,
+ * mean over the batch, S1 the grid-unit Poisson operator on the unknown
+ * grid (zero outside it). */
+int  gp_nn_grad(gp_ctx *ctx, const float *in, const float *tgt, int B, int kind, float *loss, float *grad);
 
 #ifdef __cplusplus
 }
