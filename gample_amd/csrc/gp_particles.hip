@@ -146,7 +146,10 @@ template <int ORDER, bool UNIB, int DEP, bool PUSH>
 __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
 {
     constexpr int NGB = ORDER / 2, NW = ORDER + 1;
-    constexpr int W = WIN + 2 * NGB + 1;              // node window edge
+    // node window edge: relative to X0 = cx0 - NGB the stencils reach from 0
+    // (rho: i2 - NGB with i2 = cx0) to WIN + NW - 1 (Ex, jx: i1 + 1 - NGB +
+    // NW - 1 with i1 = i2 = cx0 + WIN - 1; likewise Ey, jy in y)
+    constexpr int W = WIN + NW;
     constexpr int NF = PUSH ? W * W : 1;
     constexpr int NB = (PUSH && !UNIB) ? W * W : 1;
     constexpr int NR = DEP >= 1 ? W * W : 1;
@@ -663,7 +666,7 @@ __device__ __forceinline__ Pcg seed_stream(uint64_t master, int64_t id)
 
 struct LoadArgs {
     gp_gen g;
-    float dx, dy, ngb;
+    float dx, dy, ngb, xdom, ydom;
     uint64_t seed;
     int64_t first_id, n, ntot;
     gp_soa e, i;
@@ -733,8 +736,16 @@ __global__ void k_load(LoadArgs a)
     const float vy = g.ele_u[1] + am * sinf(2.0f * PIf * rv2);
     rv1 = r.u01(); rv2 = r.u01();
     const float vz = g.ele_u[2] + g.ele_vth * sqrtf(-logf(rv1)) * cosf(2.0f * PIf * rv2);
-    x = x / a.dx + a.ngb;
-    y = y / a.dy + a.ngb;
+    // a position inside the domain (cm) that rounds onto the upper face in
+    // grid units is moved just below it (the push takes [ngb, ngb+ng))
+    {
+        const float hx = a.ngb + a.xdom, hy = a.ngb + a.ydom;
+        const float xc = x, yc = y;
+        x = x / a.dx + a.ngb;
+        y = y / a.dy + a.ngb;
+        if (xc < a.xdom * a.dx && x >= hx) x = __int_as_float(__float_as_int(hx) - 1);
+        if (yc < a.ydom * a.dy && y >= hy) y = __int_as_float(__float_as_int(hy) - 1);
+    }
     const int64_t je = a.ne + k;
     a.e.x[je] = x; a.e.y[je] = y; a.e.vx[je] = vx; a.e.vy[je] = vy; a.e.vz[je] = vz;
     if (a.with_ion && g.gen_type != 3) {
@@ -1033,6 +1044,7 @@ int gp_launch_load(gp_ctx *c, int se, int si, const gp_gen *g, uint64_t seed, in
     LoadArgs a{};
     a.g = *g;
     a.dx = c->dxf; a.dy = c->dyf; a.ngb = (float)c->ngb;
+    a.xdom = (float)c->ngx; a.ydom = (float)c->ngy;
     a.seed = seed; a.first_id = first_id; a.n = n; a.ntot = first_id + n;
     a.e = E.a; a.ne = E.n;
     a.with_ion = si >= 0;

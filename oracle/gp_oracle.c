@@ -549,6 +549,38 @@ uint64_t gpo_splitmix64(uint64_t *seed)
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
 }
+/* Half-Maxwellian flux injection through the x-min wall (no reference
+ * function: the build's wall-flux source).  Particle k of the call uses the
+ * PCG32 stream of id = stream + k seeded like the loader (state and inc are
+ * the splitmix64(seed) outputs 2 id + 1 and 2 id + 2).  vx = u0 + vth
+ * sqrt(-2 ln r1) is Rayleigh (the flux-weighted half-Maxwellian), vy, vz =
+ * u + vth sqrt(-2 ln r2) (cos, sin)(2 pi r3); x = wall + vx dt/dx r4 (a
+ * uniform fraction of the step flown inside), y uniform.  Grid units. */
+void gpo_inject_wall(uint64_t seed, uint64_t stream, int64_t n, float vth, const float u[3], float dtdx, float lo,
+                     float ngx, float ngy, float *x, float *y, float *vx, float *vy, float *vz)
+{
+    const float PIf = 3.14159265358979f;
+    for (int64_t k = 0; k < n; k++) {
+        const uint64_t id = stream + (uint64_t)k;
+        uint64_t sd = seed + 2ULL * id * 0x9E3779B97F4A7C15ULL;
+        uint64_t st = gpo_splitmix64(&sd);
+        const uint64_t inc = gpo_splitmix64(&sd) | 1ULL;
+        const float r1 = gpo_rand01(gpo_pcg32(&st, inc)), r2 = gpo_rand01(gpo_pcg32(&st, inc));
+        const float r3 = gpo_rand01(gpo_pcg32(&st, inc)), r4 = gpo_rand01(gpo_pcg32(&st, inc));
+        const float r5 = gpo_rand01(gpo_pcg32(&st, inc));
+        const float v = u[0] + vth * sqrtf(-2.0f * logf(r1));
+        const float am = vth * sqrtf(-2.0f * logf(r2));
+        float xx = lo + v * dtdx * r4;
+        const float hi = lo + ngx;
+        if (xx >= hi) xx = nextafterf(hi, 0.0f);
+        x[k] = xx;
+        y[k] = lo + r5 * ngy;
+        vx[k] = v;
+        vy[k] = u[1] + am * cosf(2.0f * PIf * r3);
+        vz[k] = u[2] + am * sinf(2.0f * PIf * r3);
+    }
+}
+
 /* src/Collision.mm:21-27 */
 uint32_t gpo_pcg32(uint64_t *state, uint64_t inc)
 {
@@ -644,8 +676,14 @@ int gpo_load_pair(const gpo_gen *p, uint64_t master_seed, int64_t first_id, int6
         float vy = p->ele_u[1] + a * sinf(2.0f * PIF * rv2);
         rv1 = gpo_rand01(gpo_pcg32(&st, inc)); rv2 = gpo_rand01(gpo_pcg32(&st, inc));
         float vz = p->ele_u[2] + p->ele_vth * sqrtf(-logf(rv1)) * cosf(2.0f * PIF * rv2);
-        x = x / p->dx + p->ngb;
-        y = y / p->dy + p->ngb;
+        {
+            const float hx = p->ngb + p->xdom, hy = p->ngb + p->ydom;
+            const float xc = x, yc = y;
+            x = x / p->dx + p->ngb;
+            y = y / p->dy + p->ngb;
+            if (p->xdom > 0 && xc < p->xdom * p->dx && x >= hx) x = nextafterf(hx, 0.0f);
+            if (p->ydom > 0 && yc < p->ydom * p->dy && y >= hy) y = nextafterf(hy, 0.0f);
+        }
         ex[k] = x; ey[k] = y; evx[k] = vx; evy[k] = vy; evz[k] = vz;
         if (ix && p->gen_type != 3) {
             rv1 = gpo_rand01(gpo_pcg32(&st, inc)); rv2 = gpo_rand01(gpo_pcg32(&st, inc));

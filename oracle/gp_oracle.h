@@ -132,7 +132,12 @@ typedef struct {
     float ion_u[3], ion_vth;
     float dx, dy, ngb;
     float pert_delta; int pert_mode;  /* gen_type 7 */
+    float xdom, ydom;  /* domain extent in cells (0: no clamp); a position
+                          inside the domain in cm that rounds onto the upper
+                          face in grid units is moved just below it */
 } gpo_gen;
+void gpo_inject_wall(uint64_t seed, uint64_t stream, int64_t n, float vth, const float u[3], float dtdx, float lo,
+                     float ngx, float ngy, float *x, float *y, float *vx, float *vy, float *vz);
 int gpo_load_pair(const gpo_gen *p, uint64_t master_seed, int64_t first_id, int64_t n,
                   float *ex, float *ey, float *evx, float *evy, float *evz,
                   float *ix, float *iy, float *ivx, float *ivy, float *ivz);

@@ -31,7 +31,8 @@ class Gen(C.Structure):
     _fields_ = [("gen_type", C.c_int), ("xmin", C.c_float), ("xmax", C.c_float), ("ymin", C.c_float),
                 ("ymax", C.c_float), ("ele_u", C.c_float * 3), ("ele_vth", C.c_float),
                 ("ion_u", C.c_float * 3), ("ion_vth", C.c_float), ("dx", C.c_float), ("dy", C.c_float),
-                ("ngb", C.c_float), ("pert_delta", C.c_float), ("pert_mode", C.c_int)]
+                ("ngb", C.c_float), ("pert_delta", C.c_float), ("pert_mode", C.c_int),
+                ("xdom", C.c_float), ("ydom", C.c_float)]
 
 
 class Field(C.Structure):
@@ -72,6 +73,8 @@ def lib():
         L.gpo_rand01.argtypes = [C.c_uint32]
         L.gpo_rand01.restype = C.c_float
         L.gpo_load_pair.argtypes = [P, C.c_uint64, C.c_int64, C.c_int64] + [P] * 10
+        L.gpo_inject_wall.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_float, P, C.c_float, C.c_float,
+                                      C.c_float, C.c_float] + [P] * 5
         L.gpo_poisson_dims.argtypes = [P] + [C.POINTER(C.c_int)] * 4
         L.gpo_fold_rho.argtypes = [P, P]
         L.gpo_poisson_csr.argtypes = [P, P, P, P, P]
@@ -198,6 +201,13 @@ def load_pair(gen: Gen, seed, first_id, n, ion=True):
     i = [np.zeros(n, np.float32) for _ in range(5)] if ion else [None] * 5
     lib().gpo_load_pair(C.byref(gen), seed, first_id, n, *[ptr(a) for a in e], *[ptr(a) for a in i])
     return e, (i if ion else None)
+
+
+def inject_wall(seed, stream, n, vth, u, dtdx, lo, ngx, ngy):
+    out = [np.zeros(n, np.float32) for _ in range(5)]
+    uu = np.asarray(u, np.float32)
+    lib().gpo_inject_wall(seed, stream, n, vth, ptr(uu), dtdx, lo, ngx, ngy, *[ptr(a) for a in out])
+    return out
 
 
 def poisson_dims(f):

@@ -86,3 +86,21 @@ def test_loader_cosine_perturbation(orc):
     c = 0.5 * (edges[1:] + edges[:-1])
     expect = n / 48 * (1 + 0.2 * np.cos(2 * np.pi * 3 * c))
     assert np.max(np.abs(h - expect) / expect) < 0.05
+
+
+def test_loader_keeps_positions_below_the_upper_face(orc):
+    """uniform positions in [0, L) cm can round onto ngb + ngx in grid units;
+    with the domain extent given, the loader moves them just below the face"""
+    n, ngx, dx = 400000, 64, np.float32(5.2563e-3)
+    g = orc.Gen()
+    g.gen_type = 0
+    g.xmin, g.xmax, g.ymin, g.ymax = 0.0, float(ngx * dx), 0.0, float(ngx * dx)
+    g.ele_vth, g.ion_vth = 1.0, 1.0
+    g.dx = g.dy = float(dx)
+    g.ngb = 0.0
+    raw, _ = orc.load_pair(g, 0, 0, n, ion=False)
+    g.xdom = g.ydom = ngx
+    fixed, _ = orc.load_pair(g, 0, 0, n, ion=False)
+    moved = raw[0] != fixed[0]
+    assert np.all(raw[0][moved] == ngx) and np.all(fixed[0][moved] == np.nextafter(np.float32(ngx), 0))
+    assert fixed[0].max() < ngx and fixed[1].max() < ngx
