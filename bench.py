@@ -200,12 +200,15 @@ def copy_calibration(mib):
 
 
 def traffic_from_profiles():
-    f = os.path.join(ROOT, "profiles", "push_deposit_traffic.json")
-    if os.path.exists(f):
-        try:
-            return json.load(open(f))
-        except Exception:
-            return None
+    """HBM bytes per launch of the fused kernel from the committed rocprof
+    counter summary (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy)."""
+    f = os.path.join(ROOT, "profiles", "push_deposit_counters.md")
+    try:
+        for line in open(f):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return {"bytes_per_launch": float(line.split(":")[1])}
+    except OSError:
+        pass
     return None
 
 

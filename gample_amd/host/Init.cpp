@@ -71,7 +71,7 @@ Json defComp()
     return obj({{"MaximumParticleNumber", N(0.08)}, {"ThreadGroupSize", N(256)}, {"IntegrationChunkSize", N(256)},
                 {"AggregationThreshold", N(0.1)}, {"ChebyshevDegree", N(5)}, {"AMGCycleType", N(1)},
                 {"MaxiterForPoisson", N(200)}, {"ToleranceForPoisson", N(1e-7)},
-                {"PoissonSolver", S("MG-CG")}, {"SortInterval", N(20)}});
+                {"PoissonSolver", S("MG-CG")}, {"SortInterval", N(20)}, {"NetworkFile", S("")}});
 }
 Json defParticle()
 {
@@ -138,7 +138,14 @@ void Init::parse(const Json &root)
     C.maxiter = (int)ival(c, "MaxiterForPoisson");
     C.tolerance = dval(c, "ToleranceForPoisson");
     const std::string solver = sval(c, "PoissonSolver");
-    C.solver = (solver == "NN-FPCG") ? 1 : 0;
+    if (solver == "MG-CG") C.solver = 0;
+    else if (solver == "NN-FPCG") C.solver = 1;
+    else if (solver == "NN-WARM-MG") C.solver = 2;
+    else if (solver == "NN-FCG") C.solver = 3;
+    else throw std::runtime_error("PoissonSolver must be MG-CG, NN-FPCG, NN-WARM-MG or NN-FCG: " + solver);
+    C.networkFile = sval(c, "NetworkFile");
+    if (C.solver != 0 && C.networkFile.empty())
+        throw std::runtime_error("PoissonSolver " + solver + " needs NetworkFile (a trained network, gp_nn_save)");
     C.sortInterval = (int)ival(c, "SortInterval");
     // ParamForField (Init.mm:106-149)
     Json d = merged(&section(root, "ParamForField", Json::Object), defField());

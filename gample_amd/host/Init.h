@@ -29,7 +29,8 @@ struct ParamForComputing {
     int chebDegree = 5, amgCycleType = 1, maxiter = 200;
     double tolerance = 1e-7;
     // keys added by this build (ignored by the reference): solver, sort
-    int solver = 0;            // "PoissonSolver": "MG-CG" (0) or "NN-FPCG" (1)
+    int solver = 0;            // "PoissonSolver": "MG-CG" (0), "NN-FPCG" (1), "NN-WARM-MG" (2), "NN-FCG" (3)
+    std::string networkFile;   // "NetworkFile": trained network for the network solvers
     int sortInterval = 20;     // "SortInterval"
 };
 

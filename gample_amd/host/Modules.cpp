@@ -121,6 +121,7 @@ EMField::EMField(const Init &init, int device)
     tol = init.paramForComputing.tolerance;
     maxiter = init.paramForComputing.maxiter;
     solver = init.paramForComputing.solver;
+    if (solver != 0) check(gp_nn_load(ctx, init.paramForComputing.networkFile.c_str()), "NetworkFile");
     if (F.InitTypeB == "From1dXFile") {   // EMField.mm:175-197, T -> G
         auto bx = load1d(F.FilePathBx), by = load1d(F.FilePathBy), bz = load1d(F.FilePathBz);
         if ((int)bz.size() < nx + 2 || (int)bx.size() < nx + 1 || (int)by.size() < nx + 2)

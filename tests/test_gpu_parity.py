@@ -517,3 +517,34 @@ def test_drifted_particles_reach_the_window_edges(gp, orc, order):
     np.testing.assert_array_equal(ctx.acc(sid, gp.G_JX_ACC), jx)
     np.testing.assert_array_equal(ctx.acc(sid, gp.G_JY_ACC), jy)
     ctx.close()
+
+
+def test_fused_step_equals_reference_order(gp):
+    """gp_step (solve -> fused push+deposit) against the reference's call
+    order per cycle (reset -> deposit each species -> solve -> push each
+    species), main.mm:154-254: same particles and fields to the bit."""
+    say("[fused vs reference order]")
+    ngx, dx, dt = 64, 0.005, 5e-12
+    outs = []
+    for fused in (True, False):
+        ctx = mkctx(gp, ngx, ngx, 3, dx=dx, b0=(0, 0, 200.0))
+        se = ctx.add_species(-EC, ME, 800.0, 200000, vth_hint=1e8)
+        si = ctx.add_species(EC, 2.18e-22, 800.0, 200000, vth_hint=1e5)
+        ctx.upload(se, *rand_parts(150000, ngx, ngx, 1, vs=1e8, seed=61))
+        ctx.upload(si, *rand_parts(150000, ngx, ngx, 1, vs=1e5, seed=62))
+        ctx.sort(se); ctx.sort(si)
+        if fused:
+            ctx.reset_grids(); ctx.deposit(se); ctx.deposit(si)
+            for cyc in range(1, 6):
+                ctx.step(dt, tol=1e-10, maxiter=500, cycle=cyc)
+        else:
+            for cyc in range(1, 6):
+                ctx.reset_grids(); ctx.deposit(se); ctx.deposit(si)
+                ctx.solve_fields(dt, tol=1e-10, maxiter=500)
+                ctx.push(se, dt, deposit=0); ctx.push(si, dt, deposit=0)
+        outs.append([ctx.download(se), ctx.download(si), ctx.field(gp.F_EX), ctx.field(gp.F_EY)])
+        ctx.close()
+    for a, b in zip(outs[0][0] + outs[0][1], outs[1][0] + outs[1][1]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(outs[0][2], outs[1][2])
+    np.testing.assert_array_equal(outs[0][3], outs[1][3])

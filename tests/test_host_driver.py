@@ -95,3 +95,16 @@ def test_defaults_merge(tmp_path):
     d = dump(str(f))
     assert d["weightOrder"] == 5 and d["ngb"] == 2 and d["maxiter"] == 200 and d["tolerance"] == 1e-7
     assert d["pNumMax"] == 0          # default @0.08 -> integerValue 0 (Init.mm:326)
+
+
+def test_poisson_solver_key(tmp_path):
+    """the build's PoissonSolver choices; a network mode needs NetworkFile"""
+    raw = json.load(open(os.path.join(COND, "periodic.json")))
+    f = tmp_path / "s.json"
+    for name, ok in (("MG-CG", True), ("NN-FCG", False), ("bogus", False)):
+        raw.setdefault("ParamForComputing", {})["PoissonSolver"] = name
+        f.write_text(json.dumps(raw))
+        out = subprocess.run([EXE, "-i", str(f), "--dump-config"], capture_output=True, text=True, timeout=30)
+        assert (out.returncode == 0) == ok, (name, out.stderr)
+        if not ok:
+            assert "PoissonSolver" in out.stderr
