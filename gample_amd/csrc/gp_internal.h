@@ -87,6 +87,7 @@ struct gp_solver {
     int bottom = 0;            // first level solved inside one workgroup
     double *red = nullptr;     // reduction partials
     double *scal = nullptr;    // device scalars
+    float *dinv = nullptr;     // dense inverse of the coarsest level (column-major), or null
     double *hscal = nullptr;   // pinned host mirror
     int red_blocks = 0;
     int last_it = 4;           // inner iterations of the previous solve

@@ -6,15 +6,16 @@
 // Neumann boundary nodes (making the ghost-mirror rows symmetric).
 //
 // V-cycle (symmetric, so it is a valid CG preconditioner):
-//   pre-smoothing  one red-black Gauss-Seidel sweep (red, then black) from 0
+//   pre-smoothing  NSW red-black Gauss-Seidel sweeps (red, then black) from 0
 //   restriction    R = P^T / 4 (full weighting) of the residual
 //   coarse level   recursion; rediscretised operator (h doubles per level)
 //   prolongation   bilinear P, added
-//   post-smoothing one sweep in the adjoint order (black, then red)
-// Each grid level costs two launches: k_mg_down (red, black, residual,
-// restriction on a 32x32 tile whose 3-wide halo is recomputed locally) and
-// k_mg_up (prolongation, black, red with a 2-wide halo).  Recomputing the
-// halo reproduces global red-black Gauss-Seidel exactly.  All levels with at
+//   post-smoothing NSW sweeps in the adjoint order (black, then red)
+// Each grid level costs two launches: k_mg_down (the pre-sweeps, residual
+// and restriction on a 32x32 tile whose (2 NSW + 1)-wide halo is recomputed
+// locally) and k_mg_up (prolongation and the post-sweeps with a 2 NSW-wide
+// halo).  Recomputing the halo reproduces global red-black Gauss-Seidel
+// exactly.  All levels with at
 // most 4096 unknowns run inside one workgroup in LDS (k_mg_bottom).
 //
 // Outer loop in FP64: r = b - S x, stop at ||r||/||b|| <= tol; inner
@@ -23,14 +24,19 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <vector>
 
 namespace {
 
 constexpr int TB = 32;                 // tile of unknowns per workgroup
-constexpr int HD = 3, HU = 2;          // halos of the down / up kernels
-constexpr int WD = TB + 2 * HD, WU = TB + 2 * HU;
+constexpr int HU = 2, WU = TB + 2 * HU;   // window of k_rb_sweep (one full sweep)
+#ifndef MG_SWEEPS
+#define MG_SWEEPS 1
+#endif
+constexpr int NSW = MG_SWEEPS;            // smoothing sweeps per side of the V-cycle
 constexpr int BOT_MAX = 4096;          // unknowns of the largest LDS level
 constexpr int BOT_LEVELS = 12;
+constexpr int DENSE_MAX = 256;         // coarsest level solved by a dense inverse
 constexpr int RB = 256;                // workgroup of the reduction kernels
 
 struct Lev {
@@ -57,6 +63,13 @@ __device__ __forceinline__ float iweight(const Lev &L, int gi, int gj)
 {
     float a = (L.tx == GP_AX_NEU && (gi == 0 || gi == L.mx - 1)) ? 2.0f : 1.0f;
     float b = (L.ty == GP_AX_NEU && (gj == 0 || gj == L.my - 1)) ? 2.0f : 1.0f;
+    return a * b;
+}
+// w = 1 / iweight (1, 1/2 or 1/4: exact, so x * w equals x / iweight)
+__device__ __forceinline__ float rweight(const Lev &L, int gi, int gj)
+{
+    float a = (L.tx == GP_AX_NEU && (gi == 0 || gi == L.mx - 1)) ? 0.5f : 1.0f;
+    float b = (L.ty == GP_AX_NEU && (gj == 0 || gj == L.my - 1)) ? 0.5f : 1.0f;
     return a * b;
 }
 // red/black colour; on a periodic axis the extent is even (checked when the
@@ -106,57 +119,125 @@ __device__ __forceinline__ float prolong(const float *uc, const Lev &F, const Le
     return 0.25f * ((a00 + a10) + (a01 + a11));
 }
 
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_TMP, S_BB, S_MEAN, S_DONE, S_TGT, S_IT, S_MAXIT, S_RZ2, S_NS };
 // every kernel of an iteration returns at once when the device-side
 // convergence flag is set (the host enqueues iterations in batches)
-__device__ __forceinline__ bool done(const double *scal);
+__device__ __forceinline__ bool done(const double *scal) { return scal && scal[S_DONE] != 0.0; }
+
+// the scalar update of a completed reduction (see k_final)
+__device__ __forceinline__ void apply_op(double *scal, int slot, int op, double s)
+{
+    scal[slot] = s;
+    if (op == 1) scal[S_ALPHA] = s != 0.0 ? scal[S_RZ] / s : 0.0;
+}
+// Sum of np reduction partials in a fixed order, computed by a whole block
+// (threads 0..255 read, every thread passes the barriers and receives the
+// result).  Every block of a grid gets the same bits, which lets the kernel
+// that consumes a reduction finish it itself instead of a separate launch.
+__device__ double sum_parts(const double *part, int np, double *sh, int tid)
+{
+    double a = 0.0;
+    if (tid < 256)
+        for (int k = tid; k < np; k += 256) a += part[k];
+    if (tid < 256) sh[tid] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) sh[tid] += sh[tid + w];
+        __syncthreads();
+    }
+    const double s = sh[0];
+    __syncthreads();
+    return s;
+}
+// Residual test of PCG iteration it (1-based), made by the first kernel of
+// the V-cycle that follows the r update: every block sums the r.r partials;
+// block 0 records ||r||^2 and the count and raises the convergence flag at
+// the target or the iteration limit.  Returns true when converged.
+struct Gate {
+    const double *rr_part;   // null: no test in this launch
+    int np, it;
+    double *scal;
+};
+__device__ bool gate_converged(const Gate &g, double *sh, int tid, bool block0)
+{
+    const double rr = sum_parts(g.rr_part, g.np, sh, tid);
+    const bool conv = !(sqrt(rr) > g.scal[S_TGT]) || (double)g.it >= g.scal[S_MAXIT];
+    if (block0 && tid == 0) {
+        g.scal[S_RR] = rr;
+        g.scal[S_IT] = (double)g.it;
+        if (conv) g.scal[S_DONE] = 1.0;
+    }
+    return conv;
+}
 
 // ---------------------------------------------------------------------------
-// grid level, down leg: u = pre-smoothed from 0; fc = R (f - S u)
-// blockDim (32, 8)
+// grid level, down leg: u = NS symmetric-order red-black sweeps from 0
+// (red, black, red, black, ...); fc = R (f - S u).  blockDim (32, 8).
+// Every half-sweep is computed on a band one point narrower than the last,
+// from a window with a (2 NS + 1)-wide halo, which reproduces the global
+// sweeps exactly on the tile and its 1-wide rim (needed by the restriction).
+// The window's global loads are all issued before any is consumed.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float *fc, Lev L, Lev C, const double *scal)
+template <int NS>
+__global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float *fc, Lev L, Lev C, const double *scal,
+                                                 Gate gt)
 {
+    constexpr int HD = 2 * NS + 1, WD = TB + 2 * HD, NE = (WD * WD + 255) / 256;
     __shared__ float su[WD * WD], sf[WD * WD], sr[(TB + 2) * (TB + 2)];
+    __shared__ double shr[256];
     if (done(scal)) return;
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + 32 * ty;
+    if (gt.rr_part && gate_converged(gt, shr, tid, blockIdx.x == 0 && blockIdx.y == 0)) return;
     const int ti0 = blockIdx.x * TB, tj0 = blockIdx.y * TB;
     const int gi0 = ti0 - HD, gj0 = tj0 - HD;
     const float invd = 1.0f / (2.0f * (L.cx + L.cy)), d = 2.0f * (L.cx + L.cy);
-    for (int lj = ty; lj < WD; lj += 8) {
-        const int gj = gj0 + lj;
-        const bool vj = in_axis(L.ty, gj, L.my);
-        const int wj = wwrap(gj, L.my);
-        for (int li = tx; li < WD; li += 32) {
-            const int gi = gi0 + li, e = li + lj * WD;
-            const bool v = vj && in_axis(L.tx, gi, L.mx);
-            const float ff = v ? f[wwrap(gi, L.mx) + wj * L.mx] : 0.0f;
-            sf[e] = ff;
-            // red from zero (neighbours are black = 0); black starts at 0
-            su[e] = (v && colour(gi, gj) == 0) ? ff * iweight(L, gi, gj) * invd : 0.0f;
+    float fv[NE];
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        const int e = tid + 256 * k;
+        const int li = e % WD, lj = e / WD;
+        const int gi = gi0 + li, gj = gj0 + lj;
+        const bool v = e < WD * WD && in_axis(L.tx, gi, L.mx) && in_axis(L.ty, gj, L.my);
+        fv[k] = v ? f[wwrap(gi, L.mx) + wwrap(gj, L.my) * L.mx] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        const int e = tid + 256 * k;
+        if (e < WD * WD) {
+            const int li = e % WD, lj = e / WD;
+            const int gi = gi0 + li, gj = gj0 + lj;
+            sf[e] = fv[k];
+            // red from zero (its neighbours are black = 0); black starts at 0
+            su[e] = colour(gi, gj) == 0 ? fv[k] * iweight(L, gi, gj) * invd : 0.0f;
         }
     }
     __syncthreads();
-    // black on the tile + 2
-    for (int lj = 1 + ty; lj < WD - 1; lj += 8) {
-        const int gj = gj0 + lj;
-        if (!in_axis(L.ty, gj, L.my)) continue;
-        for (int li = 1 + tx; li < WD - 1; li += 32) {
-            const int gi = gi0 + li, k = li + lj * WD;
-            if (in_axis(L.tx, gi, L.mx) && colour(gi, gj) == 1)
-                su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WD, L, k, gi, gj)) * invd;
+    // half-sweeps 1 .. 2NS-1 (black, red, ...) on bands 1 .. 2NS-1
+#pragma unroll
+    for (int h = 1; h < 2 * NS; h++) {
+        const int col = h & 1;   // 1 black, 0 red
+        for (int lj = h + ty; lj < WD - h; lj += 8) {
+            const int gj = gj0 + lj;
+            if (!in_axis(L.ty, gj, L.my)) continue;
+            for (int li = h + tx; li < WD - h; li += 32) {
+                const int gi = gi0 + li, k = li + lj * WD;
+                if (in_axis(L.tx, gi, L.mx) && colour(gi, gj) == col)
+                    su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WD, L, k, gi, gj)) * invd;
+            }
         }
+        __syncthreads();
     }
-    __syncthreads();
-    // residual on the tile + 1
-    for (int lj = 2 + ty; lj < WD - 2; lj += 8) {
+    // residual on band 2NS (the tile and its rim)
+    constexpr int RB0 = 2 * NS;
+    for (int lj = RB0 + ty; lj < WD - RB0; lj += 8) {
         const int gj = gj0 + lj;
         const bool vj = in_axis(L.ty, gj, L.my);
-        for (int li = 2 + tx; li < WD - 2; li += 32) {
+        for (int li = RB0 + tx; li < WD - RB0; li += 32) {
             const int gi = gi0 + li, k = li + lj * WD;
             float r = 0.0f;
             if (vj && in_axis(L.tx, gi, L.mx))
-                r = sf[k] - (d * su[k] - nsum(su, WD, L, k, gi, gj)) / iweight(L, gi, gj);
-            sr[(li - 2) + (lj - 2) * (TB + 2)] = r;
+                r = sf[k] - (d * su[k] - nsum(su, WD, L, k, gi, gj)) * rweight(L, gi, gj);
+            sr[(li - RB0) + (lj - RB0) * (TB + 2)] = r;
         }
     }
     __syncthreads();
@@ -174,8 +255,8 @@ __global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float
         const int J = cj0 + cjj;
         if (J >= C.my) break;
         const int nyc = 2 * c_node(C.ty, J);
-        const int fv = f_unk(L.ty, nyc, L.My);
-        if (fv < tj0 || fv >= tj0 + TB) continue;
+        const int fv2 = f_unk(L.ty, nyc, L.My);
+        if (fv2 < tj0 || fv2 >= tj0 + TB) continue;
         for (int cii = tx; cii < CW; cii += 32) {
             const int I = ci0 + cii;
             if (I >= C.mx) continue;
@@ -190,7 +271,7 @@ __global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float
 #pragma unroll
                 for (int a = -1; a <= 1; a++) {
                     if (L.tx != GP_AX_PER && f_unk(L.tx, nxc + a, L.Mx) < 0) continue;
-                    acc += (a == 0 ? 0.5f : 0.25f) * wy * sr[(fu + a - ti0 + 1) + (fv + b - tj0 + 1) * (TB + 2)];
+                    acc += (a == 0 ? 0.5f : 0.25f) * wy * sr[(fu + a - ti0 + 1) + (fv2 + b - tj0 + 1) * (TB + 2)];
                 }
             }
             fc[I + J * C.mx] = acc;
@@ -199,12 +280,16 @@ __global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float
 }
 
 // ---------------------------------------------------------------------------
-// grid level, up leg: u = upre + P uc; black; red.  Optional partial of f.u
-// (the CG's r.z at the finest level).  blockDim (32, 8)
+// grid level, up leg: u = upre + P uc, then NS sweeps in the adjoint order
+// (black, red, black, red, ...) on bands 1 .. 2NS of a window with a 2NS-wide
+// halo.  Optional partial of f.u (the CG's r.z at the finest level).
+// blockDim (32, 8)
 // ---------------------------------------------------------------------------
+template <int NS>
 __global__ __launch_bounds__(256) void k_mg_up(const float *upre, const float *f, const float *uc, float *u, Lev L, Lev C,
                                                double *part, const double *scal)
 {
+    constexpr int HU = 2 * NS, WU = TB + 2 * HU, NE = (WU * WU + 255) / 256;
     __shared__ float su[WU * WU], sf[WU * WU];
     __shared__ double red[4];
     if (done(scal)) return;
@@ -212,41 +297,40 @@ __global__ __launch_bounds__(256) void k_mg_up(const float *upre, const float *f
     const int ti0 = blockIdx.x * TB, tj0 = blockIdx.y * TB;
     const int gi0 = ti0 - HU, gj0 = tj0 - HU;
     const float invd = 1.0f / (2.0f * (L.cx + L.cy));
-    for (int lj = ty; lj < WU; lj += 8) {
-        const int gj = gj0 + lj;
-        const bool vj = in_axis(L.ty, gj, L.my);
-        const int wj = wwrap(gj, L.my);
-        for (int li = tx; li < WU; li += 32) {
-            const int gi = gi0 + li, e = li + lj * WU;
-            float uu = 0.0f, ff = 0.0f;
-            if (vj && in_axis(L.tx, gi, L.mx)) {
-                const int wi = wwrap(gi, L.mx), k = wi + wj * L.mx;
-                uu = upre[k] + prolong(uc, L, C, wi, wj);
-                ff = f[k];
+    float uv[NE], fvv[NE];
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        const int e = tid + 256 * k;
+        const int li = e % WU, lj = e / WU;
+        const int gi = gi0 + li, gj = gj0 + lj;
+        uv[k] = 0.0f;
+        fvv[k] = 0.0f;
+        if (e < WU * WU && in_axis(L.tx, gi, L.mx) && in_axis(L.ty, gj, L.my)) {
+            const int wi = wwrap(gi, L.mx), wj = wwrap(gj, L.my), kk = wi + wj * L.mx;
+            uv[k] = upre[kk] + prolong(uc, L, C, wi, wj);
+            fvv[k] = f[kk];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NE; k++) {
+        const int e = tid + 256 * k;
+        if (e < WU * WU) { su[e] = uv[k]; sf[e] = fvv[k]; }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 1; h <= 2 * NS; h++) {
+        const int col = h & 1;   // 1 black, 0 red
+        for (int lj = h + ty; lj < WU - h; lj += 8) {
+            const int gj = gj0 + lj;
+            if (!in_axis(L.ty, gj, L.my)) continue;
+            for (int li = h + tx; li < WU - h; li += 32) {
+                const int gi = gi0 + li, k = li + lj * WU;
+                if (in_axis(L.tx, gi, L.mx) && colour(gi, gj) == col)
+                    su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WU, L, k, gi, gj)) * invd;
             }
-            su[e] = uu;
-            sf[e] = ff;
         }
+        __syncthreads();
     }
-    __syncthreads();
-    for (int lj = 1 + ty; lj < WU - 1; lj += 8) {
-        const int gj = gj0 + lj;
-        if (!in_axis(L.ty, gj, L.my)) continue;
-        for (int li = 1 + tx; li < WU - 1; li += 32) {
-            const int gi = gi0 + li, k = li + lj * WU;
-            if (in_axis(L.tx, gi, L.mx) && colour(gi, gj) == 1)
-                su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WU, L, k, gi, gj)) * invd;
-        }
-    }
-    __syncthreads();
-    for (int lj = HU + ty; lj < HU + TB; lj += 8) {
-        const int gj = gj0 + lj;
-        if (!in_axis(L.ty, gj, L.my)) continue;
-        const int li = HU + tx, gi = gi0 + li, k = li + lj * WU;
-        if (in_axis(L.tx, gi, L.mx) && colour(gi, gj) == 0)
-            su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WU, L, k, gi, gj)) * invd;
-    }
-    __syncthreads();
     double acc = 0.0;
     for (int lj = ty; lj < TB; lj += 8) {
         const int gj = tj0 + lj, gi = ti0 + tx;
@@ -260,7 +344,7 @@ __global__ __launch_bounds__(256) void k_mg_up(const float *upre, const float *f
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
         if ((tid & 63) == 0) red[tid >> 6] = acc;
         __syncthreads();
-        if (tid == 0) part[blockIdx.x + blockIdx.y * gridDim.x] = red[0] + red[1] + red[2] + red[3];
+        if (tid == 0) part[blockIdx.x + blockIdx.y * gridDim.x] = (red[0] + red[1]) + (red[2] + red[3]);
     }
 }
 
@@ -321,7 +405,8 @@ struct Bottom {
     int off[BOT_LEVELS];   // LDS offset of level l (u at off, f at off + n)
     int nl;
     int singular;
-    int sweeps;            // symmetric sweep pairs on the coarsest level
+    int sweeps;            // symmetric sweep pairs on the coarsest level (no dense inverse)
+    const float *dense;    // inverse of the coarsest operator, column-major (or null)
 };
 
 __device__ __forceinline__ float gs_point(const float *u, const float *f, const Lev &L, int i, int j, float invd)
@@ -338,7 +423,7 @@ __device__ __forceinline__ float gs_point(const float *u, const float *f, const 
 __device__ __forceinline__ float resid_point(const float *u, const float *f, const Lev &L, int i, int j, float invd)
 {
     const float g = gs_point(u, f, L, i, j, invd);
-    return (g - u[i + j * L.mx]) / (invd * iweight(L, i, j));
+    return (g - u[i + j * L.mx]) * (2.0f * (L.cx + L.cy)) * rweight(L, i, j);
 }
 #define BOT_LOOP(L) \
     for (int j = threadIdx.y; j < (L).my; j += 32) \
@@ -370,18 +455,30 @@ __device__ void bot_project(float *a, const Lev &L, float *scratch)
 }
 
 __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uout, Bottom B, double *part,
-                                                    const double *scal)
+                                                    const double *scal, Gate gt)
 {
     extern __shared__ float lds[];
     __shared__ float scratch[1024];
     __shared__ double dred[16];
+    __shared__ double shr[256];
     if (done(scal)) return;
     const int tid = threadIdx.x + 32 * threadIdx.y;
+    if (gt.rr_part && gate_converged(gt, shr, tid, true)) return;
     {
         const Lev &L = B.lv[0];
         const int n = L.mx * L.my;
         float *f = lds + B.off[0] + n;
-        for (int k = tid; k < n; k += 1024) f[k] = fin[k];
+        float v[BOT_MAX / 1024];
+#pragma unroll
+        for (int q = 0; q < BOT_MAX / 1024; q++) {
+            const int k = tid + 1024 * q;
+            v[q] = k < n ? fin[k] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < BOT_MAX / 1024; q++) {
+            const int k = tid + 1024 * q;
+            if (k < n) f[k] = v[q];
+        }
     }
     __syncthreads();
     for (int l = 0; l + 1 < B.nl; l++) {
@@ -393,8 +490,7 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         // red from zero, black zero
         BOT_LOOP(L) u[i + j * L.mx] = 0.0f;
         __syncthreads();
-        bot_sweep(u, f, L, 0);
-        bot_sweep(u, f, L, 1);
+        for (int sw = 0; sw < NSW; sw++) { bot_sweep(u, f, L, 0); bot_sweep(u, f, L, 1); }
         BOT_LOOP(C) {
             const int cxn = 2 * c_node(C.tx, i), cyn = 2 * c_node(C.ty, j);
             float acc = 0.0f;
@@ -417,10 +513,35 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         const int n = L.mx * L.my;
         float *u = lds + B.off[B.nl - 1], *f = u + n;
         if (B.singular) bot_project(f, L, scratch);
-        BOT_LOOP(L) u[i + j * L.mx] = 0.0f;
-        __syncthreads();
-        for (int s = 0; s < B.sweeps; s++) { bot_sweep(u, f, L, 0); bot_sweep(u, f, L, 1); }
-        for (int s = 0; s < B.sweeps; s++) { bot_sweep(u, f, L, 1); bot_sweep(u, f, L, 0); }
+        if (B.dense) {
+            // u = M f with the precomputed inverse (n <= 256): thread t sums
+            // quarter t/256 of row t%256, the four quarters add in LDS
+            const int row = tid & 255, qd = tid >> 8;
+            const int j0 = (n * qd) / 4, j1 = (n * (qd + 1)) / 4;
+            // 16 independent loads in flight per step (the matrix streams
+            // from L2; one load per dependent fma would serialise them)
+            float acc = 0.0f;
+            if (row < n) {
+                int jj = j0;
+                for (; jj + 16 <= j1; jj += 16) {
+                    float mv[16];
+#pragma unroll
+                    for (int q = 0; q < 16; q++) mv[q] = B.dense[(jj + q) * n + row];
+#pragma unroll
+                    for (int q = 0; q < 16; q++) acc = __builtin_fmaf(mv[q], f[jj + q], acc);
+                }
+                for (; jj < j1; jj++) acc = __builtin_fmaf(B.dense[jj * n + row], f[jj], acc);
+            }
+            scratch[tid] = acc;
+            __syncthreads();
+            if (tid < n) u[tid] = (scratch[tid] + scratch[tid + 256]) + (scratch[tid + 512] + scratch[tid + 768]);
+            __syncthreads();
+        } else {
+            BOT_LOOP(L) u[i + j * L.mx] = 0.0f;
+            __syncthreads();
+            for (int s = 0; s < B.sweeps; s++) { bot_sweep(u, f, L, 0); bot_sweep(u, f, L, 1); }
+            for (int s = 0; s < B.sweeps; s++) { bot_sweep(u, f, L, 1); bot_sweep(u, f, L, 0); }
+        }
         if (B.singular) bot_project(u, L, scratch);
     }
     for (int l = B.nl - 2; l >= 0; l--) {
@@ -430,8 +551,7 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         const float *uc = lds + B.off[l + 1];
         BOT_LOOP(L) u[i + j * L.mx] += prolong(uc, L, C, i, j);
         __syncthreads();
-        bot_sweep(u, f, L, 1);
-        bot_sweep(u, f, L, 0);
+        for (int sw = 0; sw < NSW; sw++) { bot_sweep(u, f, L, 1); bot_sweep(u, f, L, 0); }
     }
     {
         const Lev &L = B.lv[0];
@@ -468,12 +588,8 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
         for (int w = 0; w < RB / 64; w++) s += sh[w];
     return s;
 }
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_TMP, S_BB, S_MEAN, S_DONE, S_TGT, S_IT, S_MAXIT, S_NS };
-__device__ __forceinline__ bool done(const double *scal) { return scal && scal[S_DONE] != 0.0; }
-// final reduction of partials into scal[slot]; op 1: alpha = rz / sum; op 2:
-// beta = sum / rz, rz = sum; op 3: residual norm of a CG iteration -- count
-// it and raise the convergence flag at the target or the iteration limit.
-// gate != 0: skip once the flag is up.
+// final reduction of partials into scal[slot]; op 1 also sets
+// alpha = rz / sum.  gate != 0: skip once the convergence flag is up.
 __global__ void k_final(const double *part, int nb, double *scal, int slot, int op, int gate = 0)
 {
     __shared__ double sh[RB];
@@ -486,18 +602,16 @@ __global__ void k_final(const double *part, int nb, double *scal, int slot, int 
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const double s = sh[0];
-        scal[slot] = s;
-        if (op == 1) scal[S_ALPHA] = s != 0.0 ? scal[S_RZ] / s : 0.0;
-        if (op == 2) { scal[S_BETA] = scal[S_RZ] != 0.0 ? s / scal[S_RZ] : 0.0; scal[S_RZ] = s; }
-        if (op == 3) {
-            scal[S_IT] += 1.0;
-            if (!(sqrt(s) > scal[S_TGT]) || scal[S_IT] >= scal[S_MAXIT]) scal[S_DONE] = 1.0;
-        }
-    }
+    if (threadIdx.x == 0) apply_op(scal, slot, op, sh[0]);
 }
 // start of an inner solve: target = tol * ||r0||, flag up at once if r0 = 0
+// the residual test alone (a hierarchy without grid levels above its
+// coarsest, solved by sweeps)
+__global__ void k_gate(Gate g)
+{
+    __shared__ double shr[256];
+    (void)gate_converged(g, shr, threadIdx.x, true);
+}
 __global__ void k_start(double *scal, double inner_tol, int maxit)
 {
     scal[S_TGT] = inner_tol * sqrt(scal[S_RR]);
@@ -506,12 +620,23 @@ __global__ void k_start(double *scal, double inner_tol, int maxit)
     scal[S_MAXIT] = (double)maxit;
 }
 // p_new = z + beta p_old (beta from scal), q = S p_new, partial p_new.q
+// Iteration it of the PCG: rz_it = r.z from the partials of the V-cycle
+// (it > 1), beta = rz_it / rz_(it-1); block 0 stores rz_it in slot cur
+// (the previous value stays in slot prev, which the other blocks read).
 __global__ __launch_bounds__(RB) void k_pupdate_spmv(const float *z, const float *pold, float *pnew, float *q, Lev L,
-                                                    const double *scal, double *part)
+                                                    double *scal, double *part, const double *rz_part, int nrz,
+                                                    int first, int cur, int prev)
 {
     __shared__ double sh[RB / 64];
+    __shared__ double shr[256];
     if (done(scal)) return;
-    const float beta = (float)scal[S_BETA];
+    float beta = 0.0f;
+    if (!first) {
+        const double rz = sum_parts(rz_part, nrz, shr, threadIdx.x);
+        const double ro = scal[prev];
+        beta = ro != 0.0 ? (float)(rz / ro) : 0.0f;
+        if (blockIdx.x == 0 && threadIdx.x == 0) scal[cur] = rz;
+    }
     const int n = L.mx * L.my;
     double acc = 0.0;
     for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
@@ -533,12 +658,22 @@ __global__ __launch_bounds__(RB) void k_pupdate_spmv(const float *z, const float
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 // e += alpha p; r -= alpha q; partial r.r
+// alpha = scal[S_ALPHA], or, with pq_part, rz (slot rzs) / (sum of the p.q
+// partials), reduced by every block itself
 __global__ __launch_bounds__(RB) void k_update_xr(float *x, float *r, const float *p, const float *q, int n,
-                                                 const double *scal, double *part)
+                                                 const double *scal, double *part, const double *pq_part = nullptr,
+                                                 int npq = 0, int rzs = S_RZ)
 {
     __shared__ double sh[RB / 64];
+    __shared__ double shr[256];
     if (done(scal)) return;
-    const float alpha = (float)scal[S_ALPHA];
+    float alpha;
+    if (pq_part) {
+        const double pq = sum_parts(pq_part, npq, shr, threadIdx.x);
+        alpha = pq != 0.0 ? (float)(scal[rzs] / pq) : 0.0f;
+    } else {
+        alpha = (float)scal[S_ALPHA];
+    }
     double acc = 0.0;
     for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
         x[k] = __builtin_fmaf(alpha, p[k], x[k]);
@@ -673,14 +808,19 @@ inline dim3 tiles(const Lev &L) { return dim3((L.mx + TB - 1) / TB, (L.my + TB -
 // z = M r (s.lv[0].f aliases r, s.lv[0].u aliases z); part: partial r.z.
 // scal (may be null) carries the convergence flag that makes every kernel a
 // no-op once the CG has converged.
-int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double *scal)
+// gate (may be null): the residual test of the PCG iteration, made by the
+// V-cycle's first kernel
+int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double *scal, const Gate *gate = nullptr)
 {
+    const Gate nog{nullptr, 0, 0, nullptr};
+    bool gated = false;
+    auto take = [&]() { const Gate g = (gate && !gated) ? *gate : nog; gated = true; return g; };
     const int nl = (int)s.lv.size();
     const int lb = s.bottom;   // first level handled by k_mg_bottom (nl if none)
     const dim3 blk(32, 8);
     for (int l = 0; l < std::min(lb, nl - 1); l++) {
         const Lev L = mklev(s.lv[l]), C = mklev(s.lv[l + 1]);
-        k_mg_down<<<tiles(L), blk, 0, st>>>(s.lv[l].f, s.lv[l].r, s.lv[l + 1].f, L, C, scal);
+        k_mg_down<NSW><<<tiles(L), blk, 0, st>>>(s.lv[l].f, s.lv[l].r, s.lv[l + 1].f, L, C, scal, take());
     }
     GP_HIP(hipGetLastError());
     *nparts = 0;
@@ -696,9 +836,10 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
         const Lev &Lc = B.lv[B.nl - 1];
         B.singular = s.singular;
         B.sweeps = std::max(2, std::max(Lc.mx, Lc.my) / 2);
+        B.dense = s.dinv;
         const bool top = lb == 0;
         k_mg_bottom<<<1, dim3(32, 32), off * sizeof(float), st>>>(s.lv[lb].f, s.lv[lb].u, B, top ? part : nullptr,
-                                                                  scal);
+                                                                  scal, take());
         GP_HIP(hipGetLastError());
         if (top) *nparts = 1;
     } else {
@@ -708,6 +849,10 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
         const Lev L = mklev(s.lv[nl - 1]);
         const int sweeps = std::min(400, std::max(4, 2 * std::max(L.mx, L.my)));
         float *a = s.lv[nl - 1].u, *b = s.lv[nl - 1].r;
+        if (gate && !gated) {   // no grid level above: test the residual on its own
+            k_gate<<<1, 256, 0, st>>>(*gate);
+            gated = true;
+        }
         k_zero_f<<<256, 256, 0, st>>>(a, L.mx * L.my);
         for (int k = 0; k < 2 * sweeps; k++) {
             k_rb_sweep<<<tiles(L), blk, 0, st>>>(a, s.lv[nl - 1].f, b, L, k < sweeps ? 0 : 1, scal);
@@ -723,8 +868,8 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
     for (int l = std::min(lb, nl - 1) - 1; l >= 0; l--) {
         const Lev L = mklev(s.lv[l]), C = mklev(s.lv[l + 1]);
         const dim3 g = tiles(L);
-        k_mg_up<<<g, blk, 0, st>>>(s.lv[l].r, s.lv[l].f, s.lv[l + 1].u, s.lv[l].u, L, C, l == 0 ? part : nullptr,
-                                   scal);
+        k_mg_up<NSW><<<g, blk, 0, st>>>(s.lv[l].r, s.lv[l].f, s.lv[l + 1].u, s.lv[l].u, L, C, l == 0 ? part : nullptr,
+                                        scal);
         if (l == 0) *nparts = (int)(g.x * g.y);
     }
     GP_HIP(hipGetLastError());
@@ -743,7 +888,6 @@ int pcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int *iters)
     float *p = s.p, *p2 = s.q2;
     k_zero_f<<<ge, 256, 0, st>>>(s.e, n);
     k_zero_f<<<ge, 256, 0, st>>>(p, n);
-    k_set_scalar<<<1, 1, 0, st>>>(s.scal, S_BETA, 0.0);
     k_set_scalar<<<1, 1, 0, st>>>(s.scal, S_DONE, 0.0);
     double *red_pq = s.red, *red_rr = s.red + 1024, *red_rz = s.red + 2048;
     int np = 0;
@@ -756,17 +900,20 @@ int pcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int *iters)
     GP_HIP(hipGetLastError());
     int enq = 0;
     // first batch: the previous solve's iteration count (one more if it
-    // ran short), then single iterations
+    // ran short), then single iterations.  Each kernel finishes the
+    // reduction it consumes (beta in the p update, alpha in the x/r update,
+    // the residual test in the V-cycle's first kernel), so an iteration is
+    // one launch per kernel with no separate reduction launches.
     for (int batch = std::max(2, s.last_it);; batch = 1) {
         for (int b = 0; b < batch && enq < maxit; b++, enq++) {
-            k_pupdate_spmv<<<nb, RB, 0, st>>>(s.z, p, p2, s.q, L0, s.scal, red_pq);
+            const int it = enq + 1;
+            const int cur = (it & 1) ? S_RZ : S_RZ2, prev = (it & 1) ? S_RZ2 : S_RZ;
+            k_pupdate_spmv<<<nb, RB, 0, st>>>(s.z, p, p2, s.q, L0, s.scal, red_pq, red_rz, np, it == 1, cur, prev);
             std::swap(p, p2);
-            k_final<<<1, RB, 0, st>>>(red_pq, nb, s.scal, S_PQ, 1, 1);
-            k_update_xr<<<nb, RB, 0, st>>>(s.e, s.r, p, s.q, n, s.scal, red_rr);
-            k_final<<<1, RB, 0, st>>>(red_rr, nb, s.scal, S_RR, 3, 1);
-            rc = vcycle(st, s, red_rz, &np, s.scal);
+            k_update_xr<<<nb, RB, 0, st>>>(s.e, s.r, p, s.q, n, s.scal, red_rr, red_pq, nb, cur);
+            const Gate g{red_rr, nb, it, s.scal};
+            rc = vcycle(st, s, red_rz, &np, s.scal, &g);
             if (rc) return rc;
-            k_final<<<1, RB, 0, st>>>(red_rz, np, s.scal, S_TMP, 2, 1);
         }
         GP_HIP(hipGetLastError());
         GP_HIP(hipMemcpyAsync(s.hscal, s.scal, S_NS * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -873,7 +1020,7 @@ int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double 
     auto M = [](int t, int m) { return t == GP_AX_DIR ? m + 1 : (t == GP_AX_NEU ? m - 1 : m); };
     for (;;) {
         const gp_mg_level &f = s.lv.back();
-        if ((int64_t)f.mx * f.my <= 16) break;
+        if ((int64_t)f.mx * f.my <= DENSE_MAX) break;
         const int Mx = M(tx, f.mx), My = M(ty, f.my);
         if ((Mx & 1) || (My & 1) || Mx < 4 || My < 4) break;
         if ((int)s.lv.size() >= 24) break;
@@ -896,8 +1043,6 @@ int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double 
     s.n = n;
     if (!mklev(s.lv[0]).rb)
         return gp_fail(GP_ERR_UNSUPPORTED, "a periodic axis needs an even number of cells (red-black smoothing)");
-    if ((int64_t)s.lv[nl - 1].mx * s.lv[nl - 1].my > BOT_MAX && nl - 1 < s.bottom)
-        ;   // coarsest level solved by grid-kernel sweeps (slow, but valid)
     for (int k = 0; k < nl; k++) {
         const size_t m = (size_t)s.lv[k].mx * s.lv[k].my;
         if (k > 0) {
@@ -922,6 +1067,68 @@ int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double 
     const int ntile0 = ((mx + TB - 1) / TB) * ((my + TB - 1) / TB);
     GP_HIP(hipMalloc((void **)&s.red, (2048 + std::max(ntile0, 1024) + 64) * sizeof(double)));
     GP_HIP(hipMalloc((void **)&s.scal, S_NS * sizeof(double)));
+    // the coarsest level, when it is small and inside the LDS bottom solver,
+    // is solved exactly: the inverse of its operator (the one the smoother
+    // uses: S = W (d I - neighbours), W the Neumann row weights), regularised
+    // by (lambda/n) 1 1' on a singular level, which leaves zero-mean
+    // right-hand sides their zero-mean solution
+    {
+        const gp_mg_level &c = s.lv[nl - 1];
+        const int m = c.mx * c.my;
+        if (s.bottom < nl && m <= DENSE_MAX) {
+            std::vector<double> Sm((size_t)m * m, 0.0), Inv((size_t)m * m, 0.0);
+            const Lev L = mklev(c);
+            const double cxl = c.hx2i, cyl = c.hy2i, dg = 2.0 * (cxl + cyl);
+            auto nbr = [&](int t, int g, int mm) { return t == GP_AX_PER ? (g + mm) % mm : (t == GP_AX_NEU ? (g < 0 ? 1 : (g >= mm ? mm - 2 : g)) : g); };
+            for (int j = 0; j < c.my; j++)
+                for (int i = 0; i < c.mx; i++) {
+                    const int k = i + j * c.mx;
+                    const double w = ((L.tx == GP_AX_NEU && (i == 0 || i == c.mx - 1)) ? 0.5 : 1.0) *
+                                     ((L.ty == GP_AX_NEU && (j == 0 || j == c.my - 1)) ? 0.5 : 1.0);
+                    Sm[(size_t)k * m + k] += w * dg;
+                    const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+                    for (int q = 0; q < 4; q++) {
+                        int ii = i + di[q], jj = j + dj[q];
+                        if (di[q]) { if (L.tx == GP_AX_DIR && (ii < 0 || ii >= c.mx)) continue; ii = nbr(L.tx, ii, c.mx); }
+                        if (dj[q]) { if (L.ty == GP_AX_DIR && (jj < 0 || jj >= c.my)) continue; jj = nbr(L.ty, jj, c.my); }
+                        Sm[(size_t)k * m + ii + (size_t)jj * c.mx] -= w * (di[q] ? cxl : cyl);
+                    }
+                }
+            const bool sing = (L.tx != GP_AX_DIR) && (L.ty != GP_AX_DIR);
+            if (sing)
+                for (size_t e = 0; e < (size_t)m * m; e++) Sm[e] += dg / m;
+            // Gauss-Jordan with partial pivoting (FP64)
+            for (int k = 0; k < m; k++) Inv[(size_t)k * m + k] = 1.0;
+            for (int col = 0; col < m; col++) {
+                int piv = col;
+                for (int r = col + 1; r < m; r++)
+                    if (std::fabs(Sm[(size_t)r * m + col]) > std::fabs(Sm[(size_t)piv * m + col])) piv = r;
+                if (!(std::fabs(Sm[(size_t)piv * m + col]) > 0.0)) return gp_fail(GP_ERR_STATE, "singular coarsest operator");
+                if (piv != col)
+                    for (int q = 0; q < m; q++) {
+                        std::swap(Sm[(size_t)piv * m + q], Sm[(size_t)col * m + q]);
+                        std::swap(Inv[(size_t)piv * m + q], Inv[(size_t)col * m + q]);
+                    }
+                const double pv = 1.0 / Sm[(size_t)col * m + col];
+                for (int q = 0; q < m; q++) { Sm[(size_t)col * m + q] *= pv; Inv[(size_t)col * m + q] *= pv; }
+                for (int r = 0; r < m; r++) {
+                    if (r == col) continue;
+                    const double fct = Sm[(size_t)r * m + col];
+                    if (fct == 0.0) continue;
+                    for (int q = 0; q < m; q++) {
+                        Sm[(size_t)r * m + q] -= fct * Sm[(size_t)col * m + q];
+                        Inv[(size_t)r * m + q] -= fct * Inv[(size_t)col * m + q];
+                    }
+                }
+            }
+            std::vector<float> cm((size_t)m * m);   // column-major: cm[j*m + i] = Inv[i][j]
+            for (int i = 0; i < m; i++)
+                for (int j = 0; j < m; j++) cm[(size_t)j * m + i] = (float)Inv[(size_t)i * m + j];
+            GP_HIP(hipMalloc((void **)&s.dinv, (size_t)m * m * sizeof(float)));
+            GP_HIP(hipMemcpy(s.dinv, cm.data(), (size_t)m * m * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
+
     GP_HIP(hipHostMalloc((void **)&s.hscal, S_NS * sizeof(double), hipHostMallocDefault));
     return GP_OK;
 }
@@ -933,7 +1140,7 @@ void gp_mg_free(gp_solver &s)
         (void)hipFree(s.lv[k].r);
     }
     s.lv.clear();
-    void *ps[] = {s.b64, s.x64, s.r64, s.rhs_bc, s.r, s.z, s.p, s.q, s.q2, s.e, s.red, s.scal};
+    void *ps[] = {s.b64, s.x64, s.r64, s.rhs_bc, s.r, s.z, s.p, s.q, s.q2, s.e, s.red, s.scal, s.dinv};
     for (void *p : ps) if (p) (void)hipFree(p);
     if (s.hscal) (void)hipHostFree(s.hscal);
     s = gp_solver();
