@@ -7,7 +7,9 @@ omega_ce/omega_pe = 0.5, electrostatic, dx = 0.5 Debye length,
 dt = 0.1/omega_pe, FP32 particles.  A step is one cycle of the reference's
 loop (main.mm:154-254): Poisson solve (GPU MG-CG to a 1e-8 relative
 residual, FP64 refinement) -> fused gather + Boris push + int64 deposit of
-both species -> cell/tile sort every 20 steps.  Particles are generated on
+both species -> tile sort of a species when its last push gathered more
+than a fraction (default 1e-4) of its particles outside their tile windows
+(--sort-every N adds a fixed interval).  Particles are generated on
 the device (seeded PCG32 streams), so inputs are resident in HBM when the
 timed region starts.
 
@@ -67,16 +69,18 @@ def gpu_run(args):
     ctx.load_pair(se, si, G, 0, 0, n)
     ctx.sort(se)
     ctx.sort(si)
+    ctx.set_sort_drift(args.sort_drift)
     ctx.reset_grids()
     ctx.deposit(se)
     ctx.deposit(si)
-    dt, tol, sort_every = P["dt"], 1e-8, 20
+    dt, tol, sort_every = P["dt"], 1e-8, args.sort_every
     cyc = 0
     for _ in range(args.warmup):
         cyc += 1
         ctx.step(dt, tol=tol, maxiter=400, sort_interval=sort_every, cycle=cyc)
     ctx.sync()
     ctx.timing(True)
+    sorts0 = [ctx.sort_stats(se)[0], ctx.sort_stats(si)[0]]
     iters = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -85,6 +89,7 @@ def gpu_run(args):
         iters.append(st.iterations)
     ctx.sync()
     t1 = time.perf_counter()
+    sorts = [ctx.sort_stats(se)[0] - sorts0[0], ctx.sort_stats(si)[0] - sorts0[1]]
     push_ms, push_n = ctx.kernel_time(0)
     sort_ms, sort_n = ctx.kernel_time(1)
     solve_ms, solve_n = ctx.kernel_time(2)
@@ -94,7 +99,7 @@ def gpu_run(args):
     nnrep = None if args.no_nn else nn_poisson(ctx, args, dt, tol, cyc)
     ctx.close()
     return dict(n_total=nn, seconds=t1 - t0, push_ms=push_ms, push_n=push_n, sort_ms=sort_ms, sort_n=sort_n,
-                solve_ms=solve_ms, solve_n=solve_n, iters=iters, P=P, n_species=n, nn=nnrep)
+                solve_ms=solve_ms, solve_n=solve_n, iters=iters, P=P, n_species=n, nn=nnrep, sorts=sorts)
 
 
 def nn_poisson(ctx, args, dt, tol, cyc):
@@ -108,7 +113,7 @@ def nn_poisson(ctx, args, dt, tol, cyc):
     ctx.nn_init(32, seed=1)
     for _ in range(args.nn_sim_pairs):
         cyc += 1
-        ctx.step(dt, tol=tol, maxiter=400, sort_interval=20, cycle=cyc)
+        ctx.step(dt, tol=tol, maxiter=400, sort_interval=args.sort_every, cycle=cyc)
         ctx.nn_add_pair()
     add_random_pairs(ctx, args.nn_random_pairs, np.random.default_rng(7), True, True, tol=tol)
     t1 = time.perf_counter()
@@ -123,7 +128,7 @@ def nn_poisson(ctx, args, dt, tol, cyc):
     for _ in range(args.nn_eval_fields):
         xprev = ctx.field(gp.F_SOL).ravel().copy()
         cyc += 1
-        ctx.step(dt, tol=tol, maxiter=400, sort_interval=20, cycle=cyc)
+        ctx.step(dt, tol=tol, maxiter=400, sort_interval=args.sort_every, cycle=cyc)
         b = ctx.field(gp.F_RHS).ravel().copy()
         for key, solver, x0 in (("mg_cold", 0, None), ("mg_warm", 0, xprev), ("nn_warm_fcg", 1, None),
                                 ("nn_warm_mg", 2, None), ("nn_fcg", 3, None)):
@@ -220,6 +225,9 @@ def main():
     ap.add_argument("--ngx", type=int, default=1024)
     ap.add_argument("--ppc", type=int, default=128)
     ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--sort-every", type=int, default=0, help="fixed tile-sort interval (0: none)")
+    ap.add_argument("--sort-drift", type=float, default=1e-4,
+                    help="sort a species when this fraction of it drifted out of its tile windows (0: off)")
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -260,11 +268,13 @@ def main():
         "config": {"workload": f"PIC step {args.ngx}x{args.ngx} periodic, order {args.order} B-spline, "
                                f"{args.ppc} ppc x 2 species ({r['n_total']:,} particles), uniform Bz "
                                f"(wce/wpe=0.5), electrostatic, Poisson MG-CG to 1e-8 (FP64 refined), "
-                               f"tile sort every 20 steps",
+                               + (f"tile sort every {args.sort_every} steps, " if args.sort_every else "")
+                               + f"tile sort of a species at drift fraction > {args.sort_drift:g}",
                    "global_batch": r["n_total"], "parallelism": "single GPU"},
         "roofline": roof,
         "breakdown_ms_per_step": {"push_deposit": r["push_ms"] / args.steps, "sort": r["sort_ms"] / args.steps,
                                   "field_solve": r["solve_ms"] / args.steps},
+        "sorts_timed": {"electrons": r["sorts"][0], "ions": r["sorts"][1]},
         "poisson": {"mg_cg_iterations_mean": float(np.mean(r["iters"])), "ms_per_solve":
                     r["solve_ms"] / max(1, r["solve_n"]), "nn_pcg": r["nn"]},
     }

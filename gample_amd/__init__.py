@@ -87,7 +87,7 @@ EXPORTS = [
     "gp_ctx_create", "gp_ctx_destroy", "gp_last_error", "gp_sync", "gp_dims",
     "gp_species_create", "gp_species_upload", "gp_species_download", "gp_species_count",
     "gp_species_grid_get", "gp_load_pair", "gp_inject_wall", "gp_reset_grids", "gp_deposit",
-    "gp_solve_fields", "gp_solve_darwin", "gp_push", "gp_reduce", "gp_sort", "gp_moments",
+    "gp_solve_fields", "gp_solve_darwin", "gp_push", "gp_reduce", "gp_sort", "gp_set_sort_drift", "gp_sort_stats", "gp_moments",
     "gp_step", "gp_field_get", "gp_field_set", "gp_field_energy", "gp_timing", "gp_kernel_time",
     "gp_nn_init", "gp_nn_train", "gp_nn_infer", "gp_nn_save", "gp_nn_load", "gp_nn_add_pair", "gp_nn_params",
     "gp_nn_dims", "gp_nn_forward", "gp_nn_grad", "gp_solve_rhs", "gp_nn_config",
@@ -124,6 +124,8 @@ def lib() -> C.CDLL:
         L.gp_push.argtypes = [P, I, D, I]
         L.gp_reduce.argtypes = [P, I, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         L.gp_sort.argtypes = [P, I]
+        L.gp_set_sort_drift.argtypes = [P, D]
+        L.gp_sort_stats.argtypes = [P, I, C.POINTER(C.c_int64)]
         L.gp_moments.argtypes = [P, I, C.POINTER(C.c_double)]
         L.gp_step.argtypes = [P, D, D, I, I, I, I, C.POINTER(SolveStats), C.POINTER(Diag)]
         L.gp_field_get.argtypes = [P, I, P, I64]
@@ -282,6 +284,17 @@ class Context:
 
     def sort(self, sid):
         _check(lib().gp_sort(self.h, sid))
+
+    def set_sort_drift(self, fraction):
+        """gp_step sorts a species when its last push gathered more than
+        fraction * n particles outside their tile windows (0: off)"""
+        _check(lib().gp_set_sort_drift(self.h, fraction))
+
+    def sort_stats(self, sid):
+        """(sorts so far, drifted particles of the last push read back)"""
+        out = (C.c_int64 * 2)()
+        _check(lib().gp_sort_stats(self.h, sid, out))
+        return out[0], out[1]
 
     def moments(self, sid):
         a = np.empty((10, self.ny + 1, self.nx + 1), np.float64)

@@ -96,6 +96,9 @@ int gp_ctx_create(const gp_config *cfg, gp_ctx **out)
         if (e != hipSuccess) { delete c; return gp_fail(GP_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
         int rc = gp_fields_init(c);
         if (rc) { gp_ctx_destroy(c); return rc; }
+        e = hipHostMalloc((void **)&c->h_drift, 2 * sizeof(unsigned long long) * GP_MAX_DRIFT_SPECIES);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->drift_ev, hipEventDisableTiming);
+        if (e != hipSuccess) { gp_ctx_destroy(c); return gp_fail(GP_ERR_HIP, std::string("drift counters: ") + hipGetErrorString(e)); }
         *out = c;
         return GP_OK;
     })
@@ -115,6 +118,11 @@ int gp_ctx_destroy(gp_ctx *c)
         for (hipEvent_t e : v) (void)hipEventDestroy(e);
     gp_nn_destroy(c);
     gp_fields_free(c);
+    if (c->stage) (void)hipFree(c->stage);
+    if (c->stage_on) (void)hipFree(c->stage_on);
+    if (c->h_drift) (void)hipHostFree(c->h_drift);
+    if (c->drift_ev) (void)hipEventDestroy(c->drift_ev);
+    if (c->boff) (void)hipFree(c->boff);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -383,9 +391,34 @@ int gp_step(gp_ctx *c, double dt, double tol, int maxiter, int solver, int sort_
     }
     bool has_delete = false;
     for (int w = 0; w < 4; w++) has_delete |= c->cfg.pbc[w] == GP_PBC_DELETE;
-    for (int s = 0; s < (int)c->sp.size(); s++) {
+    const int nsp = (int)c->sp.size();
+    if (c->drift_pending) {
+        // drift counts of the previous step's pushes (copied after them; the
+        // field solve has synchronised the stream since)
+        GP_HIP(hipEventSynchronize(c->drift_ev));
+        c->drift_pending = 0;
+        for (int s = 0; s < nsp && s < GP_MAX_DRIFT_SPECIES; s++) {
+            gp_species &S = c->sp[s];
+            S.last_drift = (int64_t)c->h_drift[2 * s];
+            S.last_moved = (int64_t)c->h_drift[2 * s + 1] * (c->ntiles >= 256 ? GP_MOVED_SAMPLE : 1);
+            if (c->sort_drift > 0.0 && (double)S.last_drift > c->sort_drift * (double)S.n) {
+                int rc = gp_launch_sort(c, s);
+                if (rc) return rc;
+            }
+        }
+    }
+    for (int s = 0; s < nsp; s++) {
         int rc = gp_launch_push(c, s, dt, 1);
         if (rc) return rc;
+    }
+    if (c->h_drift) {
+        for (int s = 0; s < nsp && s < GP_MAX_DRIFT_SPECIES; s++) {
+            GP_HIP(hipMemcpyAsync(c->h_drift + 2 * s, c->sp[s].counters + 6, 2 * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, c->stream));
+            GP_HIP(hipMemsetAsync(c->sp[s].counters + 6, 0, 2 * sizeof(unsigned long long), c->stream));
+        }
+        GP_HIP(hipEventRecord(c->drift_ev, c->stream));
+        c->drift_pending = 1;
     }
     for (int s = 0; s < (int)c->sp.size(); s++) {
         int64_t cnt[4] = {0, 0, 0, 0};
@@ -406,6 +439,23 @@ int gp_step(gp_ctx *c, double dt, double tol, int maxiter, int solver, int sort_
             if (rc) return rc;
         }
     return status;
+}
+
+int gp_set_sort_drift(gp_ctx *c, double fraction)
+{
+    GP_CHECK_CTX(c);
+    if (!(fraction >= 0.0) || fraction > 1.0) return gp_fail(GP_ERR_ARG, "sort drift fraction must lie in [0, 1]");
+    c->sort_drift = fraction;
+    return GP_OK;
+}
+
+int gp_sort_stats(gp_ctx *c, int sid, int64_t out[2])
+{
+    GP_CHECK_CTX(c);
+    if (sid < 0 || sid >= (int)c->sp.size() || !out) return gp_fail(GP_ERR_ARG, "bad species or null output");
+    out[0] = c->sp[sid].nsorts;
+    out[1] = c->sp[sid].last_drift;
+    return GP_OK;
 }
 
 int gp_field_get(gp_ctx *c, int which, void *host, int64_t bytes)

@@ -58,12 +58,15 @@ __global__ void k_scale_sum(AccSet s, double *out, int64_t n)
     for (int q = 0; q < s.ns; q++) v += (double)s.a[q][k] * s.f[q];
     out[k] = v;
 }
-// fold (EMField.mm:749-850), one thread per row (x) or column (y)
-__global__ void k_fold_x(double *rho, int ngb, int ngx, int nx, int ny, int bmin, int bmax, int periodic)
+// fold (EMField.mm:749-850), one thread per row (x) or column (y); applied
+// to each species' int64 accumulator before the conversion to FP64, so the
+// folded charge is exact and independent of where a deposit landed
+template <typename T>
+__global__ void k_fold_x(T *rho, int ngb, int ngx, int nx, int ny, int bmin, int bmax, int periodic)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j > ny) return;
-    double *r = rho + (int64_t)j * (nx + 1);
+    T *r = rho + (int64_t)j * (nx + 1);
     if (!periodic) {
         if (bmin == GP_FBC_DIRICHLET) for (int i = 0; i <= ngb; i++) r[ngb + 1] += r[i];
         else if (bmin == GP_FBC_NEUMANN) for (int i = 0; i < ngb; i++) r[ngb] += r[i];
@@ -74,12 +77,13 @@ __global__ void k_fold_x(double *rho, int ngb, int ngx, int nx, int ny, int bmin
         for (int i = 0; i <= ngb; i++) r[ngb + i] += r[ngb + ngx + i];
     }
 }
-__global__ void k_fold_y(double *rho, int ngb, int ngy, int nx, int bmin, int bmax, int periodic)
+template <typename T>
+__global__ void k_fold_y(T *rho, int ngb, int ngy, int nx, int bmin, int bmax, int periodic)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > nx) return;
     const int P = nx + 1;
-    double *r = rho + i;
+    T *r = rho + i;
     if (!periodic) {
         if (bmin == GP_FBC_DIRICHLET) for (int j = 0; j <= ngb; j++) r[(ngb + 1) * P] += r[j * P];
         else if (bmin == GP_FBC_NEUMANN) for (int j = 0; j < ngb; j++) r[ngb * P] += r[j * P];
@@ -404,6 +408,18 @@ int gp_finalize_grids(gp_ctx *c)
         jxs.a[q] = S.jx_acc; jys.a[q] = S.jy_acc;
         jxs.f[q] = jys.f[q] = ldexp(1.0, -GP_CUR_FRAC) * S.vnorm * (double)S.scale;
     }
+    // fold order of EMField.mm:749-850: non-periodic x, non-periodic y,
+    // periodic x, periodic y
+    const int *b = c->cfg.fbc;
+    const int perx = b[0] == GP_FBC_PERIODIC, pery = b[2] == GP_FBC_PERIODIC;
+    for (auto &S : c->sp) {
+        long long *r = (long long *)S.rho_acc;
+        if (!perx) k_fold_x<long long><<<gp_cdiv(c->ny + 1, 64), 64, 0, st>>>(r, c->ngb, c->ngx, c->nx, c->ny, b[0], b[1], 0);
+        if (!pery) k_fold_y<long long><<<gp_cdiv(c->nx + 1, 64), 64, 0, st>>>(r, c->ngb, c->ngy, c->nx, b[2], b[3], 0);
+        if (perx) k_fold_x<long long><<<gp_cdiv(c->ny + 1, 64), 64, 0, st>>>(r, c->ngb, c->ngx, c->nx, c->ny, b[0], b[1], 1);
+        if (pery) k_fold_y<long long><<<gp_cdiv(c->nx + 1, 64), 64, 0, st>>>(r, c->ngb, c->ngy, c->nx, b[2], b[3], 1);
+        GP_HIP(hipGetLastError());
+    }
     k_scale_sum<<<gp_cdiv(nN, 256), 256, 0, st>>>(a, c->rho, nN);
     GP_HIP(hipGetLastError());
     if (c->cfg.cur_dens) {
@@ -418,15 +434,6 @@ int gp_finalize_grids(gp_ctx *c)
             GP_HIP(hipMemsetAsync(S.jy_acc, 0, nEy * 8, st));
         }
     }
-    // fold order of EMField.mm:749-850: non-periodic x, non-periodic y,
-    // periodic x, periodic y
-    const int *b = c->cfg.fbc;
-    const int perx = b[0] == GP_FBC_PERIODIC, pery = b[2] == GP_FBC_PERIODIC;
-    if (!perx) k_fold_x<<<gp_cdiv(c->ny + 1, 64), 64, 0, st>>>(c->rho, c->ngb, c->ngx, c->nx, c->ny, b[0], b[1], 0);
-    if (!pery) k_fold_y<<<gp_cdiv(c->nx + 1, 64), 64, 0, st>>>(c->rho, c->ngb, c->ngy, c->nx, b[2], b[3], 0);
-    if (perx) k_fold_x<<<gp_cdiv(c->ny + 1, 64), 64, 0, st>>>(c->rho, c->ngb, c->ngx, c->nx, c->ny, b[0], b[1], 1);
-    if (pery) k_fold_y<<<gp_cdiv(c->nx + 1, 64), 64, 0, st>>>(c->rho, c->ngb, c->ngy, c->nx, b[2], b[3], 1);
-    GP_HIP(hipGetLastError());
     return GP_OK;
 }
 

@@ -20,8 +20,10 @@
 #include "../../include/gample_amd.h"
 
 #define GP_TILE 16           // tile edge, cells
-#define GP_DRIFT 8           // cells a particle may leave its tile and still use LDS
+#define GP_DRIFT 12          // cells a particle may leave its tile and still use LDS
 #define GP_BLOCK 256         // threads per workgroup of the particle kernels
+#define GP_MAX_DRIFT_SPECIES 64   // species followed by drift-triggered sorting
+#define GP_MOVED_SAMPLE 8         // one tile in 8 counts cell changes (grids of 256+ tiles)
 #define GP_MAX_SPECIES 8
 #define GP_RHO_FRAC 46       // fractional bits of the rho accumulators
 #define GP_CUR_FRAC 40       // fractional bits of current/moment accumulators
@@ -53,7 +55,10 @@ struct gp_species {
     // absorption bookkeeping
     uint32_t *dead = nullptr;         // indices of absorbed particles (cap)
     uint32_t *dead_tmp = nullptr;
-    unsigned long long *counters = nullptr;  // [0] n_dead, [1..4] per wall
+    unsigned long long *counters = nullptr;  // [0] n_dead, [1..4] per wall, [5] escaped, [6] drifted
+    int64_t nsorts = 0;               // sorts so far
+    int64_t last_drift = -1;          // drifted particles of the last push read back (-1: none)
+    int64_t last_moved = -1;          // particles that changed cell in that push (-1: none)
     double *ke_part = nullptr;        // per-workgroup kinetic-energy partials
     int ke_nblk = 0;
     double *ke_out = nullptr;         // device scalar
@@ -123,6 +128,18 @@ struct gp_ctx {
     double t_ms[4] = {0, 0, 0, 0};
     int64_t t_n[4] = {0, 0, 0, 0};
     gp_nn *nn = nullptr;
+    // staged tile windows of the deposit (gp_particles.hip)
+    void *stage = nullptr;
+    uint8_t *stage_on = nullptr;
+    size_t stage_bytes = 0;
+    uint32_t *boff = nullptr;       // sort: first position of each (tile, class) bucket
+    // drift-triggered sorting: a species is sorted before its push when the
+    // previous push gathered more than sort_drift * n of its particles
+    // outside their tile windows (0: off)
+    double sort_drift = 0.0;
+    unsigned long long *h_drift = nullptr;   // pinned, two per species: drifted, moved
+    hipEvent_t drift_ev = nullptr;
+    int drift_pending = 0;
 };
 
 // ---- particle side (gp_particles.hip) ----

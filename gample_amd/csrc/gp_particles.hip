@@ -7,7 +7,10 @@
 //      coalesced SoA loads, gathers (B-spline order 1/3/5), applies the
 //      Boris rotation, moves, applies the particle BCs, and deposits the new
 //      position's charge (and current) into an int64 fixed-point window in
-//      LDS, which it flushes to the global int64 accumulators with atomics.
+//      LDS.  The window is written whole to the tile's slot of a staging
+//      area, and k_stage_reduce adds, for every grid node, the windows of the
+//      tiles that cover it to the global int64 accumulators (plain stores and
+//      one owner per node: no global atomics for binned particles).
 //      Integer addition is associative, so the deposit is bitwise the same
 //      for any particle order and any workgroup schedule.  Particles whose
 //      cell is outside the tile window (drifted since the last sort, or
@@ -134,10 +137,18 @@ struct PushArgs {
     float constE, constB, constX, constY;
     int ngx, ngy, ngb, nx, ny;
     int bc0, bc1, bc2, bc3, boris;
+    // charge windows by position on periodic axes: a particle whose cell
+    // lies one period away from the window deposits into the window at its
+    // image, and the windows are folded onto the interior nodes as the
+    // reference's ghost fold would (EMField.mm:749-850); rho deposit only
+    int imgx, imgy;
+    int moved_mask;   // tiles b with (b & moved_mask) == 0 count cell changes
     unsigned long long *rho, *jx, *jy;
+    unsigned long long *stage;   // per tile: W*W rho window, then jx and jy (DEP 2)
+    uint8_t *stage_on;           // per tile: 1 when its window was written
     float inv_vnorm;
     uint32_t *dead;
-    unsigned long long *counters;
+    unsigned long long *counters;   // [0] dead, [1..4] walls, [5] escaped, [6] drifted, [7] moved
     double *ke_part;
     double ke_coef;
 };
@@ -175,6 +186,10 @@ __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
         if (p1 > a.n) p1 = a.n;
     }
     const int cx0 = tx0 - GP_DRIFT, cy0 = ty0 - GP_DRIFT;   // first window cell
+    // images are needed only by the tiles on the periodic edges
+    const bool img_x = a.imgx && tiled && (b % a.ntx == 0 || b % a.ntx == a.ntx - 1);
+    const bool img_y = a.imgy && tiled && (b < a.ntx || b >= a.ntiles - a.ntx);
+    const bool count_moved = (b & a.moved_mask) == 0;
     const int X0 = cx0 - NGB, Y0 = cy0 - NGB;                // first window node
     const int pex = a.nx + 2, pey = a.nx + 1;
 
@@ -196,6 +211,8 @@ __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
     const float lox = (float)a.ngb, hix = (float)(a.ngb + a.ngx);
     const float loy = (float)a.ngb, hiy = (float)(a.ngb + a.ngy);
     double ke = 0.0;
+    int drift = 0;   // live particles gathered outside their tile window
+    int moved = 0;   // live particles whose cell changed in this push
 
     // The segment is walked in sub-segments of at most 2^28 particles, so that
     // a particle's address is a uniform base plus a 32-bit byte offset (the
@@ -231,6 +248,7 @@ __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
             continue;
         }
         int flag = 0;
+        int oi = 0, oj = 0;   // cell before the push
         if constexpr (PUSH) {
             const float xh = x - 0.5f, yh = y - 0.5f;
             const float fi1 = floorf(xh), fj1 = floorf(y), fi2 = floorf(x), fj2 = floorf(yh);
@@ -242,6 +260,8 @@ __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
             wts<ORDER>(yh - fj2, sy1);
             float Epx, Epy, Bpz;
             const bool inw = tiled && (unsigned)(i2 - cx0) < (unsigned)WIN && (unsigned)(j1 - cy0) < (unsigned)WIN;
+            if constexpr (DEP == 0) drift += inw ? 0 : 1;
+            oi = i2; oj = j1;
             if (inw) {
                 Epx = gather<NW>(sEx + (i1 + 1 - NGB - X0) + (j1 - NGB - Y0) * W, W, sx0, sy0);
                 Epy = gather<NW>(sEy + (i2 - NGB - X0) + (j2 + 1 - NGB - Y0) * W, W, sx1, sy1);
@@ -313,15 +333,22 @@ __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
             }
             const float fi2 = floorf(x), fj1 = floorf(y);
             const int i2 = (int)fi2, j1 = (int)fj1;
+            if constexpr (PUSH)
+                if (count_moved) moved += (i2 != oi || j1 != oj) ? 1 : 0;
             float sx[NW], sy[NW];
             wts<ORDER>(x - fi2, sx);
             wts<ORDER>(y - fj1, sy);
             unsigned qx[NW], qy[NW];
 #pragma unroll
             for (int k = 0; k < NW; k++) { qx[k] = q23(sx[k]); qy[k] = q23(sy[k]); }
-            const bool inw = tiled && (unsigned)(i2 - cx0) < (unsigned)WIN && (unsigned)(j1 - cy0) < (unsigned)WIN;
+            // window cell (the stencil's first node is window node (wi, wj))
+            int wi = i2 - cx0, wj = j1 - cy0;
+            if (img_x) wi += wi < 0 ? a.ngx : (wi >= WIN ? -a.ngx : 0);
+            if (img_y) wj += wj < 0 ? a.ngy : (wj >= WIN ? -a.ngy : 0);
+            const bool inw = tiled && (unsigned)wi < (unsigned)WIN && (unsigned)wj < (unsigned)WIN;
+            if constexpr (PUSH) drift += inw ? 0 : 1;
             if (inw) {
-                unsigned long long *r = sRho + (i2 - NGB - X0) + (j1 - NGB - Y0) * W;
+                unsigned long long *r = sRho + wi + wj * W;
 #pragma unroll
                 for (int j = 0; j < NW; j++)
 #pragma unroll
@@ -369,6 +396,10 @@ __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
         // deterministic workgroup sum of the kinetic energy
         for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off, 64);
         if ((tid & 63) == 0) sKE[tid >> 6] = ke;
+        for (int off = 32; off > 0; off >>= 1) drift += __shfl_down(drift, off, 64);
+        if ((tid & 63) == 0 && drift) atomicAdd(&a.counters[6], (unsigned long long)drift);
+        for (int off = 32; off > 0; off >>= 1) moved += __shfl_down(moved, off, 64);
+        if ((tid & 63) == 0 && moved) atomicAdd(&a.counters[7], (unsigned long long)moved);
     }
     __syncthreads();
     if constexpr (PUSH) {
@@ -379,19 +410,69 @@ __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
         }
     }
     if constexpr (DEP >= 1) {
-        if (tiled && p0 < p1) {
-            for (int e = tid; e < W * W; e += GP_BLOCK) {
-                const int gx = X0 + e % W, gy = Y0 + e / W;
-                const unsigned long long v = sRho[e];
-                if (v && gx >= 0 && gx <= a.nx && gy >= 0 && gy <= a.ny) atomicAdd(&a.rho[gx + (int64_t)gy * pey], v);
-                if constexpr (DEP >= 2) {
-                    const unsigned long long u = sJx[e], w = sJy[e];
-                    if (u && gx >= 0 && gx <= a.nx + 1 && gy >= 0 && gy <= a.ny) atomicAdd(&a.jx[gx + (int64_t)gy * pex], u);
-                    if (w && gx >= 0 && gx <= a.nx && gy >= 0 && gy <= a.ny + 1) atomicAdd(&a.jy[gx + (int64_t)gy * pey], w);
+        if (tiled) {
+            const bool on = p0 < p1;
+            if (tid == 0) a.stage_on[b] = on ? 1 : 0;
+            if (on) {
+                unsigned long long *st = a.stage + (int64_t)b * (W * W) * (DEP >= 2 ? 3 : 1);
+#pragma unroll 1
+                for (int e = tid; e < W * W; e += GP_BLOCK) {
+                    st[e] = sRho[e];
+                    if constexpr (DEP >= 2) { st[W * W + e] = sJx[e]; st[2 * W * W + e] = sJy[e]; }
                 }
             }
         }
     }
+}
+
+// floor(a / 16) for any sign
+__device__ __forceinline__ int fdiv16(int a) { return a >= 0 ? a / GP_TILE : -((-a + GP_TILE - 1) / GP_TILE); }
+
+// Sum of the staged tile windows into the accumulators: node (gx, gy) of
+// component comp (0 rho, 1 jx, 2 jy; extents ext_x x ext_y, row pitch ext_x)
+// lies in the window of tile (tx, ty) when X0(tx) <= gx < X0(tx) + W, with
+// X0(tx) = ngb + GP_TILE tx - GP_DRIFT - NGB (the push kernel's window origin).
+template <int ORDER>
+__global__ __launch_bounds__(GP_BLOCK) void k_stage_reduce(const unsigned long long *stage, const uint8_t *on,
+                                                           int ncomp, int ntx, int nty, int ngb, int ngx, int ngy,
+                                                           int nx, int ny, int imgx, int imgy,
+                                                           unsigned long long *rho, unsigned long long *jx,
+                                                           unsigned long long *jy)
+{
+    constexpr int NGB = ORDER / 2, NW = ORDER + 1, W = WIN + NW;
+    const int comp = blockIdx.z;
+    const int ext_x = nx + 1 + (comp == 1 ? 1 : 0), ext_y = ny + 1 + (comp == 2 ? 1 : 0);
+    const int gx = blockIdx.x * GP_BLOCK + threadIdx.x, gy = blockIdx.y;
+    if (gx >= ext_x || gy >= ext_y) return;
+    unsigned long long *acc = comp == 0 ? rho : (comp == 1 ? jx : jy);
+    const int c0 = ngb - GP_DRIFT - NGB;
+    // window positions that land on this node: the node itself, or on an
+    // imaged (periodic, rho) axis its interior positions one period either
+    // side; a ghost node of an imaged axis receives nothing from the windows
+    int px[3], py[3], npx = 0, npy = 0;
+    if (imgx && comp == 0) {
+        if (gx >= ngb && gx < ngb + ngx) { px[0] = gx - ngx; px[1] = gx; px[2] = gx + ngx; npx = 3; }
+    } else { px[0] = gx; npx = 1; }
+    if (imgy && comp == 0) {
+        if (gy >= ngb && gy < ngb + ngy) { py[0] = gy - ngy; py[1] = gy; py[2] = gy + ngy; npy = 3; }
+    } else { py[0] = gy; npy = 1; }
+    unsigned long long sum = 0;
+    for (int b = 0; b < npy; b++) {
+        const int y = py[b];
+        const int tyl = max(fdiv16(y - c0 - W) + 1, 0), tyh = min(fdiv16(y - c0), nty - 1);
+        for (int a = 0; a < npx; a++) {
+            const int x = px[a];
+            const int txl = max(fdiv16(x - c0 - W) + 1, 0), txh = min(fdiv16(x - c0), ntx - 1);
+            for (int ty = tyl; ty <= tyh; ty++)
+                for (int tx = txl; tx <= txh; tx++) {
+                    const int t = ty * ntx + tx;
+                    if (!on[t]) continue;
+                    const int lx = x - c0 - GP_TILE * tx, ly = y - c0 - GP_TILE * ty;
+                    sum += stage[((int64_t)t * ncomp + comp) * (W * W) + lx + ly * W];
+                }
+        }
+    }
+    if (sum) acc[gx + (int64_t)gy * ext_x] += sum;
 }
 
 __global__ void k_sum_partials(const double *part, int n, double *out)
@@ -507,46 +588,53 @@ int scan_u32(hipStream_t s, uint32_t *d, int64_t n)
 }
 
 // ---------------------------------------------------------------------------
-// stable LSD radix sort of (uint32 key, uint32 value), 8-bit digits
+// stable LSD radix sort of (uint32 key, uint32 value), DIG-bit digits (8 or 9)
 // ---------------------------------------------------------------------------
 constexpr int RS_TILE = 4096;
 
+template <int DIG>
 __global__ __launch_bounds__(GP_BLOCK) void k_rs_hist(const uint32_t *keys, int64_t n, int shift,
                                                       uint32_t *hist, int nblk)
 {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    constexpr int NB = 1 << DIG;
+    __shared__ uint32_t h[NB];
+    for (int d = threadIdx.x; d < NB; d += GP_BLOCK) h[d] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     for (int k = threadIdx.x; k < RS_TILE; k += GP_BLOCK) {
         const int64_t i = base + k;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & (NB - 1u)], 1u);
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < NB; d += GP_BLOCK) hist[(int64_t)d * nblk + blockIdx.x] = h[d];
 }
 
+// Each chunk of GP_BLOCK keys is ranked per digit with wavefront ballots (the
+// lanes of equal digit, in lane order), then waves in order: stable.
+template <int DIG>
 __global__ __launch_bounds__(GP_BLOCK) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                                                          uint32_t *vout, int64_t n, int shift, int init_vals,
                                                          const uint32_t *offs, int nblk)
 {
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wc[GP_BLOCK / 64][256];
+    constexpr int NB = 1 << DIG;
+    __shared__ uint32_t run[NB];
+    __shared__ uint32_t wc[GP_BLOCK / 64][NB];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    run[tid] = offs[(int64_t)tid * nblk + blockIdx.x];
+    for (int d = tid; d < NB; d += GP_BLOCK) run[d] = offs[(int64_t)d * nblk + blockIdx.x];
     const unsigned long long lt = (1ull << lane) - 1ull;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     for (int c = 0; c < RS_TILE / GP_BLOCK; c++) {
-        for (int w = 0; w < GP_BLOCK / 64; w++) wc[w][tid] = 0;
+        for (int d = tid; d < NB; d += GP_BLOCK)
+            for (int w = 0; w < GP_BLOCK / 64; w++) wc[w][d] = 0;
         __syncthreads();
         const int64_t i = base + (int64_t)c * GP_BLOCK + tid;
         const bool valid = i < n;
         const uint32_t k = valid ? kin[i] : 0u;
-        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t d = (k >> shift) & (NB - 1u);
         const uint32_t v = valid ? (init_vals ? (uint32_t)i : vin[i]) : 0u;
         unsigned long long peers = __ballot(valid);
 #pragma unroll
-        for (int bit = 0; bit < 8; bit++) {
+        for (int bit = 0; bit < DIG; bit++) {
             const bool on = (d >> bit) & 1u;
             const unsigned long long bb = __ballot(valid && on);
             peers &= on ? bb : ~bb;
@@ -561,33 +649,37 @@ __global__ __launch_bounds__(GP_BLOCK) void k_rs_scatter(const uint32_t *kin, co
             vout[off] = v;
         }
         __syncthreads();
-        uint32_t add = 0;
-        for (int w = 0; w < GP_BLOCK / 64; w++) add += wc[w][tid];
-        run[tid] += add;
+        for (int e = tid; e < NB; e += GP_BLOCK) {
+            uint32_t add = 0;
+            for (int w = 0; w < GP_BLOCK / 64; w++) add += wc[w][e];
+            run[e] += add;
+        }
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// sort keys, permutation gather, tile offsets
+// sort keys, bucket offsets, interleaved placement
+//
+// The particles of a tile are stored class-interleaved: class
+// b = (cx + W cy) mod 64 of the particle's cell (cx, cy) within the tile,
+// W the row pitch of the push kernel's LDS windows, is the LDS bank group of
+// the cell's stencil base.  Position q of the tile holds the r-th particle
+// (in the previous order) of a class, rank-major: all classes' particle 0 in
+// class order, then all particle 1, ...  A wavefront of freshly sorted
+// particles therefore addresses 64 different bank groups.
 // ---------------------------------------------------------------------------
-__global__ void k_tile_key(const float *x, const float *y, int64_t n, int ngb, int ngx, int ngy, int ntx,
-                           uint32_t *key)
+__global__ void k_tile_key(const float *x, const float *y, int64_t n, int ngb, int ngx, int ngy, int ntx, int W,
+                           int cls, uint32_t *key)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     int ci = (int)floorf(x[i]) - ngb, cj = (int)floorf(y[i]) - ngb;
     ci = ci < 0 ? 0 : (ci >= ngx ? ngx - 1 : ci);
     cj = cj < 0 ? 0 : (cj >= ngy ? ngy - 1 : cj);
-    key[i] = (uint32_t)((cj / GP_TILE) * ntx + ci / GP_TILE);
-}
-__global__ void k_permute(gp_soa in, gp_soa out, const uint32_t *perm, int64_t n)
-{
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = perm[i];
-    out.x[i] = in.x[s]; out.y[i] = in.y[s];
-    out.vx[i] = in.vx[s]; out.vy[i] = in.vy[s]; out.vz[i] = in.vz[s];
+    const uint32_t t = (uint32_t)((cj / GP_TILE) * ntx + ci / GP_TILE);
+    const uint32_t b = (uint32_t)((ci % GP_TILE) + W * (cj % GP_TILE)) & 63u;
+    key[i] = cls ? t * 64u + b : t;
 }
 __global__ void k_tile_off(const uint32_t *key, int64_t n, int ntiles, int32_t *off)
 {
@@ -598,6 +690,73 @@ __global__ void k_tile_off(const uint32_t *key, int64_t n, int ntiles, int32_t *
     for (int t = kp + 1; t <= k; t++) off[t] = (int32_t)i;
     if (i == n - 1)
         for (int t = k + 1; t <= ntiles; t++) off[t] = (int32_t)n;
+}
+// first sorted position of every key value 0..nb-1, and off[nb] = n
+__global__ void k_permute(gp_soa in, gp_soa out, const uint32_t *perm, int64_t n)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = perm[i];
+    out.x[i] = in.x[s]; out.y[i] = in.y[s];
+    out.vx[i] = in.vx[s]; out.vy[i] = in.vy[s]; out.vz[i] = in.vz[s];
+}
+__global__ void k_bucket_off(const uint32_t *key, int64_t n, int64_t nb, uint32_t *off)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t k = key[i];
+    const int64_t kp = i > 0 ? (int64_t)key[i - 1] : -1;
+    for (int64_t t = kp + 1; t <= k; t++) off[t] = (uint32_t)i;
+    if (i == n - 1)
+        for (int64_t t = k + 1; t <= nb; t++) off[t] = (uint32_t)n;
+}
+// One workgroup per tile; lane = class.  Rank r of the tile's classes goes
+// to positions M(r) .. M(r) + popcount(classes with more than r particles),
+// M(r) the particles of ranks below r: the popcounts of a chunk of ranks are
+// ballots, M is their scan.  Only the permutation is interleaved here (perm
+// holds the old index of each (tile, class)-sorted position; dst[d]
+// receives the old index of new position d): a lane walks one bucket of
+// perm.  The particles follow in one gather (k_permute).
+__global__ __launch_bounds__(GP_BLOCK) void k_interleave(const uint32_t *__restrict__ boff,
+                                                         const uint32_t *__restrict__ perm,
+                                                         uint32_t *__restrict__ dst, int32_t *tile_off, int ntiles)
+{
+    __shared__ uint32_t sM[SC_TILE];
+    const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const uint32_t b0 = boff[t * 64 + lane], nbk = boff[t * 64 + lane + 1] - b0;
+    const uint32_t t0 = boff[t * 64];
+    if (tid == 0) tile_off[t] = (int32_t)t0;
+    if (t == ntiles - 1 && tid == 0) tile_off[ntiles] = (int32_t)boff[ntiles * 64];
+    uint32_t R = nbk;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) R = max(R, (uint32_t)__shfl_xor((int)R, o, 64));
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    uint32_t carry = 0;
+    for (uint32_t c0 = 0; c0 < R; c0 += SC_TILE) {
+        for (int k = wv; k < SC_TILE; k += GP_BLOCK / 64) {
+            uint32_t pc = 0;
+            if (c0 + k < R) pc = (uint32_t)__popcll(__ballot(c0 + k < nbk));
+            if (lane == 0) sM[k] = pc;
+        }
+        __syncthreads();
+        uint32_t v[SC_ITEMS], sum = 0;
+#pragma unroll
+        for (int q = 0; q < SC_ITEMS; q++) { v[q] = sM[tid * SC_ITEMS + q]; sum += v[q]; }
+        uint32_t tot;
+        uint32_t off = block_excl_scan(sum, &tot);
+#pragma unroll
+        for (int q = 0; q < SC_ITEMS; q++) { sM[tid * SC_ITEMS + q] = carry + off; off += v[q]; }
+        __syncthreads();
+        const uint32_t kend = min((uint32_t)SC_TILE, R - c0);
+        for (uint32_t k = wv; k < kend; k += GP_BLOCK / 64) {
+            const uint32_t r = c0 + k;
+            const bool on = r < nbk;
+            const unsigned long long m = __ballot(on);
+            if (on) dst[t0 + sM[k] + (uint32_t)__popcll(m & lt)] = perm[b0 + r];
+        }
+        carry += tot;
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -869,6 +1028,21 @@ int launch_push_o(int grid, hipStream_t s, const PushArgs &a, bool unib, int dep
     return GP_OK;
 }
 
+// staging area of the tile windows (allocated on first use, kept by the context)
+int ensure_stage(gp_ctx *c, int dep)
+{
+    const int W = WIN + c->order + 1;
+    const size_t bytes = (size_t)c->ntiles * W * W * (dep >= 2 ? 3 : 1) * sizeof(unsigned long long);
+    if (c->stage && c->stage_bytes >= bytes) return GP_OK;
+    if (c->stage) GP_HIP(hipFree(c->stage));
+    if (c->stage_on) GP_HIP(hipFree(c->stage_on));
+    c->stage = nullptr; c->stage_on = nullptr; c->stage_bytes = 0;
+    GP_HIP(hipMalloc((void **)&c->stage, bytes));
+    GP_HIP(hipMalloc((void **)&c->stage_on, (size_t)c->ntiles));
+    c->stage_bytes = bytes;
+    return GP_OK;
+}
+
 int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64_t first, int64_t count)
 {
     gp_species &S = c->sp[sid];
@@ -889,11 +1063,22 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
     a.constX = (float)(dt / (double)c->dxf);                                 // Particle.mm:620
     a.constY = (float)(dt / (double)c->dyf);
     a.ngx = c->ngx; a.ngy = c->ngy; a.ngb = c->ngb; a.nx = c->nx; a.ny = c->ny;
+    a.moved_mask = c->ntiles >= 256 ? GP_MOVED_SAMPLE - 1 : 0;
     a.bc0 = c->cfg.pbc[0]; a.bc1 = c->cfg.pbc[1]; a.bc2 = c->cfg.pbc[2]; a.bc3 = c->cfg.pbc[3];
     a.boris = c->cfg.boris_order;
+    // (a period of at least 2 tiles keeps every imaged stencil node within
+    // one period of its interior node)
+    a.imgx = dep <= 1 && c->cfg.pbc[0] == GP_PBC_PERIODIC && c->cfg.fbc[0] == GP_FBC_PERIODIC && c->ngx >= 2 * GP_TILE;
+    a.imgy = dep <= 1 && c->cfg.pbc[2] == GP_PBC_PERIODIC && c->cfg.fbc[2] == GP_FBC_PERIODIC && c->ngy >= 2 * GP_TILE;
     a.rho = (unsigned long long *)S.rho_acc;
     a.jx = (unsigned long long *)S.jx_acc;
     a.jy = (unsigned long long *)S.jy_acc;
+    if (a.use_tiles && dep >= 1) {
+        int rc = ensure_stage(c, dep);
+        if (rc) return rc;
+        a.stage = (unsigned long long *)c->stage;
+        a.stage_on = c->stage_on;
+    }
     a.inv_vnorm = (float)S.inv_vnorm;
     a.dead = S.dead;
     a.counters = S.counters;
@@ -911,6 +1096,16 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
     default: rc = launch_push_o<5>(grid, c->stream, a, c->uniform_b, dep, push); break;
     }
     if (rc) return rc;
+    if (a.use_tiles && dep >= 1) {
+        const int nc = dep >= 2 ? 3 : 1;
+        const dim3 g(gp_cdiv(c->nx + 2, GP_BLOCK), c->ny + 2, nc);
+        switch (c->order) {
+        case 1: k_stage_reduce<1><<<g, GP_BLOCK, 0, c->stream>>>(a.stage, a.stage_on, nc, c->ntx, c->nty, c->ngb, c->ngx, c->ngy, c->nx, c->ny, a.imgx, a.imgy, a.rho, a.jx, a.jy); break;
+        case 3: k_stage_reduce<3><<<g, GP_BLOCK, 0, c->stream>>>(a.stage, a.stage_on, nc, c->ntx, c->nty, c->ngb, c->ngx, c->ngy, c->nx, c->ny, a.imgx, a.imgy, a.rho, a.jx, a.jy); break;
+        default: k_stage_reduce<5><<<g, GP_BLOCK, 0, c->stream>>>(a.stage, a.stage_on, nc, c->ntx, c->nty, c->ngb, c->ngx, c->ngy, c->nx, c->ny, a.imgx, a.imgy, a.rho, a.jx, a.jy); break;
+        }
+        GP_HIP(hipGetLastError());
+    }
     if (push) {
         k_sum_partials<<<1, GP_BLOCK, 0, c->stream>>>(S.ke_part, grid, S.ke_out);
         GP_HIP(hipGetLastError());
@@ -929,16 +1124,23 @@ int gp_radix_sort_u32(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *k
     uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
     if (n <= 0) { *keys_out = ki; *vals_out = vi; return GP_OK; }
     const int nblk = gp_cdiv(n, RS_TILE);
+    // fewest passes of at most 9 bits; 8-bit digits when they suffice
+    if (bits < 1) bits = 1;
+    const int passes = (bits + 8) / 9;
+    const int dig = bits <= 8 * passes ? 8 : 9;
     uint32_t *hist = nullptr;
-    GP_HIP(hipMallocAsync((void **)&hist, sizeof(uint32_t) * 256 * (size_t)nblk, s));
-    const int passes = bits <= 0 ? 1 : (bits + 7) / 8;
+    GP_HIP(hipMallocAsync((void **)&hist, sizeof(uint32_t) * ((size_t)1 << dig) * (size_t)nblk, s));
     for (int ps = 0; ps < passes; ps++) {
-        const int shift = 8 * ps;
-        k_rs_hist<<<nblk, GP_BLOCK, 0, s>>>(ki, n, shift, hist, nblk);
+        const int shift = dig * ps;
+        if (dig == 8) k_rs_hist<8><<<nblk, GP_BLOCK, 0, s>>>(ki, n, shift, hist, nblk);
+        else k_rs_hist<9><<<nblk, GP_BLOCK, 0, s>>>(ki, n, shift, hist, nblk);
         GP_HIP(hipGetLastError());
-        int rc = scan_u32(s, hist, 256 * (int64_t)nblk);
+        int rc = scan_u32(s, hist, ((int64_t)1 << dig) * (int64_t)nblk);
         if (rc) return rc;
-        k_rs_scatter<<<nblk, GP_BLOCK, 0, s>>>(ki, vi, ko, vo, n, shift, (ps == 0) ? init_vals : 0, hist, nblk);
+        if (dig == 8)
+            k_rs_scatter<8><<<nblk, GP_BLOCK, 0, s>>>(ki, vi, ko, vo, n, shift, (ps == 0) ? init_vals : 0, hist, nblk);
+        else
+            k_rs_scatter<9><<<nblk, GP_BLOCK, 0, s>>>(ki, vi, ko, vo, n, shift, (ps == 0) ? init_vals : 0, hist, nblk);
         GP_HIP(hipGetLastError());
         std::swap(ki, ko);
         std::swap(vi, vo);
@@ -1021,17 +1223,34 @@ int gp_launch_sort(gp_ctx *c, int sid)
         return GP_OK;
     }
     if (c->timing) { int rc = gp_time_mark(c, 1); if (rc) return rc; }
+    S.nsorts++;
     const int g = gp_cdiv(n, GP_BLOCK);
-    k_tile_key<<<g, GP_BLOCK, 0, s>>>(S.a.x, S.a.y, n, c->ngb, c->ngx, c->ngy, c->ntx, S.key);
+    const int W = WIN + c->order + 1;
+    // interleave the bank classes unless the species' particles change cell
+    // too often for the order to last (more than 1 % of them in the last push)
+    const bool cls = S.last_moved < 0 || (double)S.last_moved < 0.01 * (double)n;
+    k_tile_key<<<g, GP_BLOCK, 0, s>>>(S.a.x, S.a.y, n, c->ngb, c->ngx, c->ngy, c->ntx, W, cls ? 1 : 0, S.key);
     GP_HIP(hipGetLastError());
     uint32_t *ks = nullptr, *vs = nullptr;
-    int rc = gp_radix_sort_u32(s, S.key, S.val, S.key2, S.val2, n, c->tile_bits, 1, &ks, &vs);
+    int rc = gp_radix_sort_u32(s, S.key, S.val, S.key2, S.val2, n, c->tile_bits + (cls ? 6 : 0), 1, &ks, &vs);
     if (rc) return rc;
-    k_permute<<<g, GP_BLOCK, 0, s>>>(S.a, S.b, vs, n);
-    GP_HIP(hipGetLastError());
+    if (cls) {
+        const int64_t nb = (int64_t)c->ntiles * 64;
+        if (!c->boff) GP_HIP(hipMalloc((void **)&c->boff, sizeof(uint32_t) * (nb + 1)));
+        k_bucket_off<<<g, GP_BLOCK, 0, s>>>(ks, n, nb, c->boff);
+        GP_HIP(hipGetLastError());
+        uint32_t *dst = vs == S.val ? S.val2 : S.val;   // the value buffer the sort left free
+        k_interleave<<<c->ntiles, GP_BLOCK, 0, s>>>(c->boff, vs, dst, S.tile_off, c->ntiles);
+        GP_HIP(hipGetLastError());
+        k_permute<<<g, GP_BLOCK, 0, s>>>(S.a, S.b, dst, n);
+        GP_HIP(hipGetLastError());
+    } else {
+        k_permute<<<g, GP_BLOCK, 0, s>>>(S.a, S.b, vs, n);
+        GP_HIP(hipGetLastError());
+        k_tile_off<<<g, GP_BLOCK, 0, s>>>(ks, n, c->ntiles, S.tile_off);
+        GP_HIP(hipGetLastError());
+    }
     std::swap(S.a, S.b);
-    k_tile_off<<<g, GP_BLOCK, 0, s>>>(ks, n, c->ntiles, S.tile_off);
-    GP_HIP(hipGetLastError());
     S.n_binned = n;
     if (c->timing) { int rc = gp_time_mark(c, 1); if (rc) return rc; }
     return GP_OK;

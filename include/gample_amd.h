@@ -204,8 +204,18 @@ int  gp_push(gp_ctx *ctx, int sid, double dt, int deposit);
  * (holes filled from the tail), per-wall counts, FP64 kinetic energy. */
 int  gp_reduce(gp_ctx *ctx, int sid, int64_t counts[4], double *kinetic);
 /* cell/tile sort of the SoA store by a GPU LSD radix sort (no reference
- * counterpart: the reference keeps insertion order) */
+ * counterpart: the reference keeps insertion order).  Particles are grouped
+ * by 16x16-cell tile; inside a tile they are interleaved over the 64 LDS
+ * bank classes of their cells, each class in its previous order. */
 int  gp_sort(gp_ctx *ctx, int sid);
+/* drift-triggered sorting in gp_step (no reference counterpart): a species is
+ * sorted before its push when the previous step's push gathered more than
+ * fraction * n of its particles outside their tile windows; 0 turns it off.
+ * Independent of gp_step's fixed sort_interval. */
+int  gp_set_sort_drift(gp_ctx *ctx, double fraction);
+/* sorts of species sid so far (out[0]) and the drifted-particle count of
+ * its last push that gp_step read back (out[1], -1 before any) */
+int  gp_sort_stats(gp_ctx *ctx, int sid, int64_t out[2]);
 /* -[Moment integrateMoments:withEMField:withLogger:] (Moment.mm:206-270 +
  * kernel :33-135): n, n*v (3), n*vv (6) node moments in FP64,
  * 10*(nx+1)*(ny+1) doubles, unnormalised as the kernel leaves them. */

@@ -244,6 +244,32 @@ def fold(f, rho):
     return r
 
 
+def fold_acc(acc, ngb, ngx, ngy, perx=True, pery=True):
+    """The periodic branch of the ghost-charge fold (EMField.mm:749-850,
+    x then y) on an int64 node accumulator of shape (ny+1, nx+1): each ghost
+    node's charge is added to its interior image one period away, and the
+    ghost is cleared.  Integer sums are exact, so two accumulators that hold
+    the same charge at different periodic images fold to the same array."""
+    a = np.array(acc, np.int64, copy=True)
+    nx, ny = ngx + 2 * ngb, ngy + 2 * ngb
+    assert a.shape == (ny + 1, nx + 1)
+    if perx:
+        for g in range(ngb):
+            a[:, g + ngx] += a[:, g]
+            a[:, g] = 0
+        for g in range(ngb + ngx, nx + 1):
+            a[:, g - ngx] += a[:, g]
+            a[:, g] = 0
+    if pery:
+        for g in range(ngb):
+            a[g + ngy, :] += a[g, :]
+            a[g, :] = 0
+        for g in range(ngb + ngy, ny + 1):
+            a[g - ngy, :] += a[g, :]
+            a[g, :] = 0
+    return a
+
+
 def poisson_rhs(f, rho, rbc):
     nkx, nky, _, _ = poisson_dims(f)
     out = np.zeros((nkx + 1) * (nky + 1), np.float64)

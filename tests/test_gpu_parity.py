@@ -82,9 +82,11 @@ def test_push_deposit_bit_exact(gp, orc, order, boris, uniform, sorted_):
     for k, (a, b) in enumerate(zip(after, exp)):
         bad = np.nonzero(a.view(np.uint32) != b.view(np.uint32))[0]
         assert len(bad) == 0, f"component {k}: {len(bad)} mismatches, first {bad[:5]} {a[bad[:5]]} {b[bad[:5]]}"
-    rho = ctx.acc(sid)
+    # binned particles deposit by position on the periodic axes (charge at a
+    # ghost node lands on its interior image): compare after the ghost fold
+    rho = orc.fold_acc(ctx.acc(sid), ngb, ngx, ngy)
     erho, _, _ = orc.deposit_fx(g, exp)
-    np.testing.assert_array_equal(rho, erho)
+    np.testing.assert_array_equal(rho, orc.fold_acc(erho, ngb, ngx, ngy))
     ctx.close()
 
 
@@ -110,40 +112,68 @@ def test_current_deposit_bit_exact(gp, orc, order):
     ctx.close()
 
 
-def test_sort_stable_binning(gp, orc):
-    say("[sort]")
+GP_TILE, GP_DRIFT = 16, 12   # tile edge and drift margin of the push kernel's LDS windows (cells)
+
+
+def sort_order(x, y, ngx, ngy, ngb, order):
+    """Permutation gp_sort applies: grouped by tile (row-major), and inside a
+    tile rank-major over the 64 LDS bank classes b = (cx + W cy) mod 64 of
+    the cells (W = the window pitch), each class in its previous order."""
+    T, W = GP_TILE, GP_TILE + 2 * GP_DRIFT + order + 1
+    ci = np.clip(np.floor(x).astype(np.int64) - ngb, 0, ngx - 1)
+    cj = np.clip(np.floor(y).astype(np.int64) - ngb, 0, ngy - 1)
+    ntx = (ngx + T - 1) // T
+    t = (cj // T) * ntx + ci // T
+    b = ((ci % T) + W * (cj % T)) & 63
+    o1 = np.argsort(t * 64 + b, kind="stable")
+    ks = (t * 64 + b)[o1]
+    first = np.r_[0, np.flatnonzero(np.diff(ks)) + 1]
+    start = np.repeat(first, np.diff(np.r_[first, ks.size]))
+    r = np.arange(ks.size) - start
+    return o1[np.lexsort((b[o1], r, t[o1]))]
+
+
+@pytest.mark.parametrize("order", [1, 3, 5])
+def test_sort_order(gp, orc, order):
+    """The sort is a fixed function of the input: tiles in order, classes
+    interleaved inside a tile, stable within a class."""
+    say(f"[sort] order={order}")
     ngx, ngy = 64, 40
-    ctx = mkctx(gp, ngx, ngy, 3)
+    ngb = order // 2
+    ctx = mkctx(gp, ngx, ngy, order)
     sid = ctx.add_species(-EC, ME, 1.0, 300000)
-    p = rand_parts(250000, ngx, ngy, 1, seed=3)
+    p = rand_parts(250000, ngx, ngy, ngb, seed=3)
+    p[1][:5000] = np.float32(ngb + 3.5)                 # a crowded row: uneven classes
     p[4] = np.arange(len(p[0]), dtype=np.float32)      # tag with the original index
     ctx.upload(sid, *p)
     ctx.sort(sid)
     s = ctx.download(sid)
     tag = s[4].astype(np.int64)
-    assert np.array_equal(np.sort(tag), np.arange(len(tag)))
-    np.testing.assert_array_equal(s[0], p[0][tag])
-    T = 16                                             # tile edge (GP_TILE)
-    ntx = (ngx + T - 1) // T
-    key = ((np.floor(s[1]).astype(int) - 1) // T) * ntx + (np.floor(s[0]).astype(int) - 1) // T
-    assert np.all(np.diff(key) >= 0)
-    for k in np.unique(key)[:50]:
-        t = tag[key == k]
-        assert np.all(np.diff(t) > 0)                    # stable within a tile
+    np.testing.assert_array_equal(tag, sort_order(p[0], p[1], ngx, ngy, ngb, order))
+    for a, b in zip(s[:4], p[:4]):
+        np.testing.assert_array_equal(a, b[tag])
+    # a second sort of the sorted store is the same function of its input
+    ctx.sort(sid)
+    s2 = ctx.download(sid)
+    np.testing.assert_array_equal(s2[4].astype(np.int64), tag[sort_order(s[0], s[1], ngx, ngy, ngb, order)])
     ctx.close()
 
 
-def test_sorted_and_unsorted_deposits_agree(gp, orc):
-    """int64 accumulation: the LDS-tiled and the global path give the same bits."""
-    say("[tiling]")
-    ctx = mkctx(gp, 64, 64, 3, cur=1)
+@pytest.mark.parametrize("cur", [0, 1])
+def test_sorted_and_unsorted_deposits_agree(gp, orc, cur):
+    """int64 accumulation: the LDS-tiled and the global path give the same bits
+    (the charge after the periodic ghost fold; the current as deposited)."""
+    say(f"[tiling] cur={cur}")
+    ctx = mkctx(gp, 64, 64, 3, cur=cur)
     sid = ctx.add_species(-EC, ME, 1.0, 300000, vth_hint=1e8)
     ctx.upload(sid, *rand_parts(300000, 64, 64, 1, seed=5))
+    which = (gp.G_RHO_ACC, gp.G_JX_ACC, gp.G_JY_ACC) if cur else (gp.G_RHO_ACC,)
     ctx.reset_grids(); ctx.deposit(sid)
-    a = [ctx.acc(sid, w) for w in (gp.G_RHO_ACC, gp.G_JX_ACC, gp.G_JY_ACC)]
+    a = [ctx.acc(sid, w) for w in which]
     ctx.sort(sid)
     ctx.reset_grids(); ctx.deposit(sid)
-    b = [ctx.acc(sid, w) for w in (gp.G_RHO_ACC, gp.G_JX_ACC, gp.G_JY_ACC)]
+    b = [ctx.acc(sid, w) for w in which]
+    a[0], b[0] = orc.fold_acc(a[0], 1, 64, 64), orc.fold_acc(b[0], 1, 64, 64)
     for u, v in zip(a, b):
         np.testing.assert_array_equal(u, v)
     ctx.close()
@@ -373,6 +403,40 @@ def test_deterministic_rerun(gp):
         np.testing.assert_array_equal(a, b)
 
 
+def test_drift_triggered_sort(gp, orc):
+    """gp_set_sort_drift: a species whose particles leave their tile windows
+    is re-sorted by gp_step before its next push, and the run is otherwise
+    the same to the bit (particles matched by tag, rho, Ex)."""
+    say("[drift sort]")
+    ngx, dx, dt = 64, 0.005, 5e-12
+    cX = float(np.float32(dt / float(np.float32(dx))))
+    runs = []
+    for frac in (0.0, 1e-3):
+        ctx = mkctx(gp, ngx, ngx, 3, dx=dx)
+        sid = ctx.add_species(-EC, ME, 1.0, 300000, vth_hint=1e9)
+        rng = np.random.default_rng(11)
+        p = rand_parts(200000, ngx, ngx, 1, seed=12)
+        p[2] = (rng.uniform(-4.0, 4.0, p[2].size) / cX).astype(np.float32)   # up to 4 cells per step
+        p[3] = (rng.uniform(-4.0, 4.0, p[3].size) / cX).astype(np.float32)
+        p[4] = np.arange(p[0].size, dtype=np.float32)
+        ctx.upload(sid, *p)
+        ctx.sort(sid)
+        ctx.set_sort_drift(frac)
+        ctx.reset_grids(); ctx.deposit(sid)
+        for cyc in range(1, 9):
+            ctx.step(dt, tol=1e-10, maxiter=400, sort_interval=0, cycle=cyc)
+        s = ctx.download(sid)
+        o = np.argsort(s[4])
+        runs.append(([a[o] for a in s], ctx.acc(sid, gp.G_RHO_ACC), ctx.field(gp.F_EX), ctx.sort_stats(sid)))
+        ctx.close()
+    assert runs[0][3][0] == 1                       # the initial sort only
+    assert runs[1][3][0] > 1 and runs[1][3][1] >= 0
+    for a, b in zip(runs[0][0], runs[1][0]):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(orc.fold_acc(runs[0][1], 1, ngx, ngx), orc.fold_acc(runs[1][1], 1, ngx, ngx))
+    np.testing.assert_array_equal(runs[0][2], runs[1][2])
+
+
 def test_edge_cases(gp):
     say("[edges]")
     ctx = mkctx(gp, 16, 16, 3)
@@ -480,7 +544,7 @@ def test_wall_injection(gp, orc):
 
 @pytest.mark.parametrize("order", [1, 3, 5])
 def test_drifted_particles_reach_the_window_edges(gp, orc, order):
-    """Particles that have drifted up to 8.5 cells from their tile since the
+    """Particles that have drifted up to GP_DRIFT + 0.5 cells from their tile since the
     last sort: the fused push+deposit (rho and J) stays bit-exact against the
     oracle whether a particle's stencil lands on the edge nodes of the tile's
     LDS window or outside it (global path)."""
@@ -492,8 +556,9 @@ def test_drifted_particles_reach_the_window_edges(gp, orc, order):
     rng = np.random.default_rng(40 + order)
     p = rand_parts(200000, ngx, ngy, ngb, seed=50 + order)
     cX = float(np.float32(dt / float(np.float32(dx))))
-    p[2] = (rng.uniform(-8.5, 8.5, p[2].size) / cX).astype(np.float32)
-    p[3] = (rng.uniform(-8.5, 8.5, p[3].size) / cX).astype(np.float32)
+    d = GP_DRIFT + 0.5
+    p[2] = (rng.uniform(-d, d, p[2].size) / cX).astype(np.float32)
+    p[3] = (rng.uniform(-d, d, p[3].size) / cX).astype(np.float32)
     ctx.upload(sid, *p)
     ctx.sort(sid)
     s0 = ctx.download(sid)

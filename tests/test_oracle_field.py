@@ -130,3 +130,37 @@ def test_darwin_operator(orc):
     d[:, 0] = d[:, -1] = 0.5
     S = d.ravel()[:, None] * A
     np.testing.assert_allclose(S, S.T, atol=1e-9 * np.abs(A).max())
+
+
+@pytest.mark.parametrize("ngb", [0, 1, 2])
+@pytest.mark.parametrize("perx,pery", [(True, True), (True, False), (False, True)])
+def test_integer_fold_matches_reference_fold(orc, ngb, perx, pery):
+    """fold_acc (the periodic ghost fold on int64 accumulators) gives the
+    interior nodes of the reference-order FP64 fold (gpo_fold_rho) on
+    integer charge (exact in FP64), clears the periodic ghosts, and conserves
+    the total; a deposit moved to a ghost node's interior image folds to the
+    same array."""
+    ngx, ngy = 12, 9
+    nx, ny = ngx + 2 * ngb, ngy + 2 * ngb
+    rng = np.random.default_rng(10 * ngb + 2 * perx + pery)
+    a = rng.integers(0, 2 ** 40, (ny + 1, nx + 1), dtype=np.int64)
+    fbc = ("periodic" if perx else "Neumann",) * 2 + ("periodic" if pery else "Neumann",) * 2
+    f = orc.field(ngx, ngy, ngb, 0.1, 0.1, fbc=fbc)
+    ref = orc.fold(f, a.astype(np.float64))
+    got = orc.fold_acc(a, ngb, ngx, ngy, perx, pery)
+    ix = slice(ngb, ngb + ngx) if perx else slice(0, nx + 1)
+    iy = slice(ngb, ngb + ngy) if pery else slice(0, ny + 1)
+    if perx and pery:
+        np.testing.assert_array_equal(got[iy, ix].astype(np.float64), ref[iy, ix])
+        assert got.sum() == a.sum()
+    if perx:
+        assert not got[:, :ngb].any() and not got[:, ngb + ngx:].any()
+    if pery:
+        assert not got[:ngb, :].any() and not got[ngb + ngy:, :].any()
+    # moving charge from a ghost node to its interior image leaves the fold unchanged
+    if perx:
+        b = a.copy()
+        g = ngb + ngx            # first high ghost column -> interior column ngb
+        b[:, g - ngx] += b[:, g]
+        b[:, g] = 0
+        np.testing.assert_array_equal(orc.fold_acc(b, ngb, ngx, ngy, perx, pery), got)
