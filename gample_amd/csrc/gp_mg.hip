@@ -34,9 +34,14 @@ constexpr int HU = 2, WU = TB + 2 * HU;   // window of k_rb_sweep (one full swee
 #define MG_SWEEPS 1
 #endif
 constexpr int NSW = MG_SWEEPS;            // smoothing sweeps per side of the V-cycle
+#ifndef MG_BY
+#define MG_BY 32
+#endif
+constexpr int BY = MG_BY;                 // rows of the (32, BY) workgroup of the level kernels
+constexpr int NT = 32 * BY;
 constexpr int BOT_MAX = 4096;          // unknowns of the largest LDS level
 constexpr int BOT_LEVELS = 12;
-constexpr int DENSE_MAX = 256;         // coarsest level solved by a dense inverse
+constexpr int DENSE_MAX = 64;          // coarsest level solved by a dense inverse
 constexpr int RB = 256;                // workgroup of the reduction kernels
 
 struct Lev {
@@ -136,16 +141,15 @@ __device__ __forceinline__ void apply_op(double *scal, int slot, int op, double 
 // that consumes a reduction finish it itself instead of a separate launch.
 __device__ double sum_parts(const double *part, int np, double *sh, int tid)
 {
+    // threads 0..255: strided partial sums, then a fixed-order wave tree
+    // (shuffles) and the four wave sums in order
     double a = 0.0;
     if (tid < 256)
         for (int k = tid; k < np; k += 256) a += part[k];
-    if (tid < 256) sh[tid] = a;
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+    if (tid < 256 && (tid & 63) == 0) sh[tid >> 6] = a;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w) sh[tid] += sh[tid + w];
-        __syncthreads();
-    }
-    const double s = sh[0];
+    const double s = (sh[0] + sh[1]) + (sh[2] + sh[3]);
     __syncthreads();
     return s;
 }
@@ -179,10 +183,10 @@ __device__ bool gate_converged(const Gate &g, double *sh, int tid, bool block0)
 // The window's global loads are all issued before any is consumed.
 // ---------------------------------------------------------------------------
 template <int NS>
-__global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float *fc, Lev L, Lev C, const double *scal,
+__global__ __launch_bounds__(NT) void k_mg_down(const float *f, float *u, float *fc, Lev L, Lev C, const double *scal,
                                                  Gate gt)
 {
-    constexpr int HD = 2 * NS + 1, WD = TB + 2 * HD, NE = (WD * WD + 255) / 256;
+    constexpr int HD = 2 * NS + 1, WD = TB + 2 * HD, NE = (WD * WD + NT - 1) / NT;
     __shared__ float su[WD * WD], sf[WD * WD], sr[(TB + 2) * (TB + 2)];
     __shared__ double shr[256];
     if (done(scal)) return;
@@ -194,7 +198,7 @@ __global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float
     float fv[NE];
 #pragma unroll
     for (int k = 0; k < NE; k++) {
-        const int e = tid + 256 * k;
+        const int e = tid + NT * k;
         const int li = e % WD, lj = e / WD;
         const int gi = gi0 + li, gj = gj0 + lj;
         const bool v = e < WD * WD && in_axis(L.tx, gi, L.mx) && in_axis(L.ty, gj, L.my);
@@ -202,7 +206,7 @@ __global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float
     }
 #pragma unroll
     for (int k = 0; k < NE; k++) {
-        const int e = tid + 256 * k;
+        const int e = tid + NT * k;
         if (e < WD * WD) {
             const int li = e % WD, lj = e / WD;
             const int gi = gi0 + li, gj = gj0 + lj;
@@ -216,7 +220,7 @@ __global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float
 #pragma unroll
     for (int h = 1; h < 2 * NS; h++) {
         const int col = h & 1;   // 1 black, 0 red
-        for (int lj = h + ty; lj < WD - h; lj += 8) {
+        for (int lj = h + ty; lj < WD - h; lj += BY) {
             const int gj = gj0 + lj;
             if (!in_axis(L.ty, gj, L.my)) continue;
             for (int li = h + tx; li < WD - h; li += 32) {
@@ -229,7 +233,7 @@ __global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float
     }
     // residual on band 2NS (the tile and its rim)
     constexpr int RB0 = 2 * NS;
-    for (int lj = RB0 + ty; lj < WD - RB0; lj += 8) {
+    for (int lj = RB0 + ty; lj < WD - RB0; lj += BY) {
         const int gj = gj0 + lj;
         const bool vj = in_axis(L.ty, gj, L.my);
         for (int li = RB0 + tx; li < WD - RB0; li += 32) {
@@ -242,7 +246,7 @@ __global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float
     }
     __syncthreads();
     // pre-smoothed u of the tile
-    for (int lj = ty; lj < TB; lj += 8) {
+    for (int lj = ty; lj < TB; lj += BY) {
         const int gj = tj0 + lj;
         if (gj >= L.my) break;
         const int gi = ti0 + tx;
@@ -251,7 +255,7 @@ __global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float
     // restriction for the coarse unknowns centred in this tile
     const int ci0 = ti0 > 0 ? ti0 / 2 - 1 : 0, cj0 = tj0 > 0 ? tj0 / 2 - 1 : 0;
     constexpr int CW = TB / 2 + 2;
-    for (int cjj = ty; cjj < CW; cjj += 8) {
+    for (int cjj = ty; cjj < CW; cjj += BY) {
         const int J = cj0 + cjj;
         if (J >= C.my) break;
         const int nyc = 2 * c_node(C.ty, J);
@@ -283,15 +287,15 @@ __global__ __launch_bounds__(256) void k_mg_down(const float *f, float *u, float
 // grid level, up leg: u = upre + P uc, then NS sweeps in the adjoint order
 // (black, red, black, red, ...) on bands 1 .. 2NS of a window with a 2NS-wide
 // halo.  Optional partial of f.u (the CG's r.z at the finest level).
-// blockDim (32, 8)
+// blockDim (32, BY)
 // ---------------------------------------------------------------------------
 template <int NS>
-__global__ __launch_bounds__(256) void k_mg_up(const float *upre, const float *f, const float *uc, float *u, Lev L, Lev C,
+__global__ __launch_bounds__(NT) void k_mg_up(const float *upre, const float *f, const float *uc, float *u, Lev L, Lev C,
                                                double *part, const double *scal)
 {
-    constexpr int HU = 2 * NS, WU = TB + 2 * HU, NE = (WU * WU + 255) / 256;
+    constexpr int HU = 2 * NS, WU = TB + 2 * HU, NE = (WU * WU + NT - 1) / NT;
     __shared__ float su[WU * WU], sf[WU * WU];
-    __shared__ double red[4];
+    __shared__ double red[NT / 64];
     if (done(scal)) return;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + 32 * ty;
     const int ti0 = blockIdx.x * TB, tj0 = blockIdx.y * TB;
@@ -300,7 +304,7 @@ __global__ __launch_bounds__(256) void k_mg_up(const float *upre, const float *f
     float uv[NE], fvv[NE];
 #pragma unroll
     for (int k = 0; k < NE; k++) {
-        const int e = tid + 256 * k;
+        const int e = tid + NT * k;
         const int li = e % WU, lj = e / WU;
         const int gi = gi0 + li, gj = gj0 + lj;
         uv[k] = 0.0f;
@@ -313,14 +317,14 @@ __global__ __launch_bounds__(256) void k_mg_up(const float *upre, const float *f
     }
 #pragma unroll
     for (int k = 0; k < NE; k++) {
-        const int e = tid + 256 * k;
+        const int e = tid + NT * k;
         if (e < WU * WU) { su[e] = uv[k]; sf[e] = fvv[k]; }
     }
     __syncthreads();
 #pragma unroll
     for (int h = 1; h <= 2 * NS; h++) {
         const int col = h & 1;   // 1 black, 0 red
-        for (int lj = h + ty; lj < WU - h; lj += 8) {
+        for (int lj = h + ty; lj < WU - h; lj += BY) {
             const int gj = gj0 + lj;
             if (!in_axis(L.ty, gj, L.my)) continue;
             for (int li = h + tx; li < WU - h; li += 32) {
@@ -332,7 +336,7 @@ __global__ __launch_bounds__(256) void k_mg_up(const float *upre, const float *f
         __syncthreads();
     }
     double acc = 0.0;
-    for (int lj = ty; lj < TB; lj += 8) {
+    for (int lj = ty; lj < TB; lj += BY) {
         const int gj = tj0 + lj, gi = ti0 + tx;
         if (gj < L.my && gi < L.mx) {
             const int k = (HU + tx) + (HU + lj) * WU;
@@ -344,7 +348,11 @@ __global__ __launch_bounds__(256) void k_mg_up(const float *upre, const float *f
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
         if ((tid & 63) == 0) red[tid >> 6] = acc;
         __syncthreads();
-        if (tid == 0) part[blockIdx.x + blockIdx.y * gridDim.x] = (red[0] + red[1]) + (red[2] + red[3]);
+        if (tid == 0) {
+            double t = 0.0;
+            for (int w = 0; w < NT / 64; w++) t += red[w];
+            part[blockIdx.x + blockIdx.y * gridDim.x] = t;
+        }
     }
 }
 
@@ -433,8 +441,11 @@ __device__ __forceinline__ float resid_point(const float *u, const float *f, con
 // odd periodic extent)
 __device__ void bot_sweep(float *u, const float *f, const Lev &L, int color)
 {
+    // the points of one colour only: column 2 tx + ((j + color) & 1) of row j
     const float invd = 1.0f / (2.0f * (L.cx + L.cy));
-    BOT_LOOP(L) if (((i + j) & 1) == color) u[i + j * L.mx] = gs_point(u, f, L, i, j, invd);
+    for (int j = threadIdx.y; j < L.my; j += 32)
+        for (int i = 2 * threadIdx.x + ((j + color) & 1); i < L.mx; i += 64)
+            u[i + j * L.mx] = gs_point(u, f, L, i, j, invd);
     __syncthreads();
 }
 __device__ void bot_project(float *a, const Lev &L, float *scratch)
@@ -459,6 +470,7 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
 {
     extern __shared__ float lds[];
     __shared__ float scratch[1024];
+    __shared__ float res[BOT_MAX];   // residual of the level being restricted
     __shared__ double dred[16];
     __shared__ double shr[256];
     if (done(scal)) return;
@@ -491,6 +503,8 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         BOT_LOOP(L) u[i + j * L.mx] = 0.0f;
         __syncthreads();
         for (int sw = 0; sw < NSW; sw++) { bot_sweep(u, f, L, 0); bot_sweep(u, f, L, 1); }
+        BOT_LOOP(L) res[i + j * L.mx] = resid_point(u, f, L, i, j, invd);
+        __syncthreads();
         BOT_LOOP(C) {
             const int cxn = 2 * c_node(C.tx, i), cyn = 2 * c_node(C.ty, j);
             float acc = 0.0f;
@@ -501,7 +515,7 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
                 for (int a = -1; a <= 1; a++) {
                     int ii = L.tx == GP_AX_PER ? wrp(cxn + a, L.mx) : f_unk(L.tx, cxn + a, L.Mx);
                     if (ii < 0) continue;
-                    acc += (a == 0 ? 0.5f : 0.25f) * wy * resid_point(u, f, L, ii, jj, invd);
+                    acc += (a == 0 ? 0.5f : 0.25f) * wy * res[ii + jj * L.mx];
                 }
             }
             fc[i + j * C.mx] = acc;
@@ -817,7 +831,7 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
     auto take = [&]() { const Gate g = (gate && !gated) ? *gate : nog; gated = true; return g; };
     const int nl = (int)s.lv.size();
     const int lb = s.bottom;   // first level handled by k_mg_bottom (nl if none)
-    const dim3 blk(32, 8);
+    const dim3 blk(32, BY), blk8(32, 8);
     for (int l = 0; l < std::min(lb, nl - 1); l++) {
         const Lev L = mklev(s.lv[l]), C = mklev(s.lv[l + 1]);
         k_mg_down<NSW><<<tiles(L), blk, 0, st>>>(s.lv[l].f, s.lv[l].r, s.lv[l + 1].f, L, C, scal, take());
@@ -855,7 +869,7 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
         }
         k_zero_f<<<256, 256, 0, st>>>(a, L.mx * L.my);
         for (int k = 0; k < 2 * sweeps; k++) {
-            k_rb_sweep<<<tiles(L), blk, 0, st>>>(a, s.lv[nl - 1].f, b, L, k < sweeps ? 0 : 1, scal);
+            k_rb_sweep<<<tiles(L), blk8, 0, st>>>(a, s.lv[nl - 1].f, b, L, k < sweeps ? 0 : 1, scal);
             std::swap(a, b);
         }
         GP_HIP(hipGetLastError());
