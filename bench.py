@@ -8,7 +8,7 @@ dt = 0.1/omega_pe, FP32 particles.  A step is one cycle of the reference's
 loop (main.mm:154-254): Poisson solve (GPU MG-CG to a 1e-8 relative
 residual, FP64 refinement) -> fused gather + Boris push + int64 deposit of
 both species -> tile sort of a species when its last push gathered more
-than a fraction (default 1e-4) of its particles outside their tile windows
+than a fraction (default 3e-3) of its particles outside their tile windows
 (--sort-every N adds a fixed interval).  Particles are generated on
 the device (seeded PCG32 streams), so inputs are resident in HBM when the
 timed region starts.
@@ -226,7 +226,7 @@ def main():
     ap.add_argument("--ppc", type=int, default=128)
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--sort-every", type=int, default=0, help="fixed tile-sort interval (0: none)")
-    ap.add_argument("--sort-drift", type=float, default=1e-4,
+    ap.add_argument("--sort-drift", type=float, default=3e-3,
                     help="sort a species when this fraction of it drifted out of its tile windows (0: off)")
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--cpu-steps", type=int, default=3)

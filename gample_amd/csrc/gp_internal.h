@@ -20,7 +20,9 @@
 #include "../../include/gample_amd.h"
 
 #define GP_TILE 16           // tile edge, cells
+#ifndef GP_DRIFT
 #define GP_DRIFT 12          // cells a particle may leave its tile and still use LDS
+#endif
 #define GP_BLOCK 256         // threads per workgroup of the particle kernels
 #define GP_MAX_DRIFT_SPECIES 64   // species followed by drift-triggered sorting
 #define GP_MOVED_SAMPLE 8         // one tile in 8 counts cell changes (grids of 256+ tiles)

@@ -26,12 +26,20 @@
 namespace {
 
 constexpr int WIN = GP_TILE + 2 * GP_DRIFT;   // cell window of a tile
-// waves per SIMD the push kernel is register-budgeted for
-#ifdef PUSH_WAVES
-#define PUSH_OCC __attribute__((amdgpu_waves_per_eu(PUSH_WAVES, PUSH_WAVES)))
-#else
-#define PUSH_OCC
+// threads of a push workgroup (one workgroup per tile): 512 for the 1st-
+// and 3rd-order kernels, register-budgeted for PUSH_WAVES waves per SIMD (80
+// VGPRs; three workgroups with charge-only windows fit a CU's LDS); the
+// 5th-order kernel needs more registers and keeps 256 threads and the
+// compiler's budget, as do the current-deposit kernels, whose windows limit
+// a CU to fewer workgroups anyway
+#ifndef PUSH_BLOCK
+#define PUSH_BLOCK 512
 #endif
+#ifndef PUSH_WAVES
+#define PUSH_WAVES 6
+#endif
+constexpr int push_block(int order) { return order <= 3 ? PUSH_BLOCK : 256; }
+#define PUSH_OCC __attribute__((amdgpu_waves_per_eu(ORDER <= 3 && DEP <= 1 ? PUSH_WAVES : 1)))
 
 // ---------------------------------------------------------------------------
 // B-spline weights.  Horner forms of the reference polynomials
@@ -154,8 +162,9 @@ struct PushArgs {
 };
 
 template <int ORDER, bool UNIB, int DEP, bool PUSH>
-__global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
+__global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
 {
+    constexpr int PB = push_block(ORDER);
     constexpr int NGB = ORDER / 2, NW = ORDER + 1;
     // node window edge: relative to X0 = cx0 - NGB the stencils reach from 0
     // (rho: i2 - NGB with i2 = cx0) to WIN + NW - 1 (Ex, jx: i1 + 1 - NGB +
@@ -167,7 +176,7 @@ __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
     constexpr int NJ = DEP >= 2 ? W * W : 1;
     __shared__ float sEx[NF], sEy[NF], sBz[NB];
     __shared__ unsigned long long sRho[NR], sJx[NJ], sJy[NJ];
-    __shared__ double sKE[GP_BLOCK / 64];
+    __shared__ double sKE[PB / 64];
 
     const int tid = threadIdx.x, b = blockIdx.x;
     int64_t p0, p1;
@@ -194,7 +203,7 @@ __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
     const int pex = a.nx + 2, pey = a.nx + 1;
 
     if (tiled && p0 < p1) {
-        for (int e = tid; e < W * W; e += GP_BLOCK) {
+        for (int e = tid; e < W * W; e += PB) {
             const int gx = X0 + e % W, gy = Y0 + e / W;
             if constexpr (PUSH) {
                 sEx[e] = (gx >= 0 && gx < a.nx + 2 && gy >= 0 && gy < a.ny + 1) ? a.Ex[gx + gy * pex] : 0.0f;
@@ -231,14 +240,14 @@ __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
         const uint32_t o = 4u * tid;
         nx_ = ld(bx, o); ny_ = ld(by, o); nvx = ld(bvx, o); nvy = ld(bvy, o); nvz = ld(bvz, o);
     }
-    for (uint32_t k = tid; k < cnt; k += GP_BLOCK) {
+    for (uint32_t k = tid; k < cnt; k += PB) {
         const int64_t p = s0 + k;
         const uint32_t ob = 4u * k;
         float x = nx_, y = ny_;
         float vx = nvx, vy = nvy;
         const float vz = nvz;
-        if (k + GP_BLOCK < cnt) {
-            const uint32_t o = ob + 4u * GP_BLOCK;
+        if (k + PB < cnt) {
+            const uint32_t o = ob + 4u * PB;
             nx_ = ld(bx, o); ny_ = ld(by, o); nvx = ld(bvx, o); nvy = ld(bvy, o); nvz = ld(bvz, o);
         }
         // live particles lie in [ngb, ngb+ng); anything else (absorbed
@@ -405,7 +414,7 @@ __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
     if constexpr (PUSH) {
         if (tid == 0) {
             double s = 0.0;
-            for (int w = 0; w < GP_BLOCK / 64; w++) s += sKE[w];
+            for (int w = 0; w < PB / 64; w++) s += sKE[w];
             a.ke_part[b] = s;
         }
     }
@@ -416,7 +425,7 @@ __global__ __launch_bounds__(GP_BLOCK) PUSH_OCC void k_push(PushArgs a)
             if (on) {
                 unsigned long long *st = a.stage + (int64_t)b * (W * W) * (DEP >= 2 ? 3 : 1);
 #pragma unroll 1
-                for (int e = tid; e < W * W; e += GP_BLOCK) {
+                for (int e = tid; e < W * W; e += PB) {
                     st[e] = sRho[e];
                     if constexpr (DEP >= 2) { st[W * W + e] = sJx[e]; st[2 * W * W + e] = sJy[e]; }
                 }
@@ -1005,7 +1014,7 @@ __global__ void k_mom_scale(const unsigned long long *acc, int64_t ng, double sn
 template <int ORDER, bool UNIB, int DEP, bool PUSH>
 void launch_push_t(int grid, hipStream_t s, const PushArgs &a)
 {
-    k_push<ORDER, UNIB, DEP, PUSH><<<grid, GP_BLOCK, 0, s>>>(a);
+    k_push<ORDER, UNIB, DEP, PUSH><<<grid, push_block(ORDER), 0, s>>>(a);
 }
 template <int ORDER>
 int launch_push_o(int grid, hipStream_t s, const PushArgs &a, bool unib, int dep, bool push)
@@ -1056,7 +1065,7 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
     a.pfirst = a.use_tiles ? a.n_binned : first;
     a.tile_off = S.tile_off;
     a.ntx = c->ntx; a.ntiles = c->ntiles;
-    a.chunk = 16 * GP_BLOCK;
+    a.chunk = 16 * push_block(c->order);
     a.Ex = c->Ex; a.Ey = c->Ey; a.Bz = c->Bz; a.bz0 = c->bz0;
     a.constE = (float)(0.5 * (double)S.qf * dt / (double)S.mf);             // Particle.mm:618
     a.constB = (float)(0.5 * (double)S.qf * dt / (double)S.mf / 2.99792458e10);  // Particle.mm:619
