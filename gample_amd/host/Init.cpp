@@ -71,7 +71,8 @@ Json defComp()
     return obj({{"MaximumParticleNumber", N(0.08)}, {"ThreadGroupSize", N(256)}, {"IntegrationChunkSize", N(256)},
                 {"AggregationThreshold", N(0.1)}, {"ChebyshevDegree", N(5)}, {"AMGCycleType", N(1)},
                 {"MaxiterForPoisson", N(200)}, {"ToleranceForPoisson", N(1e-7)},
-                {"PoissonSolver", S("MG-CG")}, {"SortInterval", N(20)}, {"NetworkFile", S("")}});
+                {"PoissonSolver", S("MG-CG")}, {"SortInterval", N(20)}, {"SortDriftFraction", N(0)},
+                {"NetworkFile", S("")}});
 }
 Json defParticle()
 {
@@ -147,6 +148,8 @@ void Init::parse(const Json &root)
     if (C.solver != 0 && C.networkFile.empty())
         throw std::runtime_error("PoissonSolver " + solver + " needs NetworkFile (a trained network, gp_nn_save)");
     C.sortInterval = (int)ival(c, "SortInterval");
+    C.sortDrift = dval(c, "SortDriftFraction");
+    if (!(C.sortDrift >= 0.0) || C.sortDrift > 1.0) throw std::runtime_error("SortDriftFraction must lie in [0, 1]");
     // ParamForField (Init.mm:106-149)
     Json d = merged(&section(root, "ParamForField", Json::Object), defField());
     auto &F = paramForField;

@@ -31,7 +31,8 @@ struct ParamForComputing {
     // keys added by this build (ignored by the reference): solver, sort
     int solver = 0;            // "PoissonSolver": "MG-CG" (0), "NN-FPCG" (1), "NN-WARM-MG" (2), "NN-FCG" (3)
     std::string networkFile;   // "NetworkFile": trained network for the network solvers
-    int sortInterval = 20;     // "SortInterval"
+    int sortInterval = 20;     // "SortInterval": fixed tile-sort interval (0: none)
+    double sortDrift = 0.0;    // "SortDriftFraction": sort a species once this fraction drifted out of its tile windows (0: off)
 };
 
 struct ParamForParticle {

@@ -121,6 +121,7 @@ EMField::EMField(const Init &init, int device)
     tol = init.paramForComputing.tolerance;
     maxiter = init.paramForComputing.maxiter;
     solver = init.paramForComputing.solver;
+    check(gp_set_sort_drift(ctx, init.paramForComputing.sortDrift), "SortDriftFraction");
     if (solver != 0) check(gp_nn_load(ctx, init.paramForComputing.networkFile.c_str()), "NetworkFile");
     if (F.InitTypeB == "From1dXFile") {   // EMField.mm:175-197, T -> G
         auto bx = load1d(F.FilePathBx), by = load1d(F.FilePathBy), bz = load1d(F.FilePathBz);

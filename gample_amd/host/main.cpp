@@ -36,11 +36,13 @@ static void dump(const Init &in)
     const auto &F = in.paramForField;
     std::printf("{\"Particle\":%d,\"EMField\":%d,\"Start\":%d,\"End\":%d,\"TimeStep\":%.17g,"
                 "\"pNumMax\":%lld,\"maxiter\":%d,\"tolerance\":%.17g,\"ngx\":%d,\"ngy\":%d,\"ngb\":%d,"
-                "\"dx\":%.17g,\"dy\":%.17g,\"weightOrder\":%d,\"BorisOrder\":%d,\"checkCurDens\":%d,\"species\":[",
+                "\"dx\":%.17g,\"dy\":%.17g,\"weightOrder\":%d,\"BorisOrder\":%d,\"checkCurDens\":%d,"
+                "\"SortInterval\":%d,\"SortDriftFraction\":%.17g,\"species\":[",
                 in.flagForEquation.Particle, in.flagForEquation.EMField, in.paramForTimeIntegration.Start,
                 in.paramForTimeIntegration.End, in.paramForTimeIntegration.TimeStep, in.paramForComputing.pNumMax,
                 in.paramForComputing.maxiter, in.paramForComputing.tolerance, F.ngx, F.ngy, F.ngb, F.dx, F.dy,
-                F.weightOrder, F.BorisOrder, F.checkCurDens);
+                F.weightOrder, F.BorisOrder, F.checkCurDens, in.paramForComputing.sortInterval,
+                in.paramForComputing.sortDrift);
     for (size_t s = 0; s < in.paramForParticle.size(); s++) {
         const auto &p = in.paramForParticle[s];
         std::printf("%s{\"name\":\"%s\",\"pNum\":%lld,\"q\":%.17g,\"m\":%.17g,\"w\":%.17g,\"genType\":\"%s\","

@@ -108,3 +108,24 @@ def test_poisson_solver_key(tmp_path):
         assert (out.returncode == 0) == ok, (name, out.stderr)
         if not ok:
             assert "PoissonSolver" in out.stderr
+
+
+def test_sort_keys(tmp_path):
+    """SortInterval and SortDriftFraction (build keys) parse with their
+    defaults (20, off) and reject a fraction outside [0, 1]"""
+    raw = json.load(open(os.path.join(COND, "periodic.json")))
+    raw.setdefault("ParamForComputing", {}).pop("SortDriftFraction", None)
+    f = tmp_path / "s.json"
+    f.write_text(json.dumps(raw))
+    out = subprocess.run([EXE, "-i", str(f), "--dump-config"], capture_output=True, text=True, timeout=30)
+    assert out.returncode == 0, out.stderr
+    d = json.loads(out.stdout.strip().splitlines()[-1])
+    assert d["SortInterval"] == raw["ParamForComputing"].get("SortInterval", 20)
+    assert d["SortDriftFraction"] == 0.0
+    for frac, ok in ((3e-3, True), (1.5, False), (-1.0, False)):
+        raw["ParamForComputing"]["SortDriftFraction"] = frac
+        f.write_text(json.dumps(raw))
+        out = subprocess.run([EXE, "-i", str(f), "--dump-config"], capture_output=True, text=True, timeout=30)
+        assert (out.returncode == 0) == ok, (frac, out.stderr)
+        if ok:
+            assert json.loads(out.stdout.strip().splitlines()[-1])["SortDriftFraction"] == frac
