@@ -86,7 +86,7 @@ class Diag(C.Structure):
 EXPORTS = [
     "gp_ctx_create", "gp_ctx_destroy", "gp_last_error", "gp_sync", "gp_dims",
     "gp_species_create", "gp_species_upload", "gp_species_download", "gp_species_count",
-    "gp_species_grid_get", "gp_load_pair", "gp_inject_wall", "gp_reset_grids", "gp_deposit",
+    "gp_species_grid_get", "gp_species_read", "gp_load_pair", "gp_inject_wall", "gp_reset_grids", "gp_deposit",
     "gp_solve_fields", "gp_solve_darwin", "gp_push", "gp_reduce", "gp_sort", "gp_set_sort_drift", "gp_sort_stats", "gp_moments",
     "gp_step", "gp_field_get", "gp_field_set", "gp_field_energy", "gp_timing", "gp_kernel_time",
     "gp_nn_init", "gp_nn_train", "gp_nn_infer", "gp_nn_save", "gp_nn_load", "gp_nn_add_pair", "gp_nn_params",
@@ -114,6 +114,7 @@ def lib() -> C.CDLL:
         L.gp_species_upload.argtypes = [P, I, I64, fp, fp, fp, fp, fp]
         L.gp_species_download.argtypes = [P, I, I64, C.POINTER(C.c_int64), fp, fp, fp, fp, fp]
         L.gp_species_count.argtypes = [P, I, C.POINTER(C.c_int64)]
+        L.gp_species_read.argtypes = [P, I, I64, I64, C.POINTER(C.c_int64), fp, fp, fp, fp, fp]
         L.gp_species_grid_get.argtypes = [P, I, I, C.POINTER(C.c_int64), I64]
         L.gp_load_pair.argtypes = [P, I, I, C.POINTER(Gen), C.c_uint64, I64, I64]
         L.gp_inject_wall.argtypes = [P, I, I64, C.c_float, fp, D, C.c_uint64, C.c_uint64, I]
@@ -229,6 +230,14 @@ class Context:
         _check(lib().gp_species_download(self.h, sid, n, C.byref(nn),
                                          *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in out]))
         return out
+
+    def read(self, sid, first, count):
+        """particles first .. first+count-1 in store order (clipped to the count)"""
+        out = [np.empty(max(count, 0), np.float32) for _ in range(5)]
+        nn = C.c_int64()
+        _check(lib().gp_species_read(self.h, sid, first, count, C.byref(nn),
+                                     *[a.ctypes.data_as(C.POINTER(C.c_float)) for a in out]))
+        return [a[:nn.value] for a in out]
 
     def acc(self, sid, which=G_RHO_ACC):
         shape = {G_RHO_ACC: (self.ny + 1, self.nx + 1), G_JX_ACC: (self.ny + 1, self.nx + 2),

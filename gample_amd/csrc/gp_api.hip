@@ -3,6 +3,7 @@
 // Every entry point validates its arguments on the host, converts C++
 // failures into status codes and records the message for gp_last_error().
 #include "gp_internal.h"
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -239,6 +240,27 @@ int gp_species_download(gp_ctx *c, int sid, int64_t cap, int64_t *n, float *x, f
         if (vx) GP_HIP(hipMemcpyAsync(vx, S.a.vx, b, hipMemcpyDeviceToHost, c->stream));
         if (vy) GP_HIP(hipMemcpyAsync(vy, S.a.vy, b, hipMemcpyDeviceToHost, c->stream));
         if (vz) GP_HIP(hipMemcpyAsync(vz, S.a.vz, b, hipMemcpyDeviceToHost, c->stream));
+    }
+    GP_HIP(hipStreamSynchronize(c->stream));
+    return GP_OK;
+}
+
+int gp_species_read(gp_ctx *c, int sid, int64_t first, int64_t count, int64_t *n, float *x, float *y, float *vx,
+                    float *vy, float *vz)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    gp_species &S = c->sp[sid];
+    if (first < 0 || count < 0) return gp_fail(GP_ERR_ARG, "negative range");
+    const int64_t m = first >= S.n ? 0 : std::min(count, S.n - first);
+    if (n) *n = m;
+    const size_t b = (size_t)m * 4, o = (size_t)first;
+    if (m > 0) {
+        if (x) GP_HIP(hipMemcpyAsync(x, S.a.x + o, b, hipMemcpyDeviceToHost, c->stream));
+        if (y) GP_HIP(hipMemcpyAsync(y, S.a.y + o, b, hipMemcpyDeviceToHost, c->stream));
+        if (vx) GP_HIP(hipMemcpyAsync(vx, S.a.vx + o, b, hipMemcpyDeviceToHost, c->stream));
+        if (vy) GP_HIP(hipMemcpyAsync(vy, S.a.vy + o, b, hipMemcpyDeviceToHost, c->stream));
+        if (vz) GP_HIP(hipMemcpyAsync(vz, S.a.vz + o, b, hipMemcpyDeviceToHost, c->stream));
     }
     GP_HIP(hipStreamSynchronize(c->stream));
     return GP_OK;

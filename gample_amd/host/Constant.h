@@ -14,5 +14,8 @@ constexpr double VtoG = 1e6 / c;            // V/m -> statV/cm
 constexpr double VtosV = 1e8 / c;           // V -> statV
 constexpr double sVtoV = 1 / (1e8 / c);
 constexpr double JtosJ = 1 / (10 / c * 1e4);
+constexpr double GtoV = 1 / (1e6 / c);       // statV/cm -> V/m
+constexpr double sJtoJ = 10 / c * 1e4;       // statA/cm^2 -> A/m^2
+constexpr double GtoT = 1e-4;                // G -> T
 constexpr double TtoG = 1e4;
 }  // namespace gample

@@ -1,4 +1,5 @@
-// main.cpp -- `gample -i condition.json [-o tag] [--unfused] [--dump-config]`
+// main.cpp -- `gample -i condition.json [-o tag] [--unfused] [--dump-config]`; binary outputs and
+// restart files in bin/ as the reference writes them (FileIO.cpp)
 //
 // The driver loop of the reference (src/main.mm:28-263): parse the
 // condition file, build the modules, load particles, initial deposit and
@@ -16,6 +17,7 @@
 
 #include "Constant.h"
 #include "Init.h"
+#include "FileIO.h"
 #include "Modules.h"
 
 using namespace gample;
@@ -83,10 +85,6 @@ int main(int argc, char **argv)
         tag = b;
     }
     const auto &T = init.paramForTimeIntegration;
-    if (T.Start != 0) {
-        std::fprintf(stderr, "[Fatal] restart (Start != 0) is not supported by this build\n");
-        return 1;
-    }
     if (!init.particleSources.empty())
         std::fprintf(stderr, "[warn] SourceForParticle entries (artificial ionization / hollow cathode) are "
                              "not run by this build\n");
@@ -100,12 +98,24 @@ int main(int argc, char **argv)
         Collision col(fld, init, ptcl);
         (void)col;
         Moment mom(fld);
-        (void)mom;
-        // initial field (main.mm:137-141)
+        const bool out_field = T.FieldOutput != 0, out_ptcl = T.ParticleOutput != 0;
+        // the reference tests FieldOutput before taking the cycle modulo
+        // ProgressOutput (main.mm:229); a zero ProgressOutput writes nothing
+        const bool out_prog = T.FieldOutput != 0 && T.ProgressOutput != 0;
+        ensureOutputDir();
+        // restart (main.mm:125-133): particle states of cycle Start
+        if (T.Start != 0) {
+            loadProgress(T.Start, ptcl, init);
+            for (auto &p : ptcl) check(gp_sort(fld.ctx, p.sid), "sort");
+            std::printf("restarted from cycle %d\n", T.Start);
+        }
+        // initial field (main.mm:135-147) and its output
         fld.resetChargeDensity();
         for (auto &p : ptcl) p.integrateChargeDensity();
         gp_solve_stats st0 = fld.solvePoisson(0.0, nullptr);
         std::printf("initial solve: %d iterations, relres %.3e\n", st0.iterations, st0.relres);
+        outputField(T.Start, fld, init);
+        for (auto &p : ptcl) outputMoments(T.Start, fld, mom.integrateMoments(p), p.pName, init);
         if (fused) {   // accumulators for cycle 1 (the solve consumed them)
             fld.resetChargeDensity();
             for (auto &p : ptcl) p.integrateChargeDensity();
@@ -142,6 +152,16 @@ int main(int argc, char **argv)
                 if (sort_every > 0 && cycle % sort_every == 0)
                     for (auto &p : ptcl) check(gp_sort(fld.ctx, p.sid), "sort");
             }
+            // outputs of main.mm:190-233.  The fields after the push are those
+            // of the solve (the push changes no field), so the fused step
+            // writes them here too.
+            if (out_field && cycle % T.FieldOutput == 0) outputField(cycle, fld, init);
+            for (auto &p : ptcl) {
+                if (out_ptcl && cycle % T.ParticleOutput == 0) outputPhaseSpace(cycle, p, init);
+                if (out_field && cycle % T.FieldOutput == 0)
+                    outputMoments(cycle, fld, mom.integrateMoments(p), p.pName, init);
+            }
+            if (out_prog && cycle % T.ProgressOutput == 0) saveProgress(cycle, ptcl, init);
             auto c1 = std::chrono::steady_clock::now();
             if (logc) {
                 double phi, ex, ey;

@@ -154,6 +154,11 @@ int  gp_species_upload(gp_ctx *ctx, int sid, int64_t n, const float *x, const fl
 /* device -> host copy (replaces reads of particleBuffer contents) */
 int  gp_species_download(gp_ctx *ctx, int sid, int64_t cap, int64_t *n, float *x, float *y,
                          float *vx, float *vy, float *vz);
+/* particles first .. first+count-1 in the store's current order, clipped to
+ * the live count (*n receives the number copied): the reads of the first
+ * particles of particleBuffer in outputPhaseSpace (FileIO.mm:4-39) */
+int  gp_species_read(gp_ctx *ctx, int sid, int64_t first, int64_t count, int64_t *n, float *x, float *y,
+                     float *vx, float *vy, float *vz);
 int  gp_species_count(gp_ctx *ctx, int sid, int64_t *n);
 /* per-species int64 accumulators (deposit output before scaling) */
 int  gp_species_grid_get(gp_ctx *ctx, int sid, int which, int64_t *host, int64_t count);
