@@ -98,6 +98,7 @@ struct gp_solver {
     double *hscal = nullptr;   // pinned host mirror
     int red_blocks = 0;
     int last_it = 4;           // inner iterations of the previous solve
+    int last_it_r[4] = {4, 4, 4, 4};   // the same per refinement round (first batch sizes)
     int mg_ref_it = 0;         // iterations of the last plain MG-PCG solve
 };
 

@@ -34,6 +34,10 @@ constexpr int HU = 2, WU = TB + 2 * HU;   // window of k_rb_sweep (one full swee
 #define MG_SWEEPS 1
 #endif
 constexpr int NSW = MG_SWEEPS;            // smoothing sweeps per side of the V-cycle
+// smallest reduction asked of one FP32 inner solve (its own rounding limits it)
+#ifndef MG_INNER_FLOOR
+#define MG_INNER_FLOOR 2e-6
+#endif
 #ifndef MG_BY
 #define MG_BY 32
 #endif
@@ -894,8 +898,9 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
 // iterations are enqueued in batches; a device-side flag (set by the
 // residual reduction at the target or the iteration limit) turns the rest of
 // a batch into no-ops, so the host synchronises once per batch.
-int pcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int *iters)
+int pcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int *iters, int round = 0)
 {
+    int &last = s.last_it_r[round < 3 ? round : 3];
     const Lev L0 = mklev(s.lv[0]);
     const int n = s.n, nb = s.red_blocks;
     const int ge = std::min(1024, gp_cdiv(n, 256));
@@ -918,7 +923,7 @@ int pcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int *iters)
     // reduction it consumes (beta in the p update, alpha in the x/r update,
     // the residual test in the V-cycle's first kernel), so an iteration is
     // one launch per kernel with no separate reduction launches.
-    for (int batch = std::max(2, s.last_it);; batch = 1) {
+    for (int batch = std::max(1, last);; batch = 1) {
         for (int b = 0; b < batch && enq < maxit; b++, enq++) {
             const int it = enq + 1;
             const int cur = (it & 1) ? S_RZ : S_RZ2, prev = (it & 1) ? S_RZ2 : S_RZ;
@@ -936,7 +941,10 @@ int pcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int *iters)
         if (s.hscal[S_DONE] != 0.0 || enq >= maxit) break;
     }
     *iters = (int)s.hscal[S_IT];
-    s.last_it = *iters + (*iters >= enq ? 1 : 0);
+    // the next solve's same round starts with this many iterations in one
+    // batch (one more when this one ran out before converging)
+    last = *iters + (*iters >= enq ? 1 : 0);
+    s.last_it = last;
     return GP_OK;
 }
 
@@ -1180,7 +1188,7 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
         k_sub_mean64<<<ge, 256, 0, st>>>(s.b64, n, s.scal, S_MEAN);
         GP_HIP(hipGetLastError());
     }
-    int total_it = 0, outer = 0, nn_it = 0, fell_back = 0;
+    int total_it = 0, outer = 0, nn_it = 0, fell_back = 0, pcg_round = 0;
     const bool warm = hk && hk->warm, flex = hk && hk->precond;
     bool warm_pending = warm;
     const int cap = (hk && hk->cap > 0) ? hk->cap : (s.mg_ref_it > 0 ? s.mg_ref_it : 100);
@@ -1213,7 +1221,7 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
         }
         // ask the FP32 solve for just the reduction still needed (FP32 cannot
         // usefully go below ~1e-6 of its own starting residual)
-        const double inner = std::max(2e-6, std::min(1e-2, 0.3 * tol / rel));
+        const double inner = std::max(MG_INNER_FLOOR, std::min(1e-2, 0.3 * tol / rel));
         int it = 0;
         if (flex && !fell_back) {
             int conv = 0;
@@ -1223,7 +1231,7 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
             nn_it += it;
             if (!conv && nn_it >= cap) fell_back = 1;
         } else {
-            int rc = pcg32(st, s, inner, std::max(1, maxiter - total_it), &it);
+            int rc = pcg32(st, s, inner, std::max(1, maxiter - total_it), &it, pcg_round++);
             if (rc) return rc;
         }
         total_it += it;
