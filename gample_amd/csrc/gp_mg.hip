@@ -622,20 +622,12 @@ __global__ void k_final(const double *part, int nb, double *scal, int slot, int 
     }
     if (threadIdx.x == 0) apply_op(scal, slot, op, sh[0]);
 }
-// start of an inner solve: target = tol * ||r0||, flag up at once if r0 = 0
 // the residual test alone (a hierarchy without grid levels above its
 // coarsest, solved by sweeps)
 __global__ void k_gate(Gate g)
 {
     __shared__ double shr[256];
     (void)gate_converged(g, shr, threadIdx.x, true);
-}
-__global__ void k_start(double *scal, double inner_tol, int maxit)
-{
-    scal[S_TGT] = inner_tol * sqrt(scal[S_RR]);
-    scal[S_DONE] = scal[S_RR] > 0.0 ? 0.0 : 1.0;
-    scal[S_IT] = 0.0;
-    scal[S_MAXIT] = (double)maxit;
 }
 // p_new = z + beta p_old (beta from scal), q = S p_new, partial p_new.q
 // Iteration it of the PCG: rz_it = r.z from the partials of the V-cycle
@@ -788,9 +780,48 @@ __global__ __launch_bounds__(RB) void k_resid64(const double *x, const double *b
     s = block_sum(abb, sh);
     if (threadIdx.x == 0) part_bb[blockIdx.x] = s;
 }
-__global__ void k_to_f32(const double *r, float *rf, int n, double scale)
+// start of an inner solve: rf = FP32 of scale * r, e = p = 0, and the block
+// partials of rf.rf (FP64, one per block, grid-stride order)
+__global__ __launch_bounds__(RB) void k_inner_start(const double *r, float *rf, float *e, float *p, int n,
+                                                    double scale, double *part)
 {
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) rf[k] = (float)(r[k] * scale);
+    __shared__ double sh[RB / 64];
+    double acc = 0.0;
+    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
+        const float v = (float)(r[k] * scale);
+        rf[k] = v;
+        e[k] = 0.0f;
+        p[k] = 0.0f;
+        acc += (double)v * (double)v;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// both reductions of the PCG start (r.z of the first V-cycle, r.r of the
+// start kernel) and the inner solve's scalars, in one launch
+__global__ void k_pcg_start(const double *rz_part, int nrz, const double *rr_part, int nrr, double *scal,
+                            double inner_tol, int maxit)
+{
+    __shared__ double sh[256];
+    const int tid = threadIdx.x;
+    const double rz = sum_parts(rz_part, nrz, sh, tid);
+    const double rr = sum_parts(rr_part, nrr, sh, tid);
+    if (tid == 0) {
+        scal[S_RZ] = rz;
+        scal[S_RR] = rr;
+        scal[S_TGT] = inner_tol * sqrt(rr);
+        scal[S_DONE] = rr > 0.0 ? 0.0 : 1.0;
+        scal[S_IT] = 0.0;
+        scal[S_MAXIT] = (double)maxit;
+    }
+}
+// two reductions into two slots in one launch (the refinement's r.r, b.b)
+__global__ void k_final2(const double *pa, const double *pb, int nb, double *scal, int sa, int sb)
+{
+    __shared__ double sh[256];
+    const double a = sum_parts(pa, nb, sh, threadIdx.x);
+    const double b = sum_parts(pb, nb, sh, threadIdx.x);
+    if (threadIdx.x == 0) { scal[sa] = a; scal[sb] = b; }
 }
 __global__ void k_axpy64(double *x, const float *e, int n, double scale)
 {
@@ -904,18 +935,15 @@ int pcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int *iters,
     const Lev L0 = mklev(s.lv[0]);
     const int n = s.n, nb = s.red_blocks;
     const int ge = std::min(1024, gp_cdiv(n, 256));
+    // e = p = 0 and the partials of r.r come from the start kernel
+    // (k_inner_start, launched by the caller with these buffers)
     float *p = s.p, *p2 = s.q2;
-    k_zero_f<<<ge, 256, 0, st>>>(s.e, n);
-    k_zero_f<<<ge, 256, 0, st>>>(p, n);
-    k_set_scalar<<<1, 1, 0, st>>>(s.scal, S_DONE, 0.0);
     double *red_pq = s.red, *red_rr = s.red + 1024, *red_rz = s.red + 2048;
+    (void)ge;
     int np = 0;
     int rc = vcycle(st, s, red_rz, &np, nullptr);          // z = M r, partial r.z
     if (rc) return rc;
-    k_final<<<1, RB, 0, st>>>(red_rz, np, s.scal, S_RZ, 0);
-    k_dot<<<nb, RB, 0, st>>>(s.r, s.r, n, red_rr);
-    k_final<<<1, RB, 0, st>>>(red_rr, nb, s.scal, S_RR, 0);
-    k_start<<<1, 1, 0, st>>>(s.scal, inner_tol, maxit);
+    k_pcg_start<<<1, 256, 0, st>>>(red_rz, np, red_rr, nb, s.scal, inner_tol, maxit);
     GP_HIP(hipGetLastError());
     int enq = 0;
     // first batch: the previous solve's iteration count (one more if it
@@ -1196,8 +1224,7 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
     int status = GP_OK;
     for (;;) {
         k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb);
-        k_final<<<1, RB, 0, st>>>(s.red, nb, s.scal, S_RR, 0);
-        k_final<<<1, RB, 0, st>>>(s.red + nb, nb, s.scal, S_BB, 0);
+        k_final2<<<1, 256, 0, st>>>(s.red, s.red + nb, nb, s.scal, S_RR, S_BB);
         GP_HIP(hipGetLastError());
         GP_HIP(hipMemcpyAsync(s.hscal, s.scal, S_NS * sizeof(double), hipMemcpyDeviceToHost, st));
         GP_HIP(hipStreamSynchronize(st));
@@ -1208,7 +1235,8 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
         if (total_it >= maxiter || outer >= 40) { status = GP_NOT_CONVERGED; break; }
         const double rms = std::sqrt(rr / n);
         const double sc = rms > 0 ? 1.0 / rms : 1.0;
-        k_to_f32<<<ge, 256, 0, st>>>(s.r64, s.r, n, sc);
+        // FP32 residual, e = p = 0, partials of r.r for the PCG start
+        k_inner_start<<<nb, RB, 0, st>>>(s.r64, s.r, s.e, s.p, n, sc, s.red + 1024);
         GP_HIP(hipGetLastError());
         if (warm_pending) {
             int rc = hk->warm(hk->ctx, s.e, s.r);
