@@ -128,6 +128,17 @@ __device__ __forceinline__ float prolong(const float *uc, const Lev &F, const Le
     return 0.25f * ((a00 + a10) + (a01 + a11));
 }
 
+// a level descriptor with its axis types fixed at compile time (the
+// kernels are instantiated per pair of axis types, so every boundary test
+// on an axis type folds away)
+template <int TX, int TY>
+__device__ __forceinline__ Lev fixed(Lev L)
+{
+    L.tx = TX;
+    L.ty = TY;
+    return L;
+}
+
 enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_TMP, S_BB, S_MEAN, S_DONE, S_TGT, S_IT, S_MAXIT, S_RZ2, S_NS };
 // every kernel of an iteration returns at once when the device-side
 // convergence flag is set (the host enqueues iterations in batches)
@@ -186,10 +197,11 @@ __device__ bool gate_converged(const Gate &g, double *sh, int tid, bool block0)
 // sweeps exactly on the tile and its 1-wide rim (needed by the restriction).
 // The window's global loads are all issued before any is consumed.
 // ---------------------------------------------------------------------------
-template <int NS>
-__global__ __launch_bounds__(NT) void k_mg_down(const float *f, float *u, float *fc, Lev L, Lev C, const double *scal,
+template <int NS, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_mg_down(const float *f, float *u, float *fc, Lev Lin, Lev Cin, const double *scal,
                                                  Gate gt)
 {
+    const Lev L = fixed<TX, TY>(Lin), C = fixed<TX, TY>(Cin);
     constexpr int HD = 2 * NS + 1, WD = TB + 2 * HD, NE = (WD * WD + NT - 1) / NT;
     __shared__ float su[WD * WD], sf[WD * WD], sr[(TB + 2) * (TB + 2)];
     __shared__ double shr[256];
@@ -293,10 +305,11 @@ __global__ __launch_bounds__(NT) void k_mg_down(const float *f, float *u, float 
 // halo.  Optional partial of f.u (the CG's r.z at the finest level).
 // blockDim (32, BY)
 // ---------------------------------------------------------------------------
-template <int NS>
-__global__ __launch_bounds__(NT) void k_mg_up(const float *upre, const float *f, const float *uc, float *u, Lev L, Lev C,
-                                               double *part, const double *scal)
+template <int NS, int TX, int TY>
+__global__ __launch_bounds__(NT) void k_mg_up(const float *upre, const float *f, const float *uc, float *u, Lev Lin,
+                                               Lev Cin, double *part, const double *scal)
 {
+    const Lev L = fixed<TX, TY>(Lin), C = fixed<TX, TY>(Cin);
     constexpr int HU = 2 * NS, WU = TB + 2 * HU, NE = (WU * WU + NT - 1) / NT;
     __shared__ float su[WU * WU], sf[WU * WU];
     __shared__ double red[NT / 64];
@@ -469,6 +482,7 @@ __device__ void bot_project(float *a, const Lev &L, float *scratch)
     __syncthreads();
 }
 
+template <int TX, int TY>
 __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uout, Bottom B, double *part,
                                                     const double *scal, Gate gt)
 {
@@ -481,7 +495,7 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
     const int tid = threadIdx.x + 32 * threadIdx.y;
     if (gt.rr_part && gate_converged(gt, shr, tid, true)) return;
     {
-        const Lev &L = B.lv[0];
+        const Lev L = fixed<TX, TY>(B.lv[0]);
         const int n = L.mx * L.my;
         float *f = lds + B.off[0] + n;
         float v[BOT_MAX / 1024];
@@ -498,7 +512,7 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
     }
     __syncthreads();
     for (int l = 0; l + 1 < B.nl; l++) {
-        const Lev &L = B.lv[l], &C = B.lv[l + 1];
+        const Lev L = fixed<TX, TY>(B.lv[l]), C = fixed<TX, TY>(B.lv[l + 1]);
         const int n = L.mx * L.my, nc = C.mx * C.my;
         float *u = lds + B.off[l], *f = u + n;
         float *fc = lds + B.off[l + 1] + nc;
@@ -527,7 +541,7 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         __syncthreads();
     }
     {
-        const Lev &L = B.lv[B.nl - 1];
+        const Lev L = fixed<TX, TY>(B.lv[B.nl - 1]);
         const int n = L.mx * L.my;
         float *u = lds + B.off[B.nl - 1], *f = u + n;
         if (B.singular) bot_project(f, L, scratch);
@@ -563,7 +577,7 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         if (B.singular) bot_project(u, L, scratch);
     }
     for (int l = B.nl - 2; l >= 0; l--) {
-        const Lev &L = B.lv[l], &C = B.lv[l + 1];
+        const Lev L = fixed<TX, TY>(B.lv[l]), C = fixed<TX, TY>(B.lv[l + 1]);
         const int n = L.mx * L.my;
         float *u = lds + B.off[l], *f = u + n;
         const float *uc = lds + B.off[l + 1];
@@ -572,7 +586,7 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         for (int sw = 0; sw < NSW; sw++) { bot_sweep(u, f, L, 1); bot_sweep(u, f, L, 0); }
     }
     {
-        const Lev &L = B.lv[0];
+        const Lev L = fixed<TX, TY>(B.lv[0]);
         const int n = L.mx * L.my;
         const float *u = lds + B.off[0], *f = u + n;
         double acc = 0.0;
@@ -633,10 +647,12 @@ __global__ void k_gate(Gate g)
 // Iteration it of the PCG: rz_it = r.z from the partials of the V-cycle
 // (it > 1), beta = rz_it / rz_(it-1); block 0 stores rz_it in slot cur
 // (the previous value stays in slot prev, which the other blocks read).
-__global__ __launch_bounds__(RB) void k_pupdate_spmv(const float *z, const float *pold, float *pnew, float *q, Lev L,
+template <int TX, int TY>
+__global__ __launch_bounds__(RB) void k_pupdate_spmv(const float *z, const float *pold, float *pnew, float *q, Lev Lin,
                                                     double *scal, double *part, const double *rz_part, int nrz,
                                                     int first, int cur, int prev)
 {
+    const Lev L = fixed<TX, TY>(Lin);
     __shared__ double sh[RB / 64];
     __shared__ double shr[256];
     if (done(scal)) return;
@@ -859,6 +875,22 @@ inline dim3 tiles(const Lev &L) { return dim3((L.mx + TB - 1) / TB, (L.my + TB -
 // no-op once the CG has converged.
 // gate (may be null): the residual test of the PCG iteration, made by the
 // V-cycle's first kernel
+// launch CALL(TX, TY) for the runtime axis types (GP_AX_PER 0, DIR 1, NEU 2)
+#define MG_AXES(tx, ty, CALL)                                            \
+    do {                                                                 \
+        switch ((tx) * 3 + (ty)) {                                       \
+        case 0: CALL(0, 0); break;                                       \
+        case 1: CALL(0, 1); break;                                       \
+        case 2: CALL(0, 2); break;                                       \
+        case 3: CALL(1, 0); break;                                       \
+        case 4: CALL(1, 1); break;                                       \
+        case 5: CALL(1, 2); break;                                       \
+        case 6: CALL(2, 0); break;                                       \
+        case 7: CALL(2, 1); break;                                       \
+        default: CALL(2, 2); break;                                      \
+        }                                                                \
+    } while (0)
+
 int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double *scal, const Gate *gate = nullptr)
 {
     const Gate nog{nullptr, 0, 0, nullptr};
@@ -869,7 +901,10 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
     const dim3 blk(32, BY), blk8(32, 8);
     for (int l = 0; l < std::min(lb, nl - 1); l++) {
         const Lev L = mklev(s.lv[l]), C = mklev(s.lv[l + 1]);
-        k_mg_down<NSW><<<tiles(L), blk, 0, st>>>(s.lv[l].f, s.lv[l].r, s.lv[l + 1].f, L, C, scal, take());
+        const Gate g = take();
+#define MG_DOWN(X, Y) k_mg_down<NSW, X, Y><<<tiles(L), blk, 0, st>>>(s.lv[l].f, s.lv[l].r, s.lv[l + 1].f, L, C, scal, g)
+        MG_AXES(L.tx, L.ty, MG_DOWN);
+#undef MG_DOWN
     }
     GP_HIP(hipGetLastError());
     *nparts = 0;
@@ -887,8 +922,11 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
         B.sweeps = std::max(2, std::max(Lc.mx, Lc.my) / 2);
         B.dense = s.dinv;
         const bool top = lb == 0;
-        k_mg_bottom<<<1, dim3(32, 32), off * sizeof(float), st>>>(s.lv[lb].f, s.lv[lb].u, B, top ? part : nullptr,
-                                                                  scal, take());
+        double *bp = top ? part : nullptr;
+        const Gate bg = take();
+#define MG_BOT(X, Y) k_mg_bottom<X, Y><<<1, dim3(32, 32), off * sizeof(float), st>>>(s.lv[lb].f, s.lv[lb].u, B, bp, scal, bg)
+        MG_AXES(B.lv[0].tx, B.lv[0].ty, MG_BOT);
+#undef MG_BOT
         GP_HIP(hipGetLastError());
         if (top) *nparts = 1;
     } else {
@@ -917,8 +955,10 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
     for (int l = std::min(lb, nl - 1) - 1; l >= 0; l--) {
         const Lev L = mklev(s.lv[l]), C = mklev(s.lv[l + 1]);
         const dim3 g = tiles(L);
-        k_mg_up<NSW><<<g, blk, 0, st>>>(s.lv[l].r, s.lv[l].f, s.lv[l + 1].u, s.lv[l].u, L, C, l == 0 ? part : nullptr,
-                                        scal);
+        double *pp = l == 0 ? part : nullptr;
+#define MG_UP(X, Y) k_mg_up<NSW, X, Y><<<g, blk, 0, st>>>(s.lv[l].r, s.lv[l].f, s.lv[l + 1].u, s.lv[l].u, L, C, pp, scal)
+        MG_AXES(L.tx, L.ty, MG_UP);
+#undef MG_UP
         if (l == 0) *nparts = (int)(g.x * g.y);
     }
     GP_HIP(hipGetLastError());
@@ -955,7 +995,9 @@ int pcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int *iters,
         for (int b = 0; b < batch && enq < maxit; b++, enq++) {
             const int it = enq + 1;
             const int cur = (it & 1) ? S_RZ : S_RZ2, prev = (it & 1) ? S_RZ2 : S_RZ;
-            k_pupdate_spmv<<<nb, RB, 0, st>>>(s.z, p, p2, s.q, L0, s.scal, red_pq, red_rz, np, it == 1, cur, prev);
+#define MG_PUP(X, Y) k_pupdate_spmv<X, Y><<<nb, RB, 0, st>>>(s.z, p, p2, s.q, L0, s.scal, red_pq, red_rz, np, it == 1, cur, prev)
+            MG_AXES(L0.tx, L0.ty, MG_PUP);
+#undef MG_PUP
             std::swap(p, p2);
             k_update_xr<<<nb, RB, 0, st>>>(s.e, s.r, p, s.q, n, s.scal, red_rr, red_pq, nb, cur);
             const Gate g{red_rr, nb, it, s.scal};
