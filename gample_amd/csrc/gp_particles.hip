@@ -221,6 +221,19 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
     const float loy = (float)a.ngb, hiy = (float)(a.ngb + a.ngy);
     double ke = 0.0;
     int drift = 0;   // live particles gathered outside their tile window
+    // uniform B: btz = constB Bz (scaled by fN1 at Boris order 4/6, with
+    // btbt from the unscaled btz), s = 2 btz / (1 + btbt)
+    float ub_btz = 0.0f, ub_ssz = 0.0f, ub_fn1 = 1.0f;
+    if constexpr (UNIB) {
+        float btz = a.constB * a.bz0;
+        const float btbt = btz * btz;
+        if (a.boris == 4 || a.boris == 6) {
+            ub_fn1 = (a.boris == 4) ? (1.0f + btbt / 3.0f) : (1.0f + btbt / 3.0f + 2.0f * btbt * btbt / 15.0f);
+            btz = ub_fn1 * btz;
+        }
+        ub_btz = btz;
+        ub_ssz = (2.0f * btz) / (1.0f + btbt);
+    }
     int moved = 0;   // live particles whose cell changed in this push
 
     // The segment is walked in sub-segments of at most 2^28 particles, so that
@@ -284,18 +297,27 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
             }
             // Boris (Particle.mm:156-209) for the gathered components
             // Ex, Ey, Bz; Ez = Bx = By = 0 exactly as in the reference.
+            // With a uniform B the rotation factors are the kernel-wide
+            // constants computed before the loop (the same operations).
             const float ex0 = a.constE * Epx, ey0 = a.constE * Epy;
-            float etx = ex0, ety = ey0, btz = a.constB * Bpz;
-            const float btbt = btz * btz;
-            if (a.boris == 4 || a.boris == 6) {
-                const float fN1 = (a.boris == 4) ? (1.0f + btbt / 3.0f)
-                                                 : (1.0f + btbt / 3.0f + 2.0f * btbt * btbt / 15.0f);
-                etx = fN1 * etx; ety = fN1 * ety; btz = fN1 * btz;
+            float etx = ex0, ety = ey0, btz, ssz;
+            if constexpr (UNIB) {
+                btz = ub_btz;
+                ssz = ub_ssz;
+                if (a.boris == 4 || a.boris == 6) { etx = ub_fn1 * etx; ety = ub_fn1 * ety; }
+            } else {
+                btz = a.constB * Bpz;
+                const float btbt = btz * btz;
+                if (a.boris == 4 || a.boris == 6) {
+                    const float fN1 = (a.boris == 4) ? (1.0f + btbt / 3.0f)
+                                                     : (1.0f + btbt / 3.0f + 2.0f * btbt * btbt / 15.0f);
+                    etx = fN1 * etx; ety = fN1 * ety; btz = fN1 * btz;
+                }
+                ssz = (2.0f * btz) / (1.0f + btbt);
             }
             const float umx = vx + etx, umy = vy + ety;
             const float v0x = __builtin_fmaf(umy, btz, umx);
             const float v0y = __builtin_fmaf(-umx, btz, umy);
-            const float ssz = (2.0f * btz) / (1.0f + btbt);
             const float upx = __builtin_fmaf(v0y, ssz, umx);
             const float upy = __builtin_fmaf(-v0x, ssz, umy);
             vx = upx + ex0;
