@@ -372,11 +372,16 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
             unsigned qx[NW], qy[NW];
 #pragma unroll
             for (int k = 0; k < NW; k++) { qx[k] = q23(sx[k]); qy[k] = q23(sy[k]); }
-            // window cell (the stencil's first node is window node (wi, wj))
+            // window cell (the stencil's first node is window node (wi, wj));
+            // on an edge tile, a cell outside the window may have its image
+            // one period away inside it (the rare branch)
             int wi = i2 - cx0, wj = j1 - cy0;
-            if (img_x) wi += wi < 0 ? a.ngx : (wi >= WIN ? -a.ngx : 0);
-            if (img_y) wj += wj < 0 ? a.ngy : (wj >= WIN ? -a.ngy : 0);
-            const bool inw = tiled && (unsigned)wi < (unsigned)WIN && (unsigned)wj < (unsigned)WIN;
+            bool inw = tiled && (unsigned)wi < (unsigned)WIN && (unsigned)wj < (unsigned)WIN;
+            if (!inw && (img_x || img_y)) {
+                if (img_x) wi += wi < 0 ? a.ngx : (wi >= WIN ? -a.ngx : 0);
+                if (img_y) wj += wj < 0 ? a.ngy : (wj >= WIN ? -a.ngy : 0);
+                inw = (unsigned)wi < (unsigned)WIN && (unsigned)wj < (unsigned)WIN;
+            }
             if constexpr (PUSH) drift += inw ? 0 : 1;
             if (inw) {
                 unsigned long long *r = sRho + wi + wj * W;
