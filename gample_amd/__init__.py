@@ -87,7 +87,7 @@ EXPORTS = [
     "gp_ctx_create", "gp_ctx_destroy", "gp_last_error", "gp_sync", "gp_dims",
     "gp_species_create", "gp_species_upload", "gp_species_download", "gp_species_count",
     "gp_species_grid_get", "gp_species_read", "gp_load_pair", "gp_inject_wall", "gp_reset_grids", "gp_deposit",
-    "gp_solve_fields", "gp_solve_darwin", "gp_push", "gp_reduce", "gp_sort", "gp_set_sort_drift", "gp_sort_stats", "gp_moments",
+    "gp_solve_fields", "gp_solve_darwin", "gp_darwin_stats", "gp_push", "gp_reduce", "gp_sort", "gp_set_sort_drift", "gp_sort_stats", "gp_moments",
     "gp_step", "gp_field_get", "gp_field_set", "gp_field_energy", "gp_timing", "gp_kernel_time",
     "gp_nn_init", "gp_nn_train", "gp_nn_infer", "gp_nn_save", "gp_nn_load", "gp_nn_add_pair", "gp_nn_params",
     "gp_nn_dims", "gp_nn_forward", "gp_nn_grad", "gp_solve_rhs", "gp_nn_config",
@@ -122,6 +122,7 @@ def lib() -> C.CDLL:
         L.gp_deposit.argtypes = [P, I]
         L.gp_solve_fields.argtypes = [P, D, D, I, I, C.POINTER(SolveStats), C.POINTER(Diag)]
         L.gp_solve_darwin.argtypes = [P, D, D, I, I, C.POINTER(SolveStats)]
+        L.gp_darwin_stats.argtypes = [P, C.POINTER(SolveStats)]
         L.gp_push.argtypes = [P, I, D, I]
         L.gp_reduce.argtypes = [P, I, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         L.gp_sort.argtypes = [P, I]
@@ -281,6 +282,12 @@ class Context:
         st = (SolveStats * 3)()
         rc = _check(lib().gp_solve_darwin(self.h, dt, tol, maxiter, solver, st), allow_nc=True)
         return list(st), rc
+
+    def darwin_stats(self):
+        """statistics of the chi, Ax, Ay solves of the last Darwin solve"""
+        st = (SolveStats * 3)()
+        _check(lib().gp_darwin_stats(self.h, st))
+        return list(st)
 
     def push(self, sid, dt, deposit=0):
         _check(lib().gp_push(self.h, sid, dt, deposit))

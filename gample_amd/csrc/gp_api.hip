@@ -363,7 +363,20 @@ int gp_solve_darwin(gp_ctx *c, double dt, double tol, int maxiter, int solver, g
 {
     GP_CHECK_CTX(c);
     if (!st) return gp_fail(GP_ERR_ARG, "null stats");
-    GP_TRY({ return gp_solve_darwin_impl(c, dt, tol, maxiter, solver, st); })
+    GP_TRY({
+        const int rc = gp_solve_darwin_impl(c, dt, tol, maxiter, solver, st);
+        if (rc >= 0)
+            for (int k = 0; k < 3; k++) c->last_darwin[k] = st[k];
+        return rc;
+    })
+}
+
+int gp_darwin_stats(gp_ctx *c, gp_solve_stats out[3])
+{
+    GP_CHECK_CTX(c);
+    if (!out) return gp_fail(GP_ERR_ARG, "null stats");
+    for (int k = 0; k < 3; k++) out[k] = c->last_darwin[k];
+    return GP_OK;
 }
 
 int gp_push(gp_ctx *c, int sid, double dt, int deposit)
@@ -408,8 +421,9 @@ int gp_step(gp_ctx *c, double dt, double tol, int maxiter, int solver, int sort_
     if (status < 0) return status;
     if (c->cfg.em_model == 2) {
         gp_solve_stats ds[3];
-        int rc = gp_solve_darwin(c, dt, tol, maxiter, 0, ds);
+        int rc = gp_solve_darwin(c, dt, tol, maxiter, solver, ds);
         if (rc < 0) return rc;
+        if (rc == GP_NOT_CONVERGED) status = GP_NOT_CONVERGED;
     }
     bool has_delete = false;
     for (int w = 0; w < 4; w++) has_delete |= c->cfg.pbc[w] == GP_PBC_DELETE;

@@ -120,6 +120,7 @@ struct gp_ctx {
     gp_solver ps;                   // Poisson
     gp_solver ds;                   // Darwin (Neumann/periodic, singular)
     double *dsol_chi = nullptr, *dsol_ax = nullptr, *dsol_ay = nullptr, *last_ax = nullptr, *last_ay = nullptr;
+    gp_solve_stats last_darwin[3] = {};   // chi, Ax, Ay of the last Darwin solve
     double *red = nullptr;          // generic reduction scratch
     double *hred = nullptr;         // pinned
     int have_solution = 0;

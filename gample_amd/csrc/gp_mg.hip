@@ -38,6 +38,14 @@ constexpr int NSW = MG_SWEEPS;            // smoothing sweeps per side of the V-
 #ifndef MG_INNER_FLOOR
 #define MG_INNER_FLOOR 2e-6
 #endif
+// most FP32 iterations of one inner solve: where FP32 cannot reach the
+// inner target (a correction dominated by the operator's smoothest modes
+// carries rounding errors of ~eps32 cond(S) into the residual), the FP64
+// refinement restarts it from the true residual instead of letting the
+// recurrence drift
+#ifndef MG_INNER_CAP
+#define MG_INNER_CAP 20
+#endif
 #ifndef MG_BY
 #define MG_BY 32
 #endif
@@ -1012,8 +1020,9 @@ int pcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int *iters,
     }
     *iters = (int)s.hscal[S_IT];
     // the next solve's same round starts with this many iterations in one
-    // batch (one more when this one ran out before converging)
-    last = *iters + (*iters >= enq ? 1 : 0);
+    // batch (one more when this one ran out before converging); a solve
+    // stopped by its iteration limit leaves the count as it was
+    if (*iters < maxit) last = *iters + (*iters >= enq ? 1 : 0);
     s.last_it = last;
     return GP_OK;
 }
@@ -1252,10 +1261,19 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
     const int ge = std::min(1024, gp_cdiv(n, 256));
     const double cx = s.lv[0].hx2i, cy = s.lv[0].hy2i;
     auto t0 = std::chrono::steady_clock::now();
-    if (singular) {
-        k_sum64<<<nb, RB, 0, st>>>(s.b64, n, s.red);
+    // a singular (all-Neumann/periodic) system: b and every iterate are kept
+    // at zero mean.  The FP32 corrections carry a constant component that S
+    // does not see; left in x it would grow over the refinements until its
+    // FP64 rounding (ulp of the constant times the stencil) sets a floor on
+    // the residual
+    auto sub_mean = [&](double *a) {
+        k_sum64<<<nb, RB, 0, st>>>(a, n, s.red);
         k_final<<<1, RB, 0, st>>>(s.red, nb, s.scal, S_MEAN, 0);
-        k_sub_mean64<<<ge, 256, 0, st>>>(s.b64, n, s.scal, S_MEAN);
+        k_sub_mean64<<<ge, 256, 0, st>>>(a, n, s.scal, S_MEAN);
+    };
+    if (singular) {
+        sub_mean(s.b64);
+        sub_mean(s.x64);
         GP_HIP(hipGetLastError());
     }
     int total_it = 0, outer = 0, nn_it = 0, fell_back = 0, pcg_round = 0;
@@ -1284,6 +1302,7 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
             int rc = hk->warm(hk->ctx, s.e, s.r);
             if (rc) return rc;
             k_axpy64<<<ge, 256, 0, st>>>(s.x64, s.e, n, 1.0 / sc);
+            if (singular) sub_mean(s.x64);
             GP_HIP(hipGetLastError());
             warm_pending = false;
             outer++;
@@ -1301,19 +1320,14 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
             nn_it += it;
             if (!conv && nn_it >= cap) fell_back = 1;
         } else {
-            int rc = pcg32(st, s, inner, std::max(1, maxiter - total_it), &it, pcg_round++);
+            int rc = pcg32(st, s, inner, std::max(1, std::min(MG_INNER_CAP, maxiter - total_it)), &it, pcg_round++);
             if (rc) return rc;
         }
         total_it += it;
         k_axpy64<<<ge, 256, 0, st>>>(s.x64, s.e, n, 1.0 / sc);
+        if (singular) sub_mean(s.x64);
         GP_HIP(hipGetLastError());
         outer++;
-    }
-    if (singular) {
-        k_sum64<<<nb, RB, 0, st>>>(s.x64, n, s.red);
-        k_final<<<1, RB, 0, st>>>(s.red, nb, s.scal, S_MEAN, 0);
-        k_sub_mean64<<<ge, 256, 0, st>>>(s.x64, n, s.scal, S_MEAN);
-        GP_HIP(hipGetLastError());
     }
     GP_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();

@@ -147,7 +147,7 @@ gp_solve_stats EMField::solvePoisson(double dt, gp_diag *diag)
 }
 void EMField::solveVectorPotential(double dt, gp_solve_stats st[3])
 {
-    check(gp_solve_darwin(ctx, dt, tol, maxiter, 0, st), "solveVectorPotential");
+    check(gp_solve_darwin(ctx, dt, tol, maxiter, solver, st), "solveVectorPotential");
 }
 void EMField::probe(double &phi, double &ex, double &ey)
 {

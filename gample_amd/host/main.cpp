@@ -173,6 +173,17 @@ int main(int argc, char **argv)
                 std::snprintf(b, sizeof b, "%.9e", time); row["time"] = b;
                 row["iteration"] = std::to_string(st.iterations);
                 std::snprintf(b, sizeof b, "%.3e", st.relres); row["error"] = b;
+                if (init.flagForEquation.EMField == 2) {
+                    // the solveVectorPotential section's keys (EMField.mm:1349-1360)
+                    gp_solve_stats ds[3];
+                    check(gp_darwin_stats(fld.ctx, ds), "gp_darwin_stats");
+                    static const char *key[3] = {"decompose", "solveAx", "solveAy"};
+                    for (int k = 0; k < 3; k++) {
+                        row[std::string(key[k]) + "_iteration"] = std::to_string(ds[k].iterations);
+                        std::snprintf(b, sizeof b, "%.6e", ds[k].relres);
+                        row[std::string(key[k]) + "_error"] = b;
+                    }
+                }
                 std::snprintf(b, sizeof b, "%.6e", dg.we_ref * ergtoev); row["totalElectricEnergy"] = b;
                 std::snprintf(b, sizeof b, "%.9e", dg.we_cgs); row["fieldEnergyCGS"] = b;
                 std::snprintf(b, sizeof b, "%.9e", dg.kinetic[0] + dg.kinetic[1]); row["kineticEnergy"] = b;

@@ -191,7 +191,6 @@ int  gp_deposit(gp_ctx *ctx, int sid);
  * accumulators afterwards.  solver: 0 MG-CG, 1 NN warm start + FPCG. */
 int  gp_solve_fields(gp_ctx *ctx, double dt, double tol, int maxiter, int solver,
                      gp_solve_stats *st, gp_diag *diag);
-/* -[EMField solveVectorPotential:withLogger:] (EMField.mm:1138-1361) */
 /* Solve the Poisson system S x = b for a caller-given right-hand side b
  * (host FP64, (nkx+1)(nky+1) unknowns as GP_F_SOL), x in/out as the initial
  * guess, with the same solver choices as gp_solve_fields.  The context's own
@@ -199,8 +198,17 @@ int  gp_solve_fields(gp_ctx *ctx, double dt, double tol, int maxiter, int solver
  * counterpart (the reference solves only its own rhs, EMField.mm:853-866). */
 int  gp_solve_rhs(gp_ctx *ctx, const double *b, double *x, double tol, int maxiter, int solver,
                   gp_solve_stats *st);
+/* -[EMField solveVectorPotential:withLogger:] (EMField.mm:1138-1361): the
+ * transverse solves for chi, Ax, Ay from the deposited J, then delEx,
+ * delEy, delBz; st[0..2] are the three solves' statistics.  Returns
+ * GP_NOT_CONVERGED when any of them stopped short of tol. */
 int  gp_solve_darwin(gp_ctx *ctx, double dt, double tol, int maxiter, int solver,
                      gp_solve_stats st[3]);
+/* statistics of the three solves of the last Darwin solve, by
+ * gp_solve_darwin or inside gp_step (no reference counterpart: the
+ * reference logs the three AMGCL iteration counts and errors in its
+ * solveVectorPotential section, EMField.mm:1349-1360) */
+int  gp_darwin_stats(gp_ctx *ctx, gp_solve_stats out[3]);
 /* -[Particle update:withEMField:withMom:withLogger:] (Particle.mm:615-672 +
  * kernel :65-278).  deposit != 0 fuses the next step's deposit of the
  * pushed particles into the same kernel (Particle.mm:281-389). */
@@ -228,8 +236,11 @@ int  gp_moments(gp_ctx *ctx, int sid, double *host10);
 
 /* ---- whole step ------------------------------------------------------- */
 /* One cycle of the main loop (main.mm:154-254) for every species, with the
- * deposit fused into the push: fields from the current accumulators, push +
- * deposit, reduce, and a cell sort every sort_interval cycles (0 = never). */
+ * deposit fused into the push: fields from the current accumulators (and the
+ * Darwin solves when em_model = 2, statistics by gp_darwin_stats), push +
+ * deposit, reduce, and a cell sort every sort_interval cycles (0 = never).
+ * st holds the Poisson solve's statistics; GP_NOT_CONVERGED when any solve
+ * stopped short of tol. */
 int  gp_step(gp_ctx *ctx, double dt, double tol, int maxiter, int solver, int sort_interval,
              int cycle, gp_solve_stats *st, gp_diag *diag);
 
