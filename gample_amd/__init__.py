@@ -80,13 +80,14 @@ class Diag(C.Structure):
     _fields_ = [
         ("kinetic", C.c_double * 8), ("we_ref", C.c_double), ("we_cgs", C.c_double),
         ("joule", C.c_double), ("count", C.c_int64 * 8), ("absorbed", (C.c_int64 * 4) * 8),
+        ("mean_cathode", C.c_double),
     ]
 
 
 EXPORTS = [
     "gp_ctx_create", "gp_ctx_destroy", "gp_last_error", "gp_sync", "gp_dims",
     "gp_species_create", "gp_species_upload", "gp_species_download", "gp_species_count",
-    "gp_species_grid_get", "gp_species_read", "gp_load_pair", "gp_inject_wall", "gp_reset_grids", "gp_deposit",
+    "gp_species_grid_get", "gp_species_read", "gp_load_pair", "gp_generate", "gp_set_cathode", "gp_inject_wall", "gp_reset_grids", "gp_deposit",
     "gp_solve_fields", "gp_solve_darwin", "gp_darwin_stats", "gp_push", "gp_reduce", "gp_sort", "gp_set_sort_drift", "gp_sort_stats", "gp_moments",
     "gp_step", "gp_field_get", "gp_field_set", "gp_field_energy", "gp_timing", "gp_kernel_time",
     "gp_nn_init", "gp_nn_train", "gp_nn_infer", "gp_nn_save", "gp_nn_load", "gp_nn_add_pair", "gp_nn_params",
@@ -117,6 +118,8 @@ def lib() -> C.CDLL:
         L.gp_species_read.argtypes = [P, I, I64, I64, C.POINTER(C.c_int64), fp, fp, fp, fp, fp]
         L.gp_species_grid_get.argtypes = [P, I, I, C.POINTER(C.c_int64), I64]
         L.gp_load_pair.argtypes = [P, I, I, C.POINTER(Gen), C.c_uint64, I64, I64]
+        L.gp_generate.argtypes = [P, I, I, C.POINTER(Gen), I64, I]
+        L.gp_set_cathode.argtypes = [P, D]
         L.gp_inject_wall.argtypes = [P, I, I64, C.c_float, fp, D, C.c_uint64, C.c_uint64, I]
         L.gp_reset_grids.argtypes = [P]
         L.gp_deposit.argtypes = [P, I]
@@ -249,6 +252,14 @@ class Context:
 
     def load_pair(self, se, si, gen: Gen, seed: int, first_id: int, n: int):
         _check(lib().gp_load_pair(self.h, se, si, C.byref(gen), seed, first_id, n))
+
+    def generate(self, se, si, gen: Gen, addn: int, deposit=1):
+        """append addn generated pairs (electrons only for si < 0 or
+        gen_type 3) from the context's continuing generation streams"""
+        _check(lib().gp_generate(self.h, se, si, C.byref(gen), int(addn), deposit))
+
+    def set_cathode(self, x_cm):
+        _check(lib().gp_set_cathode(self.h, float(x_cm)))
 
     def inject_wall(self, sid, n, vth, u, dt, seed, stream, deposit=1):
         uu = (C.c_float * 3)(*u)

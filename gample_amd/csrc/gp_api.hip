@@ -124,6 +124,8 @@ int gp_ctx_destroy(gp_ctx *c)
     if (c->h_drift) (void)hipHostFree(c->h_drift);
     if (c->drift_ev) (void)hipEventDestroy(c->drift_ev);
     if (c->boff) (void)hipFree(c->boff);
+    if (c->rng_s) (void)hipFree(c->rng_s);
+    if (c->rng_inc) (void)hipFree(c->rng_inc);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -307,6 +309,35 @@ int gp_load_pair(gp_ctx *c, int se, int si, const gp_gen *g, uint64_t seed, int6
     if (rc) return rc;
     c->sp[se].n_binned = std::min(c->sp[se].n_binned, c->sp[se].n);
     GP_HIP(hipStreamSynchronize(c->stream));
+    return GP_OK;
+}
+
+int gp_generate(gp_ctx *c, int se, int si, const gp_gen *g, int64_t addn, int deposit)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, se);
+    if (si >= 0) GP_CHECK_SID(c, si);
+    if (!g || addn < 0) return gp_fail(GP_ERR_ARG, "bad generation arguments");
+    if (g->gen_type < 0 || g->gen_type > 7) return gp_fail(GP_ERR_ARG, "unknown generation type");
+    // a zero-width region is a line source (the hollow cathode's GeneratePosX)
+    if (!(g->xmax >= g->xmin) || !(g->ymax >= g->ymin)) return gp_fail(GP_ERR_ARG, "inverted generation region");
+    if (g->gen_type == 4 && !(g->xmax > g->xmin && g->ymax > g->ymin))
+        return gp_fail(GP_ERR_ARG, "quiet start needs a region of positive area");
+    GP_TRY({
+        int rc = gp_launch_generate(c, se, si, g, addn, deposit);
+        if (rc) return rc;
+    })
+    GP_HIP(hipStreamSynchronize(c->stream));
+    return GP_OK;
+}
+
+int gp_set_cathode(gp_ctx *c, double x_cm)
+{
+    GP_CHECK_CTX(c);
+    if (!(x_cm == x_cm)) return gp_fail(GP_ERR_ARG, "hollow-cathode position is NaN");
+    if (x_cm > 0.0 && c->cfg.fbc[0] == GP_FBC_PERIODIC)
+        return gp_fail(GP_ERR_ARG, "the hollow-cathode ramp needs non-periodic x walls");
+    c->cathode_pos = x_cm > 0.0 ? x_cm : -1.0;
     return GP_OK;
 }
 

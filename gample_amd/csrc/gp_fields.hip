@@ -147,6 +147,29 @@ __global__ void k_gradient(const double *phi, float *Ex, float *Ey, int nx, int 
     if (i <= nx && j <= ny + 1)
         Ey[i + j * (nx + 1)] = (float)(-(phi[i + 1 + (j + 1) * P] - phi[i + 1 + j * P]) / dy);
 }
+// hollow-cathode ramp (EMField.mm:1063-1077): the mean of the solution over
+// the nky+1 rows of unknown column col, then phi -= mean ((i - ngb - 2) dx) / pos
+// with (i - ngb - 2) dx a float product, as the reference's float dx makes it
+__global__ __launch_bounds__(256) void k_cathode_mean(const double *sol, int col, int K, int rows, double *out)
+{
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (int j = threadIdx.x; j < rows; j += 256) m += sol[col + (int64_t)j * K];
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0] / (double)rows;
+}
+__global__ void k_cathode_ramp(double *phi, int nx, int ny, int ngb, float dx, double pos, const double *mean)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > nx + 2 || j > ny + 2) return;
+    const float xi = (float)(i - ngb - 2) * dx;
+    phi[i + (int64_t)j * (nx + 3)] -= mean[0] * (double)xi / pos;
+}
 // energy partials: 0 sum E^2 (whole), 1 sum E.J, 2 CGS sum over unique points
 struct EnArgs {
     int ngb, ngx, ngy, nx, ny, perx, pery;
@@ -314,7 +337,7 @@ int gp_fields_init(gp_ctx *c)
     rc = fill(c->Bz, nBz, c->cfg.b0[2]); if (rc) return rc;
     c->bz0 = (float)c->cfg.b0[2];
     c->uniform_b = 1;
-    GP_HIP(hipMalloc((void **)&c->red, 3 * 1024 * sizeof(double)));
+    GP_HIP(hipMalloc((void **)&c->red, (3 * 1024 + 8) * sizeof(double)));   // + the cathode mean
     GP_HIP(hipHostMalloc((void **)&c->hred, 8 * sizeof(double), hipHostMallocDefault));
 
     // Poisson unknown grid (EMField.mm:247-271)
@@ -497,12 +520,24 @@ int gp_solve_fields_impl(gp_ctx *c, double dt, double tol, int maxiter, int solv
     ra.ngb = c->ngb; ra.ngx = c->ngx; ra.ngy = c->ngy; ra.nx = c->nx; ra.ny = c->ny; ra.nkx = s.nkx; ra.nky = s.nky;
     for (int w = 0; w < 4; w++) { ra.b[w] = c->cfg.fbc[w]; ra.v[w] = c->cfg.fbc_value[w]; }
     k_reconstruct<<<g2(c->nx + 3, c->ny + 3), b2, 0, sm>>>(s.x64, c->phi, ra);
+    if (c->cathode_pos > 0.0) {
+        const int col = (int)(c->cathode_pos / (double)c->dxf);
+        if (col < 0 || col > s.nkx) return gp_fail(GP_ERR_ARG, "hollow-cathode position outside the potential unknowns");
+        double *mean = c->red + 3 * 1024;
+        k_cathode_mean<<<1, 256, 0, sm>>>(s.x64, col, s.nkx + 1, s.nky + 1, mean);
+        k_cathode_ramp<<<g2(c->nx + 3, c->ny + 3), b2, 0, sm>>>(c->phi, c->nx, c->ny, c->ngb, c->dxf, c->cathode_pos, mean);
+        GP_HIP(hipGetLastError());
+        GP_HIP(hipMemcpyAsync(c->hred + 4, mean, sizeof(double), hipMemcpyDeviceToHost, sm));
+        GP_HIP(hipStreamSynchronize(sm));
+        c->cathode_mean = c->hred[4];
+    }
     k_gradient<<<g2(c->nx + 2, c->ny + 2), b2, 0, sm>>>(c->phi, c->Ex, c->Ey, c->nx, c->ny, (double)c->dxf, (double)c->dyf);
     GP_HIP(hipGetLastError());
     c->have_solution = 1;
     if (diag) {
         rc = gp_energy_impl(c, dt, &diag->we_ref, &diag->we_cgs, &diag->joule);
         if (rc) return rc;
+        diag->mean_cathode = c->cathode_pos > 0.0 ? c->cathode_mean : 0.0;
     }
     return status;
 }

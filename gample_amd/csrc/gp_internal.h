@@ -144,6 +144,23 @@ struct gp_ctx {
     unsigned long long *h_drift = nullptr;   // pinned, two per species: drifted, moved
     hipEvent_t drift_ev = nullptr;
     int drift_pending = 0;
+    // generation streams (Collision.mm:189-336, 464-678): thread id of every
+    // generation call draws from one PCG32 stream, seeded by splitmix64
+    // outputs 2 id + 1 and 2 id + 2 of the master seed and continued from
+    // call to call.  States are materialised on the device only for the ids
+    // that gp_generate has used (rng_n); a seeded load records its id range
+    // and draw count, and the states of ids it covered but that are not yet
+    // materialised are recomputed from these records when first needed.
+    uint64_t *rng_s = nullptr, *rng_inc = nullptr;
+    int64_t rng_cap = 0, rng_n = 0;
+    uint64_t rng_seed = 0;
+    int rng_seeded = 0;
+    struct rng_load { int64_t first, n; int draws; };
+    std::vector<rng_load> rng_loads;
+    // hollow-cathode potential ramp (EMField.mm:1063-1077): x position in cm
+    // (<= 0: off) and the last solve's mean potential on that column (statV)
+    double cathode_pos = -1.0;
+    double cathode_mean = 0.0;
 };
 
 // ---- particle side (gp_particles.hip) ----
@@ -152,6 +169,7 @@ int gp_launch_deposit(gp_ctx *c, int sid, int64_t first, int64_t count);
 int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke);
 int gp_launch_sort(gp_ctx *c, int sid);
 int gp_launch_load(gp_ctx *c, int se, int si, const gp_gen *g, uint64_t seed, int64_t first_id, int64_t n);
+int gp_launch_generate(gp_ctx *c, int se, int si, const gp_gen *g, int64_t addn, int deposit);
 int gp_launch_inject_wall(gp_ctx *c, int sid, int64_t n, float vth, const float u[3], double dt,
                           uint64_t seed, uint64_t stream, int deposit);
 int gp_launch_moments(gp_ctx *c, int sid, double *host10);

@@ -859,6 +859,9 @@ __device__ __forceinline__ Pcg seed_stream(uint64_t master, int64_t id)
     return r;
 }
 
+// seeded loads recorded per context whose draws a stream state recomputed
+// from its seed must replay
+constexpr int GP_RNG_RECS = 8;
 struct LoadArgs {
     gp_gen g;
     float dx, dy, ngb, xdom, ydom;
@@ -867,7 +870,51 @@ struct LoadArgs {
     gp_soa e, i;
     int64_t ne, ni;
     int with_ion;
+    // materialised stream states of ids < rng_n (read, then written back)
+    uint64_t *rs, *ri;
+    int64_t rng_n;
+    // earlier loads on the same seed: ids in [rec_first, rec_first + rec_n)
+    // have already drawn rec_draws numbers
+    int nrec;
+    int64_t rec_first[GP_RNG_RECS], rec_n[GP_RNG_RECS];
+    int rec_draws[GP_RNG_RECS];
 };
+
+// numbers one generation call draws per thread (Collision.mm:63-174):
+// position 2 (3 for the Gaussian profiles), electron velocity 4, ion
+// velocity 4 when the call also writes the ion
+int gen_draws(const gp_gen *g, bool with_ion)
+{
+    const int t = g->gen_type;
+    return 2 + ((t == 5 || t == 6) ? 1 : 0) + 4 + ((with_ion && t != 3) ? 4 : 0);
+}
+
+// the stream of thread id: its materialised state, or the state seeded from
+// the master seed and advanced past the draws of the recorded loads
+__device__ __forceinline__ Pcg stream_of(const LoadArgs &a, int64_t id)
+{
+    if (id < a.rng_n) {
+        Pcg r;
+        r.s = a.rs[id];
+        r.inc = a.ri[id];
+        return r;
+    }
+    Pcg r = seed_stream(a.seed, id);
+    for (int q = 0; q < a.nrec; q++)
+        if (id >= a.rec_first[q] && id < a.rec_first[q] + a.rec_n[q])
+            for (int d = 0; d < a.rec_draws[q]; d++) (void)r.next();
+    return r;
+}
+
+// materialise the stream states of ids [lo, hi)
+__global__ void k_rng_init(LoadArgs a, int64_t lo, int64_t hi, uint64_t *rs, uint64_t *ri)
+{
+    const int64_t id = lo + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (id >= hi) return;
+    const Pcg r = stream_of(a, id);
+    rs[id] = r.s;
+    ri[id] = r.inc;
+}
 
 __global__ void k_load(LoadArgs a)
 {
@@ -875,7 +922,7 @@ __global__ void k_load(LoadArgs a)
     if (k >= a.n) return;
     const float PIf = 3.14159265358979f;
     const int64_t id = a.first_id + k;
-    Pcg r = seed_stream(a.seed, id);
+    Pcg r = stream_of(a, id);
     const gp_gen &g = a.g;
     const float rx = r.u01(), ry = r.u01();
     float x, y;
@@ -953,6 +1000,7 @@ __global__ void k_load(LoadArgs a)
         a.i.vz[ji] = g.ion_u[2] + g.ion_vth * sqrtf(-logf(rv1)) * cosf(2.0f * PIf * rv2);
         a.i.x[ji] = x; a.i.y[ji] = y;
     }
+    if (id < a.rng_n) a.rs[id] = r.s;   // the stream continues in the next call
 }
 
 // half-Maxwellian flux through the x-min wall: vx Rayleigh (flux-weighted),
@@ -1292,11 +1340,11 @@ int gp_launch_sort(gp_ctx *c, int sid)
     return GP_OK;
 }
 
-int gp_launch_load(gp_ctx *c, int se, int si, const gp_gen *g, uint64_t seed, int64_t first_id, int64_t n)
+static void load_args(gp_ctx *c, int se, int si, const gp_gen *g, uint64_t seed, int64_t first_id, int64_t n,
+                      LoadArgs &a)
 {
-    if (n <= 0) return GP_OK;
     gp_species &E = c->sp[se];
-    LoadArgs a{};
+    a = LoadArgs{};
     a.g = *g;
     a.dx = c->dxf; a.dy = c->dyf; a.ngb = (float)c->ngb;
     a.xdom = (float)c->ngx; a.ydom = (float)c->ngy;
@@ -1304,12 +1352,95 @@ int gp_launch_load(gp_ctx *c, int se, int si, const gp_gen *g, uint64_t seed, in
     a.e = E.a; a.ne = E.n;
     a.with_ion = si >= 0;
     if (si >= 0) { a.i = c->sp[si].a; a.ni = c->sp[si].n; }
+}
+
+// the context's generation streams for LoadArgs a: its materialised states
+// and the recorded loads on its seed
+static int bind_streams(gp_ctx *c, LoadArgs &a)
+{
+    a.rs = c->rng_s; a.ri = c->rng_inc; a.rng_n = c->rng_n;
+    if ((int)c->rng_loads.size() > GP_RNG_RECS)
+        return gp_fail(GP_ERR_STATE, "more than 8 seeded loads on one context's generation streams");
+    a.nrec = (int)c->rng_loads.size();
+    for (int q = 0; q < a.nrec; q++) {
+        a.rec_first[q] = c->rng_loads[q].first;
+        a.rec_n[q] = c->rng_loads[q].n;
+        a.rec_draws[q] = c->rng_loads[q].draws;
+    }
+    return GP_OK;
+}
+
+int gp_launch_load(gp_ctx *c, int se, int si, const gp_gen *g, uint64_t seed, int64_t first_id, int64_t n)
+{
+    if (n <= 0) return GP_OK;
+    gp_species &E = c->sp[se];
     if (E.n + n > E.cap || (si >= 0 && c->sp[si].n + n > c->sp[si].cap))
         return gp_fail(GP_ERR_CAPACITY, "pNum > pNumMax in particle generation");
+    LoadArgs a;
+    load_args(c, se, si, g, seed, first_id, n, a);
+    // the first seeded load fixes the context's generation streams; a load on
+    // another seed draws from independent streams and is not recorded
+    if (!c->rng_seeded) { c->rng_seed = seed; c->rng_seeded = 1; }
+    const bool ours = seed == c->rng_seed;
+    if (ours) {
+        int rc = bind_streams(c, a);
+        if (rc) return rc;
+    }
     k_load<<<gp_cdiv(n, GP_BLOCK), GP_BLOCK, 0, c->stream>>>(a);
     GP_HIP(hipGetLastError());
+    if (ours) c->rng_loads.push_back({first_id, n, gen_draws(g, si >= 0)});
     E.n += n;
     if (si >= 0 && g->gen_type != 3) c->sp[si].n += n;
+    return GP_OK;
+}
+
+int gp_launch_generate(gp_ctx *c, int se, int si, const gp_gen *g, int64_t addn, int deposit)
+{
+    if (addn <= 0) return GP_OK;
+    gp_species &E = c->sp[se];
+    const bool ion = si >= 0 && g->gen_type != 3;
+    if (E.n + addn > E.cap || (ion && c->sp[si].n + addn > c->sp[si].cap))
+        return gp_fail(GP_ERR_CAPACITY, "pNum + addn > pNumMax in particle generation");
+    hipStream_t s = c->stream;
+    if (!c->rng_seeded) { c->rng_seed = 0; c->rng_seeded = 1; }   // the reference's master seed
+    LoadArgs a;
+    load_args(c, se, ion ? si : -1, g, c->rng_seed, 0, addn, a);
+    int rc = bind_streams(c, a);
+    if (rc) return rc;
+    if (addn > c->rng_n) {
+        // materialise the states of ids [rng_n, addn)
+        if (addn > c->rng_cap) {
+            const int64_t cap = std::max(addn, 2 * c->rng_cap);
+            uint64_t *ns = nullptr, *ni = nullptr;
+            GP_HIP(hipMalloc((void **)&ns, sizeof(uint64_t) * cap));
+            GP_HIP(hipMalloc((void **)&ni, sizeof(uint64_t) * cap));
+            if (c->rng_n > 0) {
+                GP_HIP(hipMemcpyAsync(ns, c->rng_s, sizeof(uint64_t) * c->rng_n, hipMemcpyDeviceToDevice, s));
+                GP_HIP(hipMemcpyAsync(ni, c->rng_inc, sizeof(uint64_t) * c->rng_n, hipMemcpyDeviceToDevice, s));
+            }
+            GP_HIP(hipStreamSynchronize(s));
+            if (c->rng_s) GP_HIP(hipFree(c->rng_s));
+            if (c->rng_inc) GP_HIP(hipFree(c->rng_inc));
+            c->rng_s = ns; c->rng_inc = ni; c->rng_cap = cap;
+        }
+        const int64_t lo = c->rng_n;
+        a.rs = c->rng_s; a.ri = c->rng_inc; a.rng_n = lo;
+        k_rng_init<<<gp_cdiv(addn - lo, GP_BLOCK), GP_BLOCK, 0, s>>>(a, lo, addn, c->rng_s, c->rng_inc);
+        GP_HIP(hipGetLastError());
+        c->rng_n = addn;
+    }
+    a.rs = c->rng_s; a.ri = c->rng_inc; a.rng_n = c->rng_n;
+    a.nrec = 0;   // every id of the call is materialised
+    k_load<<<gp_cdiv(addn, GP_BLOCK), GP_BLOCK, 0, s>>>(a);
+    GP_HIP(hipGetLastError());
+    const int64_t e0 = E.n, i0 = ion ? c->sp[si].n : 0;
+    E.n += addn;
+    if (ion) c->sp[si].n += addn;
+    if (deposit) {
+        rc = gp_launch_deposit(c, se, e0, addn);
+        if (rc) return rc;
+        if (ion && (rc = gp_launch_deposit(c, si, i0, addn))) return rc;
+    }
     return GP_OK;
 }
 

@@ -106,6 +106,9 @@ typedef struct {
     double joule;             /* sum(E.J) dt dx dy */
     int64_t count[8];         /* live particles per species */
     int64_t absorbed[8][4];   /* particles absorbed per species per wall this step */
+    double mean_cathode;      /* statV: mean potential on the hollow-cathode column
+                                 (EMField.mm:1063-1071, logged as meanCathode :1122);
+                                 0 without a cathode */
 } gp_diag;
 
 /* field identifiers for gp_field_get / gp_field_set */
@@ -170,6 +173,24 @@ int  gp_species_grid_get(gp_ctx *ctx, int sid, int which, int64_t *host, int64_t
  * particle-id order starting at first_id.  sid_ion < 0: electrons only. */
 int  gp_load_pair(gp_ctx *ctx, int sid_ele, int sid_ion, const gp_gen *g, uint64_t master_seed,
                   int64_t first_id, int64_t n);
+/* -[Collision artificialIonization:] / -[Collision hollowCathode:]
+ * (Collision.mm:464-566, 568-678; kernel :63-174): addn particles appended
+ * to sid_ele (and, unless sid_ion < 0 or gen_type is 3, addn co-located ions
+ * to sid_ion) with the region, drift and thermal speeds of g.  Thread id of
+ * the call draws from the context's generation stream id: the PCG32 stream
+ * of the master seed of the first seeded gp_load_pair (0 without one),
+ * continued from the draws the loads and earlier gp_generate calls made on
+ * it, as the reference's persistent RNGState array (Collision.mm:300-313,
+ * 507-520).  A region of zero width in x or y is a line source.  addn is the
+ * caller's (the reference rounds src w^-1 dt stochastically, and the
+ * hollow cathode takes the electron current lost at its wall).  deposit != 0
+ * also deposits the new particles into the accumulators. */
+int  gp_generate(gp_ctx *ctx, int sid_ele, int sid_ion, const gp_gen *g, int64_t addn, int deposit);
+/* the hollow-cathode boundary entry of BoundaryConditionForField
+ * (EMField.mm:231-232, 1063-1077): after each Poisson solve the mean of the
+ * potential unknowns on column (int)(x_cm / dx) is removed along the ramp
+ * phi -= mean ((i - ngb - 2) dx) / x_cm; x_cm <= 0 turns it off. */
+int  gp_set_cathode(gp_ctx *ctx, double x_cm);
 /* Half-Maxwellian flux injection through the x-min wall (no reference
  * counterpart; the reference's nearest analogue is the hollow-cathode line
  * source, Collision.mm:568-678): n particles with vx from the flux-weighted

@@ -652,20 +652,15 @@ static float load_pos_x(const gpo_gen *p, int64_t id, int64_t ntot, float rx, fl
     return x;
 }
 
-/* src/Collision.mm:63-174 with the seeding of src/Collision.mm:280-294:
- * particle id takes splitmix64 outputs 2*id+1 (state) and 2*id+2 (inc|1)
- * of the master sequence. */
-int gpo_load_pair(const gpo_gen *p, uint64_t master_seed, int64_t first_id, int64_t n,
-                  float *ex, float *ey, float *evx, float *evy, float *evz,
-                  float *ix, float *iy, float *ivx, float *ivy, float *ivz)
+/* one thread of the generation kernel src/Collision.mm:63-174 on stream
+ * (st, inc): electron k, and ion k when ix is given and gen_type is not 3
+ * (the hollow cathode, :136-139) */
+static void gen_one(const gpo_gen *p, int64_t id, int64_t ntot, uint64_t *stp, uint64_t inc, int64_t k,
+                    float *ex, float *ey, float *evx, float *evy, float *evz,
+                    float *ix, float *iy, float *ivx, float *ivy, float *ivz)
 {
-    const int64_t ntot = first_id + n;
-#pragma omp parallel for schedule(static)
-    for (int64_t k = 0; k < n; k++) {
-        int64_t id = first_id + k;
-        uint64_t sd = master_seed + (uint64_t)(2 * id) * 0x9E3779B97F4A7C15ULL;
-        uint64_t st = gpo_splitmix64(&sd);
-        uint64_t inc = gpo_splitmix64(&sd) | 1ULL;
+    uint64_t st = *stp;
+    {
         float rx = gpo_rand01(gpo_pcg32(&st, inc));
         float ry = gpo_rand01(gpo_pcg32(&st, inc));
         float y;
@@ -695,6 +690,47 @@ int gpo_load_pair(const gpo_gen *p, uint64_t master_seed, int64_t first_id, int6
             ix[k] = x; iy[k] = y;
         }
     }
+    *stp = st;
+}
+
+/* src/Collision.mm:280-294 (and :507-520 for later ids): the states are
+ * drawn from one splitmix64 sequence in id order, state then inc|1, so id
+ * takes outputs 2*id+1 and 2*id+2 of the master sequence. */
+void gpo_stream_states(uint64_t master_seed, int64_t first_id, int64_t n, uint64_t *state, uint64_t *inc)
+{
+    for (int64_t k = 0; k < n; k++) {
+        uint64_t sd = master_seed + (uint64_t)(2 * (first_id + k)) * 0x9E3779B97F4A7C15ULL;
+        state[k] = gpo_splitmix64(&sd);
+        inc[k] = gpo_splitmix64(&sd) | 1ULL;
+    }
+}
+
+/* src/Collision.mm:63-174 with the seeding of src/Collision.mm:280-294:
+ * particle id takes splitmix64 outputs 2*id+1 (state) and 2*id+2 (inc|1)
+ * of the master sequence. */
+int gpo_load_pair(const gpo_gen *p, uint64_t master_seed, int64_t first_id, int64_t n,
+                  float *ex, float *ey, float *evx, float *evy, float *evz,
+                  float *ix, float *iy, float *ivx, float *ivy, float *ivz)
+{
+    const int64_t ntot = first_id + n;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; k++) {
+        uint64_t st, inc;
+        gpo_stream_states(master_seed, first_id + k, 1, &st, &inc);
+        gen_one(p, first_id + k, ntot, &st, inc, k, ex, ey, evx, evy, evz, ix, iy, ivx, ivy, ivz);
+    }
+    return 0;
+}
+
+/* -[Collision artificialIonization:] / -[Collision hollowCathode:] with
+ * the reference's persistent RNGState array (src/Collision.mm:464-678): thread
+ * id < addn reads state[id], draws, and writes it back (:136-139, :171-172). */
+int gpo_generate(const gpo_gen *p, uint64_t *state, const uint64_t *inc, int64_t addn,
+                 float *ex, float *ey, float *evx, float *evy, float *evz,
+                 float *ix, float *iy, float *ivx, float *ivy, float *ivz)
+{
+    for (int64_t id = 0; id < addn; id++)
+        gen_one(p, id, addn, &state[id], inc[id], id, ex, ey, evx, evy, evz, ix, iy, ivx, ivy, ivz);
     return 0;
 }
 
@@ -933,6 +969,24 @@ void gpo_reconstruct_phi(const gpo_field *f, const double *sol, double *phi)
             phi[i + (int64_t)j * (nx + 3)] = v;
         }
     }
+}
+
+/* src/EMField.mm:1063-1077: the mean of the unknowns on column
+ * (int)(pos/dx) over its nky+1 rows, summed in row order, then
+ * phi -= mean*((i-ngb-2)*dx)/pos over the whole (nx+3)(ny+3) array, with dx
+ * the float member and (i-ngb-2)*dx a float product. */
+double gpo_cathode_ramp(const gpo_field *f, const double *sol, double pos, double *phi)
+{
+    int nkx, nky, ikmin, jkmin;
+    gpo_poisson_dims(f, &nkx, &nky, &ikmin, &jkmin);
+    const int nx = f->ngx + 2 * f->ngb, ny = f->ngy + 2 * f->ngb;
+    double mean = 0.0;
+    for (int j = 0; j <= nky; j++) mean += sol[(int)(pos / f->dx) + j * (nkx + 1)];
+    mean /= nky + 1;
+    for (int j = 0; j <= ny + 2; j++)
+        for (int i = 0; i <= nx + 2; i++)
+            phi[i + j * (nx + 3)] -= mean * ((i - f->ngb - 2) * f->dx) / pos;
+    return mean;
 }
 
 /* src/EMField.mm:1080-1094 */

@@ -141,6 +141,10 @@ void gpo_inject_wall(uint64_t seed, uint64_t stream, int64_t n, float vth, const
 int gpo_load_pair(const gpo_gen *p, uint64_t master_seed, int64_t first_id, int64_t n,
                   float *ex, float *ey, float *evx, float *evy, float *evz,
                   float *ix, float *iy, float *ivx, float *ivy, float *ivz);
+void gpo_stream_states(uint64_t master_seed, int64_t first_id, int64_t n, uint64_t *state, uint64_t *inc);
+int gpo_generate(const gpo_gen *p, uint64_t *state, const uint64_t *inc, int64_t addn,
+                 float *ex, float *ey, float *evx, float *evy, float *evz,
+                 float *ix, float *iy, float *ivx, float *ivy, float *ivz);
 
 /* ---------------- field side ---------------- */
 typedef struct {
@@ -168,6 +172,7 @@ double gpo_relres(int64_t n, const int32_t *ptr, const int32_t *col, const doubl
 /* phi (nx+3)*(ny+3) from the solution (corrected periodic-x mapping) */
 void gpo_reconstruct_phi(const gpo_field *f, const double *sol, double *phi);
 /* Ex (nx+2)*(ny+1), Ey (nx+1)*(ny+2) = -grad phi, stored FP32 */
+double gpo_cathode_ramp(const gpo_field *f, const double *sol, double pos, double *phi);
 void gpo_gradient(const gpo_field *f, const double *phi, float *Ex, float *Ey);
 /* energies: reference form sum(E^2)*dx*dy/2 over whole arrays, CGS form
  * sum(E^2)*dx*dy/(8 pi) over unique interior points, Joule sum(E.J)*dt*dx*dy */

@@ -73,6 +73,10 @@ def lib():
         L.gpo_rand01.argtypes = [C.c_uint32]
         L.gpo_rand01.restype = C.c_float
         L.gpo_load_pair.argtypes = [P, C.c_uint64, C.c_int64, C.c_int64] + [P] * 10
+        L.gpo_stream_states.argtypes = [C.c_uint64, C.c_int64, C.c_int64, P, P]
+        L.gpo_generate.argtypes = [P, P, P, C.c_int64] + [P] * 10
+        L.gpo_cathode_ramp.argtypes = [P, P, C.c_double, P]
+        L.gpo_cathode_ramp.restype = C.c_double
         L.gpo_inject_wall.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, C.c_float, P, C.c_float, C.c_float,
                                       C.c_float, C.c_float] + [P] * 5
         L.gpo_poisson_dims.argtypes = [P] + [C.POINTER(C.c_int)] * 4
@@ -201,6 +205,29 @@ def load_pair(gen: Gen, seed, first_id, n, ion=True):
     i = [np.zeros(n, np.float32) for _ in range(5)] if ion else [None] * 5
     lib().gpo_load_pair(C.byref(gen), seed, first_id, n, *[ptr(a) for a in e], *[ptr(a) for a in i])
     return e, (i if ion else None)
+
+
+def stream_states(seed, first_id, n):
+    """(state, inc) of generation threads first_id .. first_id + n - 1"""
+    st = np.zeros(n, np.uint64)
+    inc = np.zeros(n, np.uint64)
+    lib().gpo_stream_states(seed, first_id, n, ptr(st), ptr(inc))
+    return st, inc
+
+
+def generate(gen: Gen, state, inc, addn, ion=True):
+    """one generation call on the persistent states (advanced in place)"""
+    assert state.dtype == np.uint64 and len(state) >= addn and len(inc) >= addn
+    e = [np.zeros(addn, np.float32) for _ in range(5)]
+    i = [np.zeros(addn, np.float32) for _ in range(5)] if ion else [None] * 5
+    lib().gpo_generate(C.byref(gen), ptr(state), ptr(inc), addn, *[ptr(a) for a in e], *[ptr(a) for a in i])
+    return e, (i if ion else None)
+
+
+def cathode_ramp(f, sol, pos, phi):
+    """apply the hollow-cathode ramp to phi in place; returns the mean"""
+    assert phi.dtype == np.float64 and phi.flags.c_contiguous
+    return lib().gpo_cathode_ramp(C.byref(f), ptr(np.ascontiguousarray(sol, np.float64)), pos, ptr(phi))
 
 
 def inject_wall(seed, stream, n, vth, u, dtdx, lo, ngx, ngy):

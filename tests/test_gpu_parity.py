@@ -542,6 +542,107 @@ def test_wall_injection(gp, orc):
     ctx.close()
 
 
+
+def _gen_pair(gp, orc, gt, region, ngx, ngy, dx, ele=(1.9e8, (0.0, 0.0, 0.0)), ion=(9e4, (0.0, 0.0, 0.0))):
+    G, O = gp.Gen(), orc.Gen()
+    for q in (G, O):
+        q.gen_type = gt
+        q.xmin, q.xmax, q.ymin, q.ymax = region
+        q.ele_vth, q.ion_vth = ele[0], ion[0]
+        q.ele_u[:] = list(ele[1])
+        q.ion_u[:] = list(ion[1])
+    O.dx = O.dy = dx
+    O.ngb = 1.0
+    O.xdom, O.ydom = ngx, ngy
+    return G, O
+
+
+def test_generation_streams(gp, orc):
+    """gp_generate (artificial ionization and hollow cathode, Collision.mm:
+    464-678) against the oracle's persistent state array: a seeded load of
+    5000 pairs, an ionization call of 7000 x-sinusoidal pairs in a
+    sub-region (ids below 5000 continue their streams after the load, the
+    rest start fresh), a hollow-cathode line source of 3000 electrons (every
+    id continues), and a deposited call whose int64 charge equals the
+    oracle deposit of the new particles.  y is bit-exact (uniform draws), x
+    bit-exact for the uniform and line sources and within 2e-5 cells for asinf,
+    velocities within 2e-5 relative (device vs libm transcendentals)."""
+    say("[generate] load 5000, ionize 7000, cathode 3000, ionize 2000 deposited")
+    ngx, ngy, dx = 64, 48, 0.01
+    ctx = mkctx(gp, ngx, ngy, 3, pbc=("Delete", "Delete", "periodic", "periodic"),
+                fbc=("Dirichlet", "Dirichlet", "periodic", "periodic"), dx=dx)
+    se = ctx.add_species(-EC, ME, 1.0, 20000)
+    si = ctx.add_species(EC, 2.18e-22, 1.0, 20000)
+    full = (0.0, ngx * dx, 0.0, ngy * dx)
+    Gl, Ol = _gen_pair(gp, orc, 0, full, ngx, ngy, dx)
+    Ga, Oa = _gen_pair(gp, orc, 1, (0.1, 0.5, 0.0, ngy * dx), ngx, ngy, dx, ion=(9e4, (1e5, 0.0, 0.0)))
+    Gh, Oh = _gen_pair(gp, orc, 3, (0.3, 0.3, 0.0, ngy * dx), ngx, ngy, dx, ele=(1.5e8, (0.0, 1e6, 0.0)),
+                       ion=(0.0, (0.0, 0.0, 0.0)))
+    ctx.load_pair(se, si, Gl, 0, 0, 5000)
+    ctx.generate(se, si, Ga, 7000, deposit=0)
+    ctx.generate(se, -1, Gh, 3000, deposit=0)
+    e, i = ctx.download(se), ctx.download(si)
+    assert len(e[0]) == 15000 and len(i[0]) == 12000
+    st, inc = orc.stream_states(0, 0, 7000)
+    oe0, oi0 = orc.generate(Ol, st, inc, 5000)
+    oe1, oi1 = orc.generate(Oa, st, inc, 7000)
+    oe2, _ = orc.generate(Oh, st, inc, 3000, ion=False)
+    cuts = [(0, 5000, oe0, True), (5000, 12000, oe1, False), (12000, 15000, oe2, True)]
+    for lo, hi, oe, exact_x in cuts:
+        np.testing.assert_array_equal(e[1][lo:hi], oe[1])
+        if exact_x:
+            np.testing.assert_array_equal(e[0][lo:hi], oe[0])
+        else:
+            np.testing.assert_allclose(e[0][lo:hi], oe[0], rtol=0, atol=2e-5)
+        for a, b in zip(e[2:], oe[2:]):
+            np.testing.assert_allclose(a[lo:hi], b, rtol=2e-5, atol=1e-6 * np.abs(b).max())
+    for lo, hi, oi in ((0, 5000, oi0), (5000, 12000, oi1)):
+        np.testing.assert_array_equal(i[1][lo:hi], oi[1])
+        np.testing.assert_array_equal(i[0][lo:hi], e[0][lo:hi])          # co-located (Collision.mm:150-151)
+        for a, b in zip(i[2:], oi[2:]):
+            np.testing.assert_allclose(a[lo:hi], b, rtol=2e-5, atol=1e-6 * np.abs(b).max())
+    assert np.all(e[0][12000:] == np.float32(np.float32(0.3) / np.float32(dx)) + np.float32(1.0))
+    ctx.reset_grids()
+    ctx.generate(se, si, Ga, 2000, deposit=1)
+    e, i = ctx.download(se), ctx.download(si)
+    g = orc.grid(ngx, ngy, 3, pbc=("Delete", "Delete", "periodic", "periodic"))
+    for sid, arr in ((se, e), (si, i)):
+        rho, _, _ = orc.deposit_fx(g, [a[-2000:] for a in arr])
+        np.testing.assert_array_equal(ctx.acc(sid), rho)
+    ctx.close()
+
+
+def test_cathode_ramp(gp, orc):
+    """gp_set_cathode (EMField.mm:231-232, 1063-1077): the same charge solved
+    without and then with the hollow-cathode entry (the ramp acts after the
+    solve, so the unknowns agree to the solve tolerance); phi equals the
+    oracle's ramp applied to the oracle reconstruction of the device's
+    unknowns to FP64 summation order, E to a few float ulps, and the
+    diagnostics report the mean."""
+    say("[cathode] 64x48 Dirichlet x, cathode at 0.1234 cm")
+    fbc = ("Dirichlet", "Dirichlet", "periodic", "periodic")
+    ctx, st, acc_e, acc_i, dx = _poisson_case(gp, orc, 3, fbc, 64, 48)
+    sol0 = ctx.field(gp.F_SOL).ravel()
+    pos = 0.1234
+    ctx.set_cathode(pos)
+    ctx.reset_grids()
+    for sid in (0, 1):              # the particles are still in the store
+        ctx.deposit(sid)
+    st2, dg, rc = ctx.solve_fields(0.0, tol=1e-8, maxiter=400, diag=True)
+    assert st2.relres <= 1e-8
+    sol = ctx.field(gp.F_SOL).ravel()
+    assert np.linalg.norm(sol - sol0) <= 1e-6 * np.linalg.norm(sol0)
+    f = orc.field(64, 48, 1, dx, dx, fbc=fbc)
+    phi = orc.reconstruct(f, sol)
+    mean = orc.cathode_ramp(f, sol, pos, phi)
+    assert dg.mean_cathode == pytest.approx(mean, rel=1e-13)
+    np.testing.assert_allclose(ctx.field(gp.F_PHI), phi, rtol=0, atol=1e-12 * np.abs(phi).max())
+    Ex, Ey = orc.gradient(f, phi)
+    for got, exp in ((ctx.field(gp.F_EX), Ex), (ctx.field(gp.F_EY), Ey)):
+        np.testing.assert_allclose(got, exp, rtol=0, atol=1e-6 * np.abs(exp).max())
+    ctx.set_cathode(0.0)
+    ctx.close()
+
 @pytest.mark.parametrize("order", [1, 3, 5])
 def test_drifted_particles_reach_the_window_edges(gp, orc, order):
     """Particles that have drifted up to GP_DRIFT + 0.5 cells from their tile since the

@@ -104,3 +104,92 @@ def test_loader_keeps_positions_below_the_upper_face(orc):
     moved = raw[0] != fixed[0]
     assert np.all(raw[0][moved] == ngx) and np.all(fixed[0][moved] == np.nextafter(np.float32(ngx), 0))
     assert fixed[0].max() < ngx and fixed[1].max() < ngx
+
+
+M64 = (1 << 64) - 1
+
+
+def test_stream_states_follow_the_sequential_init(orc):
+    """gpo_stream_states equals the reference's id-order initialisation
+    (Collision.mm:300-313, 507-520): state then inc|1 drawn from one
+    splitmix64 sequence, here run step by step with the pinned splitmix64"""
+    for seed in (0, 987654321):
+        m = C.c_uint64(seed)
+        st_seq, inc_seq = [], []
+        for _ in range(300):
+            st_seq.append(orc.lib().gpo_splitmix64(C.byref(m)))
+            inc_seq.append(orc.lib().gpo_splitmix64(C.byref(m)) | 1)
+        st, inc = orc.stream_states(seed, 0, 300)
+        assert st.tolist() == st_seq and inc.tolist() == inc_seq
+        st, inc = orc.stream_states(seed, 137, 50)
+        assert st.tolist() == st_seq[137:187] and inc.tolist() == inc_seq[137:187]
+
+
+@pytest.mark.parametrize("gt,ion", [(0, True), (1, True), (2, True), (5, True), (6, False), (7, True), (3, False)])
+def test_generate_on_fresh_states_is_the_load(orc, gt, ion):
+    """a generation call on freshly seeded states draws exactly what the
+    initial load does (both are the kernel of Collision.mm:63-174)"""
+    g = _gen(orc, gt)
+    g.pert_delta, g.pert_mode = 0.02, 3
+    n = 3000
+    st, inc = orc.stream_states(0, 0, n)
+    e, i = orc.generate(g, st, inc, n, ion=ion)
+    le, li = orc.load_pair(g, 0, 0, n, ion=ion)
+    for a, b in zip(e, le):
+        np.testing.assert_array_equal(a, b)
+    if ion and gt != 3:
+        for a, b in zip(i, li):
+            np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("gt,ion,draws", [(0, True, 10), (1, True, 10), (5, True, 11), (6, False, 7),
+                                          (3, True, 6), (3, False, 6), (4, False, 6)])
+def test_generate_advances_each_stream_by_its_draws(orc, gt, ion, draws):
+    """the kernel writes each thread's state back after exactly the numbers
+    it drew (Collision.mm:136-139, 171-172): 2 for the position (3 for the
+    Gaussian profiles), 4 for the electron velocity, 4 for the ion's unless
+    the call is the hollow cathode's; stepping the PCG32 LCG that many times
+    from the initial state must land on the written-back state"""
+    g = _gen(orc, gt)
+    n = 257
+    st0, inc = orc.stream_states(11, 0, n)
+    st = st0.copy()
+    orc.generate(g, st, inc, n, ion=ion)
+    for k in range(n):
+        s = int(st0[k])
+        for _ in range(draws):
+            s = (s * 6364136223846793005 + int(inc[k])) & M64
+        assert s == int(st[k])
+
+
+def test_cathode_ramp_pins_the_cathode_column(orc):
+    """EMField.mm:1063-1077 on a random solution: the ramp subtracts a
+    linear function of x only, so Ey is unchanged and Ex moves by the
+    uniform mean/pos; it vanishes at phi index ngb+2; and on a Dirichlet-x
+    grid (ikmin = ngb + 1) the row mean of phi on the cathode column falls to
+    mean (1 - col dx / pos), col = int(pos / dx)"""
+    f = orc.Field()
+    f.ngx, f.ngy, f.ngb, f.dx, f.dy = 40, 24, 1, 0.005, 0.005
+    f.fbc[:] = [1, 1, 0, 0]
+    f.fval[:] = [30.0, 0.0, 0.0, 0.0]
+    nkx, nky, ikmin, _ = orc.poisson_dims(f)
+    assert ikmin == f.ngb + 1
+    rng = np.random.default_rng(4)
+    sol = rng.standard_normal((nky + 1) * (nkx + 1)) + 5.0
+    pos = 0.1234
+    col = int(pos / np.float32(0.005))
+    phi0 = orc.reconstruct(f, sol)
+    phi1 = phi0.copy()
+    mean = orc.cathode_ramp(f, sol, pos, phi1)
+    assert mean == pytest.approx(sol.reshape(nky + 1, nkx + 1)[:, col].mean(), rel=1e-14)
+    np.testing.assert_array_equal(phi1[:, f.ngb + 2], phi0[:, f.ngb + 2])
+    ex0, ey0 = orc.gradient(f, phi0)
+    ex1, ey1 = orc.gradient(f, phi1)
+    np.testing.assert_allclose(ey1, ey0, rtol=0, atol=1e-9 * np.abs(ey0).max())
+    np.testing.assert_allclose(ex1 - ex0, mean / pos, rtol=1e-5)
+    ci = col + ikmin + 1                                     # phi index of unknown column col
+    rows = slice(1 + f.ngb, 2 + f.ngb + nky)                 # the unknown rows (periodic y)
+    before = phi0[rows, ci].mean()
+    assert before == pytest.approx(mean, rel=1e-12)
+    xc = float(np.float32(col) * np.float32(0.005))          # the float product of the reference
+    assert phi1[rows, ci].mean() == pytest.approx(mean * (1 - xc / pos), rel=1e-9)
