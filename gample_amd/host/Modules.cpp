@@ -102,7 +102,7 @@ EMField::EMField(const Init &init, int device)
     }
     for (auto &b : init.fieldBoundaries) {
         int w = wall_index(b.position);
-        if (w < 0) continue;   // hollow-cathode position: out of scope
+        if (w < 0) continue;   // the hollow-cathode entry: below
         cfg.fbc[w] = fbc_code(b.type);
         if (cfg.fbc[w] == GP_FBC_DIRICHLET) cfg.fbc_value[w] = b.val * VtosV;   // EMField.mm:206
     }
@@ -122,6 +122,8 @@ EMField::EMField(const Init &init, int device)
     maxiter = init.paramForComputing.maxiter;
     solver = init.paramForComputing.solver;
     check(gp_set_sort_drift(ctx, init.paramForComputing.sortDrift), "SortDriftFraction");
+    for (auto &b : init.fieldBoundaries)   // EMField.mm:231-232
+        if (b.position == "hollow-cathode") check(gp_set_cathode(ctx, b.val), "hollow-cathode");
     if (solver != 0) check(gp_nn_load(ctx, init.paramForComputing.networkFile.c_str()), "NetworkFile");
     if (F.InitTypeB == "From1dXFile") {   // EMField.mm:175-197, T -> G
         auto bx = load1d(F.FilePathBx), by = load1d(F.FilePathBy), bz = load1d(F.FilePathBz);
@@ -248,8 +250,17 @@ static int gen_code(const std::string &t)
     throw std::runtime_error("unknown GenerateType '" + t + "'");
 }
 
-Collision::Collision(EMField &f, const Init &init, std::vector<Particle> &ptcl) : fld(f)
+static uint64_t splitmix64(uint64_t *x)   // Collision.h:11-16
 {
+    uint64_t z = (*x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+Collision::Collision(EMField &f, const Init &init, std::vector<Particle> &ptcl_) : fld(f), ptcl(ptcl_)
+{
+    std::vector<Particle> &ptcl = ptcl_;
     const auto &P = init.paramForParticle;
     const auto &F = init.paramForField;
     if (P.empty()) return;
@@ -280,6 +291,91 @@ Collision::Collision(EMField &f, const Init &init, std::vector<Particle> &ptcl) 
     // Collision.mm:280-336: master seed 0, ids 0..addn-1, ions co-located
     check(gp_load_pair(fld.ctx, ptcl[se].sid, si >= 0 ? ptcl[si].sid : -1, &g, 0ULL, 0, P[0].pNum), "initial load");
     for (auto &p : ptcl) check(gp_sort(fld.ctx, p.sid), "sort");
+
+    // sources (Collision.mm:366-458)
+    auto species = [&](const std::string &name) {
+        for (size_t s = 0; s < ptcl.size(); s++)
+            if (ptcl[s].pName == name) return (int)s;
+        return -1;
+    };
+    auto region = [&](const SourceForParticle &src, gp_gen &q) {
+        if (src.genX.size() < 2 || src.genX[0] < 0) { q.xmin = 0.0f; q.xmax = (float)(F.dx * F.ngx); }
+        else { q.xmin = (float)src.genX[0]; q.xmax = (float)src.genX[1]; }
+        if (src.genY.size() < 2 || src.genY[0] < 0) { q.ymin = 0.0f; q.ymax = (float)(F.dy * F.ngy); }
+        else { q.ymin = (float)src.genY[0]; q.ymax = (float)src.genY[1]; }
+    };
+    for (const auto &src : init.particleSources) {
+        int t;
+        if (src.genType == "uniformIonization") t = 0;
+        else if (src.genType == "XsinusoidalIonization") t = 1;
+        else if (src.genType == "YsinusoidalIonization") t = 2;
+        else if (src.genType == "hollow-cathode") t = 3;
+        else throw std::runtime_error("unknown source GenerateType '" + src.genType + "'");
+        if (t != 3) {
+            existAI = true;
+            std::memset(&ai, 0, sizeof ai);
+            ai.gen_type = t;
+            region(src, ai);
+            for (int k = 0; k < 3 && k < (int)src.genU_ele.size(); k++) ai.ele_u[k] = (float)src.genU_ele[k];
+            for (int k = 0; k < 3 && k < (int)src.genU.size(); k++) ai.ion_u[k] = (float)src.genU[k];
+            ai.ele_vth = (float)std::sqrt(2 * kb * src.genT_ele / me);
+            aiEle = species("electron");
+            aiIon = species(src.pName);
+            if (aiEle < 0 || aiIon < 0) throw std::runtime_error("ionization source needs species electron and " + src.pName);
+            ai.ion_vth = (float)std::sqrt(2 * kb * src.genT / init.paramForParticle[aiIon].m);
+            aiSrc = (float)src.src;
+        } else {
+            existHC = true;
+            std::memset(&hc, 0, sizeof hc);
+            hc.gen_type = 3;
+            region(src, hc);
+            for (int k = 0; k < 3 && k < (int)src.genU.size(); k++) hc.ele_u[k] = (float)src.genU[k];
+            hc.ele_vth = (float)std::sqrt(2 * kb * src.genT / me);
+            hcEle = species("electron");
+            if (hcEle < 0) throw std::runtime_error("hollow-cathode source needs species electron");
+            hcWall = (int)src.src;   // 0 Xmin, 1 Xmax, 2 Ymin, 3 Ymax
+        }
+    }
+    uint64_t sd = 0x5eedULL;
+    rndState = splitmix64(&sd);
+    rndInc = splitmix64(&sd) | 1ULL;
+}
+
+double Collision::uniform01()
+{
+    const uint64_t old = rndState;
+    rndState = old * 6364136223846793005ULL + rndInc;
+    const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u), rot = (uint32_t)(old >> 59u);
+    const uint32_t r = (xs >> rot) | (xs << ((-rot) & 31u));
+    return (double)r * (1.0 / 4294967296.0);
+}
+
+int Collision::artificialIonization(double dt, bool deposit)
+{
+    if (!existAI) return 0;
+    // float src over the float weight, then times dt (Collision.mm:482)
+    const double addn_d = (double)(aiSrc / ptcl[aiEle].w) * dt;
+    const int base = (int)addn_d;
+    const int addn = uniform01() < addn_d - base ? base + 1 : base;
+    check(gp_generate(fld.ctx, ptcl[aiEle].sid, ptcl[aiIon].sid, &ai, addn, deposit ? 1 : 0), "artificialIonization");
+    return addn;
+}
+
+int Collision::hollowCathode(bool deposit, int &neglected)
+{
+    neglected = 0;
+    if (!existHC) return 0;
+    int addn = 0;
+    for (auto &p : ptcl) {
+        const int pin[4] = {p.pinum_Xmin, p.pinum_Xmax, p.pinum_Ymin, p.pinum_Ymax};
+        addn -= pin[hcWall];
+    }
+    if (addn <= 0) {
+        neglected = -addn;
+        return 0;
+    }
+    check(gp_generate(fld.ctx, ptcl[hcEle].sid, -1, &hc, addn, deposit ? 1 : 0), "hollowCathode");
+    return addn;
 }
 
 }  // namespace gample

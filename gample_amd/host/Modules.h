@@ -77,9 +77,30 @@ public:
 
 class Collision {
 public:
-    // initial seeded loading, Collision.mm:189-336
+    // initial seeded loading, Collision.mm:189-336, and the set-up of the
+    // SourceForParticle entries (:366-458): the last artificial-ionization
+    // entry and the last hollow-cathode entry are the ones run
     Collision(EMField &fld, const Init &init, std::vector<Particle> &ptcl);
+    // Collision.mm:464-566: addn = src / w_e * dt pairs, rounded up with the
+    // probability of its fraction; returns addn.  deposit: deposit the new
+    // particles at once (the fused step's accumulators)
+    int artificialIonization(double dt, bool deposit);
+    // Collision.mm:568-678: addn = -(sum over species of pinum at the
+    // source's wall) electrons on the source line; a non-positive addn is
+    // neglected and reported in `neglected`; returns the injected count
+    int hollowCathode(bool deposit, int &neglected);
+    bool existAI = false, existHC = false;
     EMField &fld;
+
+private:
+    std::vector<Particle> &ptcl;
+    gp_gen ai{}, hc{};
+    int aiEle = -1, aiIon = -1, hcEle = -1, hcWall = 0;
+    float aiSrc = 0.0f;
+    // the reference draws the rounding from an unseeded std engine; here a
+    // PCG32 stream of a fixed seed makes runs repeatable
+    uint64_t rndState = 0, rndInc = 1;
+    double uniform01();
 };
 
 }  // namespace gample

@@ -85,9 +85,6 @@ int main(int argc, char **argv)
         tag = b;
     }
     const auto &T = init.paramForTimeIntegration;
-    if (!init.particleSources.empty())
-        std::fprintf(stderr, "[warn] SourceForParticle entries (artificial ionization / hollow cathode) are "
-                             "not run by this build\n");
     const bool fused = !args.count("--unfused");
     try {
         Logger logger(T.ProjectName + "_" + tag, true);
@@ -96,7 +93,8 @@ int main(int argc, char **argv)
         ptcl.reserve(init.flagForEquation.Particle);
         for (int s = 0; s < init.flagForEquation.Particle; s++) ptcl.emplace_back(fld, init, s);
         Collision col(fld, init, ptcl);
-        (void)col;
+        bool has_cathode = false;
+        for (auto &b : init.fieldBoundaries) has_cathode |= b.position == "hollow-cathode";
         Moment mom(fld);
         const bool out_field = T.FieldOutput != 0, out_ptcl = T.ParticleOutput != 0;
         // the reference tests FieldOutput before taking the cycle modulo
@@ -135,8 +133,17 @@ int main(int argc, char **argv)
             std::memset(&dg, 0, sizeof dg);
             std::vector<std::map<std::string, std::string>> flow;
             if (fused) {
-                check(gp_step(fld.ctx, dt, fld.tol, fld.maxiter, fld.solver, sort_every, cycle, &st, logc ? &dg : nullptr),
+                // the hollow cathode needs every cycle's wall counts
+                const bool want = logc || col.existHC;
+                check(gp_step(fld.ctx, dt, fld.tol, fld.maxiter, fld.solver, sort_every, cycle, &st, want ? &dg : nullptr),
                       "gp_step");
+                if (want)
+                    for (size_t s = 0; s < ptcl.size() && s < 8; s++) {   // Particle.mm:914-961
+                        Particle &p = ptcl[s];
+                        const int qe = (int)(p.q / (float)ec);
+                        p.pinum_Xmin = (int)dg.absorbed[s][0] * qe; p.pinum_Xmax = (int)dg.absorbed[s][1] * qe;
+                        p.pinum_Ymin = (int)dg.absorbed[s][2] * qe; p.pinum_Ymax = (int)dg.absorbed[s][3] * qe;
+                    }
             } else {
                 fld.resetChargeDensity();
                 for (auto &p : ptcl) p.integrateChargeDensity();
@@ -152,6 +159,11 @@ int main(int argc, char **argv)
                 if (sort_every > 0 && cycle % sort_every == 0)
                     for (auto &p : ptcl) check(gp_sort(fld.ctx, p.sid), "sort");
             }
+            // particle sources after the reduce (main.mm:219-232); the fused
+            // step deposits the new particles into the next solve's charge
+            const int ai_addn = col.artificialIonization(dt, fused);
+            int hc_neglected = 0;
+            const int hc_addn = col.hollowCathode(fused, hc_neglected);
             // outputs of main.mm:190-233.  The fields after the push are those
             // of the solve (the push changes no field), so the fused step
             // writes them here too.
@@ -184,13 +196,28 @@ int main(int argc, char **argv)
                         row[std::string(key[k]) + "_error"] = b;
                     }
                 }
+                if (col.existAI) row["artificialIonization_addn_injected"] = std::to_string(ai_addn);
+                if (col.existHC) {
+                    row["hollowCathode_addn_injected"] = std::to_string(hc_addn);
+                    row["hollowCathode_addn_neglected"] = std::to_string(hc_neglected);
+                }
+                if (has_cathode) {
+                    std::snprintf(b, sizeof b, "%.6e", dg.mean_cathode * sVtoV); row["meanCathode"] = b;
+                }
                 std::snprintf(b, sizeof b, "%.6e", dg.we_ref * ergtoev); row["totalElectricEnergy"] = b;
                 std::snprintf(b, sizeof b, "%.9e", dg.we_cgs); row["fieldEnergyCGS"] = b;
                 std::snprintf(b, sizeof b, "%.9e", dg.kinetic[0] + dg.kinetic[1]); row["kineticEnergy"] = b;
                 row["elapsed_us"] = std::to_string(
                     std::chrono::duration_cast<std::chrono::microseconds>(c1 - c0).count());
-                for (size_t s = 0; s < ptcl.size(); s++)
-                    row["pNum_" + ptcl[s].pName] = std::to_string(ptcl[s].count());
+                for (size_t s = 0; s < ptcl.size(); s++) {
+                    const Particle &p = ptcl[s];
+                    row["pNum_" + p.pName] = std::to_string(p.count());
+                    // the flowout_<species> wall counts in charge units (Particle.mm:990-1001)
+                    row[p.pName + "_Xmin"] = std::to_string(p.pinum_Xmin);
+                    row[p.pName + "_Xmax"] = std::to_string(p.pinum_Xmax);
+                    row[p.pName + "_Ymin"] = std::to_string(p.pinum_Ymin);
+                    row[p.pName + "_Ymax"] = std::to_string(p.pinum_Ymax);
+                }
                 logger.row(row);
                 std::printf("Frame %d completed\n", cycle);
             }

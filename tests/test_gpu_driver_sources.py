@@ -1,0 +1,111 @@
+"""The driver's particle sources on the reference's Hall-thruster condition
+(tests/golden/conditions/org.json, narrowed to 40 rows and 10 particles per
+cell, 30 cycles): x-sinusoidal artificial ionization in 0.25-1.0 cm
+(Collision.mm:464-566), the hollow-cathode electron line at 2.4 cm fed by the
+current lost at Xmin (:568-678), the cathode potential ramp
+(EMField.mm:1063-1077), and a 1D magnetic field read from files in the
+format of tools/makeBfld.py.
+
+From the per-cycle log: each species' count moves by exactly its wall
+losses plus the sources; the ionization count is src / w * dt rounded down
+or up; the cathode injects -(sum of the species' charge lost at Xmin) when
+that is positive and reports it as neglected otherwise; every solve
+converges.  The fused step and the reference-order pieces both keep the
+balance."""
+import csv
+import json
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "gample_amd", "bin", "gample")
+COND = os.path.join(ROOT, "tests", "golden", "conditions", "org.json")
+C = 2.99792458e10
+EC = 4.803204197e-10
+JTOSJ = 1 / (10 / C * 1e4)
+NGY, PPC, END = 40, 10, 30
+
+
+@pytest.fixture(scope="module", autouse=True)
+def built():
+    if not os.path.exists(EXE):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "gample_amd", "host")])
+
+
+def write_1d(path, values):
+    """tools/makeBfld.py write_1d_field: int32 n, float32 d, n + 1 float32"""
+    v = np.asarray(values, np.float32)
+    with open(path, "wb") as f:
+        f.write(np.int32(len(v) - 1).tobytes())
+        f.write(np.float32(0.005).tobytes())
+        f.write(v.tobytes())
+
+
+def condition(tmp, project):
+    raw = json.load(open(COND))
+    raw["ParamForTimeIntegration"].update(End=END, LogOutput=1, FieldOutput=1000, ProgressOutput=100000,
+                                          ProjectName=project)
+    raw["ParamForComputing"].update(MaximumParticleNumber=500000, ToleranceForPoisson=1e-8, MaxiterForPoisson=400)
+    F = raw["ParamForField"]
+    F["NumberOfGridY"] = NGY
+    nx = F["NumberOfGridX"]                          # order 1: no ghost cells
+    x = np.arange(nx + 2) * F["GridSizeOfX"]
+    bz = 0.015 * np.exp(-((x - 1.0) / 0.5) ** 2)    # T
+    for k, v in (("Bx", np.zeros_like(x)), ("By", np.zeros_like(x)), ("Bz", bz)):
+        write_1d(os.path.join(tmp, k + ".bin"), v)
+        F["FilePathOf" + k] = os.path.join(tmp, k + ".bin")
+    for p in raw["ParamForParticle"]:
+        p["pNumValue"] = PPC
+    path = os.path.join(tmp, project + ".json")
+    json.dump(raw, open(path, "w"))
+    return path, raw
+
+
+def run(tmp, cfg, extra=()):
+    print(f"[driver sources] {cfg} {' '.join(extra)}", flush=True)
+    out = subprocess.run([EXE, "-i", cfg, "-o", "t", *extra], cwd=tmp, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+
+
+@pytest.mark.parametrize("mode", ["fused", "unfused"])
+def test_hall_thruster_sources(tmp_path, mode):
+    tmp = str(tmp_path)
+    cfg, raw = condition(tmp, "src")
+    run(tmp, cfg, () if mode == "fused" else ("--unfused",))
+    rows = list(csv.DictReader(open(os.path.join(tmp, "src_t_log.csv"))))
+    assert [int(r["cycle"]) for r in rows] == list(range(1, END + 1))
+    F = raw["ParamForField"]
+    n0 = F["NumberOfGridX"] * NGY * PPC
+    # addn = float(src) / float(w_e) * dt (Collision.mm:482), src from
+    # CurrentDensity[A/m2] (Init.mm:240-241), w from InitialNumDens (Init.mm:192-200)
+    ai = raw["SourceForParticle"][0]
+    src = np.float32(ai["SrcValue"] * JTOSJ * NGY * F["GridSizeOfY"] / EC)
+    w = np.float32(raw["ParamForParticle"][0]["WeightValue"] * F["GridSizeOfX"] * F["GridSizeOfY"] / PPC)
+    addn_d = float(src / w) * raw["ParamForTimeIntegration"]["TimeStep"]
+    ne, ni = n0, n0
+    ai_counts, hc_total = [], 0
+    for r in rows:
+        walls = ("_Xmin", "_Xmax", "_Ymin", "_Ymax")
+        pe = [int(r["electron" + k]) for k in walls]    # charge units: minus the electrons lost
+        pi = [int(r["ion_Xe1" + k]) for k in walls]
+        a = int(r["artificialIonization_addn_injected"])
+        h = int(r["hollowCathode_addn_injected"])
+        neg = int(r["hollowCathode_addn_neglected"])
+        assert a in (math.floor(addn_d), math.floor(addn_d) + 1)
+        net = -(pe[0] + pi[0])
+        assert (h, neg) == ((net, 0) if net > 0 else (0, -net))
+        ne += sum(pe) + a + h
+        ni += -sum(pi) + a
+        assert int(r["pNum_electron"]) == ne and int(r["pNum_ion_Xe1"]) == ni
+        assert float(r["error"]) <= 1e-8
+        assert math.isfinite(float(r["meanCathode"]))
+        ai_counts.append(a)
+        hc_total += h
+    assert abs(np.mean(ai_counts) - addn_d) < 0.5
+    assert hc_total > 0
