@@ -9,6 +9,7 @@
 #include <stdexcept>
 
 #include "Constant.h"
+#include "XmlLogger.h"
 
 namespace gample {
 
@@ -54,6 +55,25 @@ void record(File &o, const std::string &path, const std::string &name, int type,
     o.put(v.data(), v.size() * sizeof(float), path);
 }
 
+// min and max of an array as floats (FileIO.mm:68-88 keeps float arrays)
+template <typename T>
+std::pair<float, float> minmax(const T *a, size_t n)
+{
+    float lo = 1e20f, hi = -1e20f;
+    for (size_t i = 0; i < n; i++) {
+        const float v = (float)a[i];
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    }
+    return {lo, hi};
+}
+
+void put_minmax(std::map<std::string, std::string> &d, const std::string &key, std::pair<float, float> mm, double scale)
+{
+    d[key + "_min"] = fmtSci((float)(mm.first * scale), 6);
+    d[key + "_max"] = fmtSci((float)(mm.second * scale), 6);
+}
+
 template <typename T>
 std::vector<T> get(EMField &fld, int which, size_t n)
 {
@@ -69,7 +89,7 @@ void ensureOutputDir()
     if (mkdir("bin", 0755) != 0 && errno != EEXIST) throw std::runtime_error("cannot create the output directory bin");
 }
 
-void outputField(int cycle, EMField &fld, const Init &init)
+std::map<std::string, std::string> outputField(int cycle, EMField &fld, const Init &init)
 {
     const std::string path = "bin/" + init.paramForTimeIntegration.ProjectName + "_EMField_" + fmt_cycle(cycle) + ".bin";
     File o(path, "wb");
@@ -96,10 +116,17 @@ void outputField(int cycle, EMField &fld, const Init &init)
         record(o, path, "delEy", 2, dey.data(), nEy, (float)GtoV);
         record(o, path, "delBz", 3, dbz.data(), nB, (float)GtoT);
     }
+    // the outputField log section (FileIO.mm:89-101)
+    std::map<std::string, std::string> d;
+    put_minmax(d, "rho", minmax(rho.data(), nN), 1.0);
+    put_minmax(d, "phi", minmax(phi.data(), nP), sVtoV);
+    put_minmax(d, "Ex", minmax(ex.data(), nEx), GtoV);
+    put_minmax(d, "Ey", minmax(ey.data(), nEy), GtoV);
+    return d;
 }
 
-void outputMoments(int cycle, EMField &fld, const std::vector<double> &mom, const std::string &pName,
-                   const Init &init)
+std::map<std::string, std::string> outputMoments(int cycle, EMField &fld, const std::vector<double> &mom,
+                                                 const std::string &pName, const Init &init)
 {
     const std::string path = "bin/" + init.paramForTimeIntegration.ProjectName + "_Moments_" + pName + "_" +
                              fmt_cycle(cycle) + ".bin";
@@ -111,6 +138,11 @@ void outputMoments(int cycle, EMField &fld, const std::vector<double> &mom, cons
     static const char *suffix[10] = {"_n", "_ux", "_uy", "_uz", "_Pxx", "_Pxy", "_Pxz", "_Pyy", "_Pyz", "_Pzz"};
     for (int k = 0; k < 10; k++)   // 1/cm3, cm/s, and 0.1 Pa -> Pa
         record(o, path, pName + suffix[k], 0, mom.data() + k * ng, ng, k < 4 ? 1.0f : 0.1f);
+    // the outputMoment_<species> log section (FileIO.mm:181-197); uz is
+    // computed here, where the reference's loop repeats uy (FileIO.mm:178)
+    std::map<std::string, std::string> d;
+    for (int k = 0; k < 10; k++) put_minmax(d, suffix[k] + 1, minmax(mom.data() + k * ng, ng), k < 4 ? 1.0 : 0.1);
+    return d;
 }
 
 void outputPhaseSpace(int cycle, Particle &p, const Init &init)

@@ -12,6 +12,7 @@
 // units, as tools/field.py reads them.  A progress file is the particle
 // array as 24-byte records {x, y, vx, vy, vz: float32; flag: int32}.
 #pragma once
+#include <map>
 #include <string>
 #include <vector>
 
@@ -22,9 +23,11 @@ namespace gample {
 
 // the output directory of every file above ("bin", created when missing)
 void ensureOutputDir();
-void outputField(int cycle, EMField &fld, const Init &init);
-void outputMoments(int cycle, EMField &fld, const std::vector<double> &mom, const std::string &pName,
-                   const Init &init);
+// each returns its log section: min and max of the written arrays in the
+// written units (outputField FileIO.mm:68-101, outputMoments :162-197)
+std::map<std::string, std::string> outputField(int cycle, EMField &fld, const Init &init);
+std::map<std::string, std::string> outputMoments(int cycle, EMField &fld, const std::vector<double> &mom,
+                                                 const std::string &pName, const Init &init);
 void outputPhaseSpace(int cycle, Particle &p, const Init &init);
 void saveProgress(int cycle, std::vector<Particle> &ptcl, const Init &init);
 // reads bin/<RestartName>_ProgressData_<species>_%08d.bin into each species

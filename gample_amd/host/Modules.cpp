@@ -14,39 +14,6 @@ void check(int rc, const char *what)
     if (rc < 0) throw std::runtime_error(std::string(what) + ": " + gp_last_error());
 }
 
-// ---------------------------------------------------------------- Logger
-Logger::Logger(const std::string &b, bool enabled) : base(b)
-{
-    if (!enabled) return;
-    log_ = std::fopen((base + "_log.csv").c_str(), "w");
-    csv_ = std::fopen((base + "_timeseries.csv").c_str(), "w");
-    if (csv_) std::fprintf(csv_, "cycle,time,phi1,Ex1,Ey1\n");   // main.mm:84
-}
-Logger::~Logger()
-{
-    if (log_) std::fclose(log_);
-    if (csv_) std::fclose(csv_);
-}
-void Logger::row(const std::map<std::string, std::string> &kv)
-{
-    if (!log_) return;
-    if (cols_.empty()) {
-        for (auto &p : kv) cols_.push_back(p.first);
-        for (size_t i = 0; i < cols_.size(); i++) std::fprintf(log_, "%s%s", i ? "," : "", cols_[i].c_str());
-        std::fprintf(log_, "\n");
-    }
-    for (size_t i = 0; i < cols_.size(); i++) {
-        auto it = kv.find(cols_[i]);
-        std::fprintf(log_, "%s%s", i ? "," : "", it == kv.end() ? "" : it->second.c_str());
-    }
-    std::fprintf(log_, "\n");
-    std::fflush(log_);
-}
-void Logger::probe(int cycle, double time, double phi, double ex, double ey)
-{
-    if (csv_) std::fprintf(csv_, "%d,%.6e,%.6e,%.6e,%.6e\n", cycle, time, phi, ex, ey);
-}
-
 // ---------------------------------------------------------------- EMField
 static int fbc_code(const std::string &t)
 {
@@ -196,18 +163,19 @@ std::map<std::string, std::string> Particle::reduce()
     const int qe = (int)(q / (float)ec);      // charge units, Particle.mm:914-961
     pinum_Xmin = (int)cnt[0] * qe; pinum_Xmax = (int)cnt[1] * qe;
     pinum_Ymin = (int)cnt[2] * qe; pinum_Ymax = (int)cnt[3] * qe;
+    return flowout(cnt[0] + cnt[1] + cnt[2] + cnt[3], ke);
+}
+
+std::map<std::string, std::string> Particle::flowout(int64_t pulled, double ke)
+{
     const double diff = ke - lastKE;
     lastKE = ke;
-    char b[64];
-    std::map<std::string, std::string> row;
-    row["species"] = pName;
-    row["particleNumber"] = std::to_string(count());
-    row["pulledPtclNum"] = std::to_string(cnt[0] + cnt[1] + cnt[2] + cnt[3]);
-    row["Xmin"] = std::to_string(pinum_Xmin); row["Xmax"] = std::to_string(pinum_Xmax);
-    row["Ymin"] = std::to_string(pinum_Ymin); row["Ymax"] = std::to_string(pinum_Ymax);
-    std::snprintf(b, sizeof b, "%.6e", ke * ergtoev); row["totalKineticEnergy"] = b;
-    std::snprintf(b, sizeof b, "%.6e", diff * ergtoev); row["totalKEincrement"] = b;
-    return row;
+    return {{"particleNumber", std::to_string(count())},
+            {"pulledPtclNum", std::to_string(pulled)},
+            {"Xmin", std::to_string(pinum_Xmin)}, {"Xmax", std::to_string(pinum_Xmax)},
+            {"Ymin", std::to_string(pinum_Ymin)}, {"Ymax", std::to_string(pinum_Ymax)},
+            {"totalKineticEnergy", fmtSci((float)(ke * ergtoev), 6)},
+            {"totalKEincrement", fmtSci((float)(diff * ergtoev), 6)}};
 }
 
 // ---------------------------------------------------------------- Moment

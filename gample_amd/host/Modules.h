@@ -14,25 +14,11 @@
 
 #include "../../include/gample_amd.h"
 #include "Init.h"
+#include "XmlLogger.h"
 
 namespace gample {
 
 void check(int rc, const char *what);
-
-// per-cycle log: the scalars of the reference's XML sections
-// (flowout_<species>, solvePoisson, elapsedTime) as CSV rows
-class Logger {
-public:
-    Logger(const std::string &base, bool enabled);
-    ~Logger();
-    void row(const std::map<std::string, std::string> &kv);
-    void probe(int cycle, double time, double phi, double ex, double ey);
-    std::string base;
-
-private:
-    FILE *log_ = nullptr, *csv_ = nullptr;
-    std::vector<std::string> cols_;
-};
 
 class EMField {
 public:
@@ -57,6 +43,9 @@ public:
     void update(double dt, bool fused_deposit);     // Particle.mm:615-891
     // Particle.mm:893-1002; returns the flowout_<species> log section
     std::map<std::string, std::string> reduce();
+    // the flowout_<species> section (Particle.mm:989-1001) from pulled
+    // particles and the kinetic energy (erg), pinum_* already set
+    std::map<std::string, std::string> flowout(int64_t pulled, double ke);
     long long count() const;
     std::string pName;
     float m, q, w;

@@ -6,9 +6,16 @@
 // Poisson solve, then for each cycle: deposit -> Poisson [-> Darwin] ->
 // push -> reduce, with the Kahan-compensated time (main.mm:150-163).  By
 // default the deposit of the next cycle is fused into the push kernel
-// (gp_step); --unfused runs the pieces in the reference's order.
+// (gp_step); --unfused runs the pieces in the reference's order.  Every
+// LogOutput cycles the run log <ProjectName>_<tag>_log.xml receives the
+// reference's sections (solvePoisson, solveVectorPotential, outputField,
+// flowout_<species>, outputMoment_<species>, artificialIonization,
+// hollowCathode, elapsedTime, memoryUsage) and _timeseries.csv a probe row.
+// Under the fused step elapsedTime holds gp_step and the sources; the
+// reference-order pieces time integCDens_, solvePoisson, update_, reduce_.
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <ctime>
 #include <map>
@@ -33,32 +40,70 @@ static std::map<std::string, std::string> parseArgs(int argc, char **argv)   // 
     return o;
 }
 
-static void dump(const Init &in)
+static std::string dump(const Init &in)
 {
     const auto &F = in.paramForField;
-    std::printf("{\"Particle\":%d,\"EMField\":%d,\"Start\":%d,\"End\":%d,\"TimeStep\":%.17g,"
-                "\"pNumMax\":%lld,\"maxiter\":%d,\"tolerance\":%.17g,\"ngx\":%d,\"ngy\":%d,\"ngb\":%d,"
-                "\"dx\":%.17g,\"dy\":%.17g,\"weightOrder\":%d,\"BorisOrder\":%d,\"checkCurDens\":%d,"
-                "\"SortInterval\":%d,\"SortDriftFraction\":%.17g,\"species\":[",
-                in.flagForEquation.Particle, in.flagForEquation.EMField, in.paramForTimeIntegration.Start,
-                in.paramForTimeIntegration.End, in.paramForTimeIntegration.TimeStep, in.paramForComputing.pNumMax,
-                in.paramForComputing.maxiter, in.paramForComputing.tolerance, F.ngx, F.ngy, F.ngb, F.dx, F.dy,
-                F.weightOrder, F.BorisOrder, F.checkCurDens, in.paramForComputing.sortInterval,
-                in.paramForComputing.sortDrift);
+    std::string o;
+    char b[1024];
+    std::snprintf(b, sizeof b,
+                  "{\"Particle\":%d,\"EMField\":%d,\"Start\":%d,\"End\":%d,\"TimeStep\":%.17g,"
+                  "\"pNumMax\":%lld,\"maxiter\":%d,\"tolerance\":%.17g,\"ngx\":%d,\"ngy\":%d,\"ngb\":%d,"
+                  "\"dx\":%.17g,\"dy\":%.17g,\"weightOrder\":%d,\"BorisOrder\":%d,\"checkCurDens\":%d,"
+                  "\"SortInterval\":%d,\"SortDriftFraction\":%.17g,\"species\":[",
+                  in.flagForEquation.Particle, in.flagForEquation.EMField, in.paramForTimeIntegration.Start,
+                  in.paramForTimeIntegration.End, in.paramForTimeIntegration.TimeStep, in.paramForComputing.pNumMax,
+                  in.paramForComputing.maxiter, in.paramForComputing.tolerance, F.ngx, F.ngy, F.ngb, F.dx, F.dy,
+                  F.weightOrder, F.BorisOrder, F.checkCurDens, in.paramForComputing.sortInterval,
+                  in.paramForComputing.sortDrift);
+    o += b;
     for (size_t s = 0; s < in.paramForParticle.size(); s++) {
         const auto &p = in.paramForParticle[s];
-        std::printf("%s{\"name\":\"%s\",\"pNum\":%lld,\"q\":%.17g,\"m\":%.17g,\"w\":%.17g,\"genType\":\"%s\","
-                    "\"genT\":%.17g}", s ? "," : "", p.pName.c_str(), p.pNum, p.q, p.m, p.w, p.genType.c_str(), p.genT);
+        std::snprintf(b, sizeof b,
+                      "%s{\"name\":\"%s\",\"pNum\":%lld,\"q\":%.17g,\"m\":%.17g,\"w\":%.17g,\"genType\":\"%s\","
+                      "\"genT\":%.17g}",
+                      s ? "," : "", p.pName.c_str(), p.pNum, p.q, p.m, p.w, p.genType.c_str(), p.genT);
+        o += b;
     }
-    std::printf("],\"pbc\":[");
-    for (size_t i = 0; i < in.particleBoundaries.size(); i++)
-        std::printf("%s[\"%s\",\"%s\"]", i ? "," : "", in.particleBoundaries[i].position.c_str(),
-                    in.particleBoundaries[i].type.c_str());
-    std::printf("],\"fbc\":[");
-    for (size_t i = 0; i < in.fieldBoundaries.size(); i++)
-        std::printf("%s[\"%s\",\"%s\",%.17g]", i ? "," : "", in.fieldBoundaries[i].position.c_str(),
-                    in.fieldBoundaries[i].type.c_str(), in.fieldBoundaries[i].val);
-    std::printf("],\"sources\":%zu}\n", in.particleSources.size());
+    o += "],\"pbc\":[";
+    for (size_t i = 0; i < in.particleBoundaries.size(); i++) {
+        std::snprintf(b, sizeof b, "%s[\"%s\",\"%s\"]", i ? "," : "", in.particleBoundaries[i].position.c_str(),
+                      in.particleBoundaries[i].type.c_str());
+        o += b;
+    }
+    o += "],\"fbc\":[";
+    for (size_t i = 0; i < in.fieldBoundaries.size(); i++) {
+        std::snprintf(b, sizeof b, "%s[\"%s\",\"%s\",%.17g]", i ? "," : "", in.fieldBoundaries[i].position.c_str(),
+                      in.fieldBoundaries[i].type.c_str(), in.fieldBoundaries[i].val);
+        o += b;
+    }
+    std::snprintf(b, sizeof b, "],\"sources\":%zu}", in.particleSources.size());
+    return o + b;
+}
+
+// resident set of this process in kB (/proc/self/status VmRSS)
+static long rss_kb()
+{
+    FILE *f = std::fopen("/proc/self/status", "r");
+    if (!f) return 0;
+    char line[256];
+    long kb = 0;
+    while (std::fgets(line, sizeof line, f))
+        if (std::strncmp(line, "VmRSS:", 6) == 0) { kb = std::atol(line + 6); break; }
+    std::fclose(f);
+    return kb;
+}
+
+// the solvePoisson section (EMField.mm:1118-1127)
+static std::map<std::string, std::string> poisson_section(const gp_solve_stats &st, const gp_diag &dg, double &lastEE)
+{
+    const double diff = dg.we_ref - lastEE;
+    lastEE = dg.we_ref;
+    return {{"iteration", std::to_string(st.iterations)},
+            {"error", fmtSci((float)st.relres, 6)},
+            {"meanCathode", fmtSci((float)(dg.mean_cathode * sVtoV), 6)},
+            {"totalElectricEnergy", fmtSci((float)(dg.we_ref * ergtoev), 6)},
+            {"totalEEincrement", fmtSci((float)(diff * ergtoev), 6)},
+            {"jouleHeating", fmtSci((float)(dg.joule * ergtoev), 6)}};
 }
 
 int main(int argc, char **argv)
@@ -75,7 +120,7 @@ int main(int argc, char **argv)
         std::fprintf(stderr, "[Fatal] parseInputFile Failed: %s\n", e.what());
         return -1;
     }
-    if (args.count("--dump-config")) { dump(init); return 0; }
+    if (args.count("--dump-config")) { std::printf("%s\n", dump(init).c_str()); return 0; }
     std::string tag;
     if (args.count("-o")) tag = args["-o"];
     else {
@@ -87,7 +132,9 @@ int main(int argc, char **argv)
     const auto &T = init.paramForTimeIntegration;
     const bool fused = !args.count("--unfused");
     try {
-        Logger logger(T.ProjectName + "_" + tag, true);
+        // main.mm:80-88: <ProjectName>_<tag>_log.xml and _timeseries.csv
+        XmlLogger logger(T.ProjectName + "_" + tag, "cycle,time,phi1,Ex1,Ey1");
+        logger.logComment("parsed condition: " + dump(init) + "\n");
         EMField fld(init, 0);
         std::vector<Particle> ptcl;
         ptcl.reserve(init.flagForEquation.Particle);
@@ -122,109 +169,124 @@ int main(int argc, char **argv)
         double time = T.Start * dt, comp = 0.0;
         const int sort_every = init.paramForComputing.sortInterval;
         auto t_start = std::chrono::steady_clock::now();
+        const int LO = T.LogOutput;
+        double lastEE = 0.0;
         for (int cycle = T.Start + 1; cycle <= T.End; cycle++) {
+            const bool logc = LO > 0 && cycle % LO == 0;
+            logger.switchLog(logc);
             const double y = dt - comp, t = time + y;
             comp = (t - time) - y;
             time = t;
-            const bool logc = T.LogOutput > 0 && cycle % T.LogOutput == 0;
-            auto c0 = std::chrono::steady_clock::now();
+            logger.logCycleStart(cycle, time);
+            // the increments of the logged energies are against the previous
+            // cycle, so the diagnostics are also taken the cycle before a
+            // logged one; the hollow cathode needs every cycle's wall counts
+            const bool want = logc || (LO > 0 && (cycle + 1) % LO == 0) || col.existHC;
+            const bool fieldc = out_field && cycle % T.FieldOutput == 0;
+            std::map<std::string, std::string> elapsed;
+            auto timed = [&](const std::string &name, auto &&fn) {   // ResourceUtils.h:40-48
+                auto a0 = std::chrono::steady_clock::now();
+                fn();
+                auto a1 = std::chrono::steady_clock::now();
+                elapsed[name] = std::to_string(std::chrono::duration_cast<std::chrono::microseconds>(a1 - a0).count());
+            };
             gp_solve_stats st;
             gp_diag dg;
+            std::memset(&st, 0, sizeof st);
             std::memset(&dg, 0, sizeof dg);
-            std::vector<std::map<std::string, std::string>> flow;
-            if (fused) {
-                // the hollow cathode needs every cycle's wall counts
-                const bool want = logc || col.existHC;
-                check(gp_step(fld.ctx, dt, fld.tol, fld.maxiter, fld.solver, sort_every, cycle, &st, want ? &dg : nullptr),
-                      "gp_step");
-                if (want)
-                    for (size_t s = 0; s < ptcl.size() && s < 8; s++) {   // Particle.mm:914-961
-                        Particle &p = ptcl[s];
-                        const int qe = (int)(p.q / (float)ec);
-                        p.pinum_Xmin = (int)dg.absorbed[s][0] * qe; p.pinum_Xmax = (int)dg.absorbed[s][1] * qe;
-                        p.pinum_Ymin = (int)dg.absorbed[s][2] * qe; p.pinum_Ymax = (int)dg.absorbed[s][3] * qe;
+            // the solve's log sections on logged cycles (the energy of the cycle
+            // before one is kept for the increment), then the field output
+            auto field_sections = [&]() {
+                if (logc) {
+                    logger.logSection("solvePoisson", poisson_section(st, dg, lastEE));
+                    double phi, ex, ey;
+                    fld.probe(phi, ex, ey);
+                    logger.csvOutput(fmtSci((float)phi, 6) + "," + fmtSci((float)ex, 6) + "," + fmtSci((float)ey, 6));
+                    if (init.flagForEquation.EMField == 2) {
+                        // the solveVectorPotential section (EMField.mm:1349-1360)
+                        gp_solve_stats ds[3];
+                        check(gp_darwin_stats(fld.ctx, ds), "gp_darwin_stats");
+                        static const char *key[3] = {"decompose", "solveAx", "solveAy"};
+                        std::map<std::string, std::string> d;
+                        for (int k = 0; k < 3; k++) {
+                            d[std::string(key[k]) + "_iteration"] = std::to_string(ds[k].iterations);
+                            d[std::string(key[k]) + "_error"] = fmtSci((float)ds[k].relres, 6);
+                        }
+                        logger.logSection("solveVectorPotential", d);
                     }
+                } else if (want) {
+                    lastEE = dg.we_ref;
+                }
+                if (fieldc) logger.logSection("outputField", outputField(cycle, fld, init));
+            };
+            auto species_outputs = [&](Particle &p, const std::map<std::string, std::string> &flow) {
+                logger.logSection("flowout_" + p.pName, flow);
+                if (out_ptcl && cycle % T.ParticleOutput == 0) outputPhaseSpace(cycle, p, init);
+                if (fieldc)
+                    logger.logSection("outputMoment_" + p.pName,
+                                      outputMoments(cycle, fld, mom.integrateMoments(p), p.pName, init));
+            };
+            if (fused) {
+                timed("gp_step", [&] {
+                    check(gp_step(fld.ctx, dt, fld.tol, fld.maxiter, fld.solver, sort_every, cycle, &st,
+                                  want ? &dg : nullptr), "gp_step");
+                });
+                // the fields after the push are those of the solve (the push
+                // changes no field), so the field outputs follow the step
+                field_sections();
+                for (size_t s = 0; s < ptcl.size(); s++) {
+                    Particle &p = ptcl[s];
+                    if (!want || s >= 8) {
+                        if (fieldc || (out_ptcl && cycle % T.ParticleOutput == 0)) species_outputs(p, {});
+                        continue;
+                    }
+                    const int qe = (int)(p.q / (float)ec);   // Particle.mm:914-961
+                    const int64_t *a = dg.absorbed[s];
+                    p.pinum_Xmin = (int)a[0] * qe; p.pinum_Xmax = (int)a[1] * qe;
+                    p.pinum_Ymin = (int)a[2] * qe; p.pinum_Ymax = (int)a[3] * qe;
+                    species_outputs(p, p.flowout(a[0] + a[1] + a[2] + a[3], dg.kinetic[s]));
+                }
             } else {
                 fld.resetChargeDensity();
-                for (auto &p : ptcl) p.integrateChargeDensity();
-                st = fld.solvePoisson(dt, logc ? &dg : nullptr);
+                for (auto &p : ptcl) timed("integCDens_" + p.pName, [&] { p.integrateChargeDensity(); });
+                timed("solvePoisson", [&] { st = fld.solvePoisson(dt, want ? &dg : nullptr); });
                 if (init.flagForEquation.EMField == 2) {
                     gp_solve_stats ds[3];
-                    fld.solveVectorPotential(dt, ds);
+                    timed("solveVectorPotential", [&] { fld.solveVectorPotential(dt, ds); });
                 }
+                field_sections();
                 for (auto &p : ptcl) {
-                    p.update(dt, false);
-                    flow.push_back(p.reduce());
+                    std::map<std::string, std::string> flow;
+                    timed("update_" + p.pName, [&] { p.update(dt, false); });
+                    timed("reduce_" + p.pName, [&] { flow = p.reduce(); });
+                    species_outputs(p, flow);
                 }
                 if (sort_every > 0 && cycle % sort_every == 0)
                     for (auto &p : ptcl) check(gp_sort(fld.ctx, p.sid), "sort");
             }
-            // particle sources after the reduce (main.mm:219-232); the fused
-            // step deposits the new particles into the next solve's charge
-            const int ai_addn = col.artificialIonization(dt, fused);
-            int hc_neglected = 0;
-            const int hc_addn = col.hollowCathode(fused, hc_neglected);
-            // outputs of main.mm:190-233.  The fields after the push are those
-            // of the solve (the push changes no field), so the fused step
-            // writes them here too.
-            if (out_field && cycle % T.FieldOutput == 0) outputField(cycle, fld, init);
-            for (auto &p : ptcl) {
-                if (out_ptcl && cycle % T.ParticleOutput == 0) outputPhaseSpace(cycle, p, init);
-                if (out_field && cycle % T.FieldOutput == 0)
-                    outputMoments(cycle, fld, mom.integrateMoments(p), p.pName, init);
-            }
+            // particle sources after the reduce and the species outputs
+            // (main.mm:219-232); the fused step deposits the new particles
+            // into the next solve's charge
+            int ai_addn = 0, hc_addn = 0, hc_neglected = 0;
+            timed("artificialIonization", [&] { ai_addn = col.artificialIonization(dt, fused); });
+            if (col.existAI) logger.logSection("artificialIonization", {{"addn_injected", std::to_string(ai_addn)}});
+            timed("hollowCathode", [&] { hc_addn = col.hollowCathode(fused, hc_neglected); });
+            if (col.existHC)
+                logger.logSection("hollowCathode", {{"addn_neglected", std::to_string(hc_neglected)},
+                                                    {"addn_injected", std::to_string(hc_addn)}});
             if (out_prog && cycle % T.ProgressOutput == 0) saveProgress(cycle, ptcl, init);
-            auto c1 = std::chrono::steady_clock::now();
-            if (logc) {
-                double phi, ex, ey;
-                fld.probe(phi, ex, ey);
-                logger.probe(cycle, time, phi, ex, ey);
-                char b[64];
-                std::map<std::string, std::string> row;
-                row["cycle"] = std::to_string(cycle);
-                std::snprintf(b, sizeof b, "%.9e", time); row["time"] = b;
-                row["iteration"] = std::to_string(st.iterations);
-                std::snprintf(b, sizeof b, "%.3e", st.relres); row["error"] = b;
-                if (init.flagForEquation.EMField == 2) {
-                    // the solveVectorPotential section's keys (EMField.mm:1349-1360)
-                    gp_solve_stats ds[3];
-                    check(gp_darwin_stats(fld.ctx, ds), "gp_darwin_stats");
-                    static const char *key[3] = {"decompose", "solveAx", "solveAy"};
-                    for (int k = 0; k < 3; k++) {
-                        row[std::string(key[k]) + "_iteration"] = std::to_string(ds[k].iterations);
-                        std::snprintf(b, sizeof b, "%.6e", ds[k].relres);
-                        row[std::string(key[k]) + "_error"] = b;
-                    }
-                }
-                if (col.existAI) row["artificialIonization_addn_injected"] = std::to_string(ai_addn);
-                if (col.existHC) {
-                    row["hollowCathode_addn_injected"] = std::to_string(hc_addn);
-                    row["hollowCathode_addn_neglected"] = std::to_string(hc_neglected);
-                }
-                if (has_cathode) {
-                    std::snprintf(b, sizeof b, "%.6e", dg.mean_cathode * sVtoV); row["meanCathode"] = b;
-                }
-                std::snprintf(b, sizeof b, "%.6e", dg.we_ref * ergtoev); row["totalElectricEnergy"] = b;
-                std::snprintf(b, sizeof b, "%.9e", dg.we_cgs); row["fieldEnergyCGS"] = b;
-                std::snprintf(b, sizeof b, "%.9e", dg.kinetic[0] + dg.kinetic[1]); row["kineticEnergy"] = b;
-                row["elapsed_us"] = std::to_string(
-                    std::chrono::duration_cast<std::chrono::microseconds>(c1 - c0).count());
-                for (size_t s = 0; s < ptcl.size(); s++) {
-                    const Particle &p = ptcl[s];
-                    row["pNum_" + p.pName] = std::to_string(p.count());
-                    // the flowout_<species> wall counts in charge units (Particle.mm:990-1001)
-                    row[p.pName + "_Xmin"] = std::to_string(p.pinum_Xmin);
-                    row[p.pName + "_Xmax"] = std::to_string(p.pinum_Xmax);
-                    row[p.pName + "_Ymin"] = std::to_string(p.pinum_Ymin);
-                    row[p.pName + "_Ymax"] = std::to_string(p.pinum_Ymax);
-                }
-                logger.row(row);
-                std::printf("Frame %d completed\n", cycle);
-            }
+            // resources (main.mm:242-248): times in us, memory in kB (the
+            // resident set for both; the device memory is not counted)
+            logger.logSection("elapsedTime", elapsed);
+            const std::string rss = std::to_string(rss_kb());
+            logger.logSection("memoryUsage", {{"physicalFootprint", rss}, {"residentSetSize", rss}});
+            if (logc) std::printf("Frame %d completed\n", cycle);
+            logger.logCycleEnd();
         }
         auto t_end = std::chrono::steady_clock::now();
-        std::printf("calculation end. total elapsed time: %.3f sec.\n",
-                    std::chrono::duration<double>(t_end - t_start).count());
+        const double secs = std::chrono::duration<double>(t_end - t_start).count();
+        std::printf("calculation end. total elapsed time: %.3f sec.\n", secs);
+        logger.logComment("culculation end.\ntotal elapsed time: " + std::to_string((long)secs) + " sec.\n");
     } catch (const std::exception &e) {
         std::fprintf(stderr, "[Fatal] %s\n", e.what());
         return 1;

@@ -6,17 +6,17 @@ current lost at Xmin (:568-678), the cathode potential ramp
 (EMField.mm:1063-1077), and a 1D magnetic field read from files in the
 format of tools/makeBfld.py.
 
-From the per-cycle log: each species' count moves by exactly its wall
-losses plus the sources; the ionization count is src / w * dt rounded down
+From the per-cycle XML log, read as tools/parseXML.py reads it: each
+species' count moves by exactly its wall losses plus the sources; the ionization count is src / w * dt rounded down
 or up; the cathode injects -(sum of the species' charge lost at Xmin) when
 that is positive and reports it as neglected otherwise; every solve
 converges.  The fused step and the reference-order pieces both keep the
 balance."""
-import csv
 import json
 import math
 import os
 import subprocess
+import xml.etree.ElementTree as ET
 
 import numpy as np
 import pytest
@@ -67,6 +67,18 @@ def condition(tmp, project):
     return path, raw
 
 
+def read_log(path):
+    """the run log as tools/parseXML.py walks it: Cycle elements by ID, their
+    Sections by Name, each a dict of tag -> text"""
+    out = []
+    for cyc in ET.parse(path).getroot().findall("Cycle"):
+        r = {"cycle": int(cyc.get("ID"))}
+        for sec in cyc.findall("Section"):
+            r[sec.get("Name")] = {e.tag: e.text for e in sec}
+        out.append(r)
+    return out
+
+
 def run(tmp, cfg, extra=()):
     print(f"[driver sources] {cfg} {' '.join(extra)}", flush=True)
     out = subprocess.run([EXE, "-i", cfg, "-o", "t", *extra], cwd=tmp, capture_output=True, text=True, timeout=300)
@@ -78,8 +90,8 @@ def test_hall_thruster_sources(tmp_path, mode):
     tmp = str(tmp_path)
     cfg, raw = condition(tmp, "src")
     run(tmp, cfg, () if mode == "fused" else ("--unfused",))
-    rows = list(csv.DictReader(open(os.path.join(tmp, "src_t_log.csv"))))
-    assert [int(r["cycle"]) for r in rows] == list(range(1, END + 1))
+    rows = read_log(os.path.join(tmp, "src_t_log.xml"))
+    assert [r["cycle"] for r in rows] == list(range(1, END + 1))
     F = raw["ParamForField"]
     n0 = F["NumberOfGridX"] * NGY * PPC
     # addn = float(src) / float(w_e) * dt (Collision.mm:482), src from
@@ -91,20 +103,26 @@ def test_hall_thruster_sources(tmp_path, mode):
     ne, ni = n0, n0
     ai_counts, hc_total = [], 0
     for r in rows:
-        walls = ("_Xmin", "_Xmax", "_Ymin", "_Ymax")
-        pe = [int(r["electron" + k]) for k in walls]    # charge units: minus the electrons lost
-        pi = [int(r["ion_Xe1" + k]) for k in walls]
-        a = int(r["artificialIonization_addn_injected"])
-        h = int(r["hollowCathode_addn_injected"])
-        neg = int(r["hollowCathode_addn_neglected"])
+        walls = ("Xmin", "Xmax", "Ymin", "Ymax")
+        fe, fi = r["flowout_electron"], r["flowout_ion_Xe1"]
+        pe = [int(fe[k]) for k in walls]                # charge units: minus the electrons lost
+        pi = [int(fi[k]) for k in walls]
+        assert int(fe["pulledPtclNum"]) == -sum(pe) and int(fi["pulledPtclNum"]) == sum(pi)
+        a = int(r["artificialIonization"]["addn_injected"])
+        h = int(r["hollowCathode"]["addn_injected"])
+        neg = int(r["hollowCathode"]["addn_neglected"])
         assert a in (math.floor(addn_d), math.floor(addn_d) + 1)
         net = -(pe[0] + pi[0])
         assert (h, neg) == ((net, 0) if net > 0 else (0, -net))
+        # flowout's particleNumber is the count after the reduce, before the sources
+        assert int(fe["particleNumber"]) == ne + sum(pe) and int(fi["particleNumber"]) == ni - sum(pi)
         ne += sum(pe) + a + h
         ni += -sum(pi) + a
-        assert int(r["pNum_electron"]) == ne and int(r["pNum_ion_Xe1"]) == ni
-        assert float(r["error"]) <= 1e-8
-        assert math.isfinite(float(r["meanCathode"]))
+        sp = r["solvePoisson"]
+        assert float(sp["error"]) <= 1e-8 and int(sp["iteration"]) >= 0
+        assert math.isfinite(float(sp["meanCathode"])) and math.isfinite(float(sp["totalEEincrement"]))
+        assert set(r["memoryUsage"]) == {"physicalFootprint", "residentSetSize"}
+        assert all(int(v) >= 0 for v in r["elapsedTime"].values())
         ai_counts.append(a)
         hc_total += h
     assert abs(np.mean(ai_counts) - addn_d) < 0.5
