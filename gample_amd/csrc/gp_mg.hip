@@ -473,19 +473,30 @@ __device__ void bot_sweep(float *u, const float *f, const Lev &L, int color)
             u[i + j * L.mx] = gs_point(u, f, L, i, j, invd);
     __syncthreads();
 }
+// the first red sweep of a smoothing from u = 0: a red point's neighbours
+// are black and still zero, so gs_point gives (f w^-1 + 0 + 0) / d; the black
+// points need no zeroing (a sweep never reads the point it updates)
+__device__ void bot_sweep_red0(float *u, const float *f, const Lev &L)
+{
+    const float invd = 1.0f / (2.0f * (L.cx + L.cy));
+    for (int j = threadIdx.y; j < L.my; j += 32)
+        for (int i = 2 * threadIdx.x + (j & 1); i < L.mx; i += 64) {
+            const int k = i + j * L.mx;
+            u[k] = (f[k] * iweight(L, i, j) + L.cx * (0.0f + 0.0f) + L.cy * (0.0f + 0.0f)) * invd;
+        }
+    __syncthreads();
+}
 __device__ void bot_project(float *a, const Lev &L, float *scratch)
 {
     const int tid = threadIdx.x + 32 * threadIdx.y;
     float s = 0.0f;
     BOT_LOOP(L) s += a[i + j * L.mx];
-    scratch[tid] = s;
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((tid & 63) == 0) scratch[tid >> 6] = s;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
-        if (tid < w) scratch[tid] += scratch[tid + w];
-        __syncthreads();
-    }
-    const float m = scratch[0] / (float)(L.mx * L.my);
-    __syncthreads();
+    float t = 0.0f;
+    for (int w = 0; w < 16; w++) t += scratch[w];   // the same order in every thread
+    const float m = t / (float)(L.mx * L.my);
     BOT_LOOP(L) a[i + j * L.mx] -= m;
     __syncthreads();
 }
@@ -525,10 +536,10 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         float *u = lds + B.off[l], *f = u + n;
         float *fc = lds + B.off[l + 1] + nc;
         const float invd = 1.0f / (2.0f * (L.cx + L.cy));
-        // red from zero, black zero
-        BOT_LOOP(L) u[i + j * L.mx] = 0.0f;
-        __syncthreads();
-        for (int sw = 0; sw < NSW; sw++) { bot_sweep(u, f, L, 0); bot_sweep(u, f, L, 1); }
+        // from u = 0: the first red sweep reads no u
+        bot_sweep_red0(u, f, L);
+        bot_sweep(u, f, L, 1);
+        for (int sw = 1; sw < NSW; sw++) { bot_sweep(u, f, L, 0); bot_sweep(u, f, L, 1); }
         BOT_LOOP(L) res[i + j * L.mx] = resid_point(u, f, L, i, j, invd);
         __syncthreads();
         BOT_LOOP(C) {
@@ -552,7 +563,8 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         const Lev L = fixed<TX, TY>(B.lv[B.nl - 1]);
         const int n = L.mx * L.my;
         float *u = lds + B.off[B.nl - 1], *f = u + n;
-        if (B.singular) bot_project(f, L, scratch);
+        // (the dense inverse of a singular level has the projections folded in)
+        if (B.singular && !B.dense) bot_project(f, L, scratch);
         if (B.dense) {
             // u = M f with the precomputed inverse (n <= 256): thread t sums
             // quarter t/256 of row t%256, the four quarters add in LDS
@@ -577,12 +589,12 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
             if (tid < n) u[tid] = (scratch[tid] + scratch[tid + 256]) + (scratch[tid + 512] + scratch[tid + 768]);
             __syncthreads();
         } else {
-            BOT_LOOP(L) u[i + j * L.mx] = 0.0f;
-            __syncthreads();
-            for (int s = 0; s < B.sweeps; s++) { bot_sweep(u, f, L, 0); bot_sweep(u, f, L, 1); }
+            bot_sweep_red0(u, f, L);
+            bot_sweep(u, f, L, 1);
+            for (int s = 1; s < B.sweeps; s++) { bot_sweep(u, f, L, 0); bot_sweep(u, f, L, 1); }
             for (int s = 0; s < B.sweeps; s++) { bot_sweep(u, f, L, 1); bot_sweep(u, f, L, 0); }
         }
-        if (B.singular) bot_project(u, L, scratch);
+        if (B.singular && !B.dense) bot_project(u, L, scratch);
     }
     for (int l = B.nl - 2; l >= 0; l--) {
         const Lev L = fixed<TX, TY>(B.lv[l]), C = fixed<TX, TY>(B.lv[l + 1]);
@@ -1221,6 +1233,21 @@ int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double 
                         Inv[(size_t)r * m + q] -= fct * Inv[(size_t)col * m + q];
                     }
                 }
+            }
+            if (sing) {
+                // fold the projections of the singular level (zero-mean rhs in,
+                // zero-mean solution out) into the matrix: Inv <- P Inv P with
+                // P = I - 1 1' / m
+                std::vector<double> rs(m, 0.0), cs(m, 0.0);
+                double tot = 0.0;
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++) {
+                        const double v = Inv[(size_t)i * m + j];
+                        rs[i] += v; cs[j] += v; tot += v;
+                    }
+                for (int i = 0; i < m; i++)
+                    for (int j = 0; j < m; j++)
+                        Inv[(size_t)i * m + j] += (tot / m - rs[i] - cs[j]) / m;
             }
             std::vector<float> cm((size_t)m * m);   // column-major: cm[j*m + i] = Inv[i][j]
             for (int i = 0; i < m; i++)
