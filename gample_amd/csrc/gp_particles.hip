@@ -98,6 +98,32 @@ __device__ __forceinline__ float gather(P b, int pitch, const float *sx, const f
     return acc;
 }
 
+// row r of a window of pitch W as an offset, for 0 <= r < 2^24 (the 24-bit
+// multiply issues at full rate, a 32-bit one at a quarter)
+template <int W>
+__device__ __forceinline__ int rowW(int r) { return (int)__umul24((unsigned)r, (unsigned)W); }
+
+// the same gather from an LDS window: the window address is passed through an
+// empty asm statement, so the compiler keeps it in one register and reaches
+// the taps (at most 3 + 3 W dwords away) through the instructions' offset
+// fields, where it would otherwise add the array's LDS base for every pair
+template <int NW, int PITCH>
+__device__ __forceinline__ float gather_lds(const float *b, const float *sx, const float *sy)
+{
+    typedef const __attribute__((address_space(3))) float lds_f;
+    lds_f *p = (lds_f *)b;
+    asm volatile("" : "+v"(p));
+    float acc = 0.0f;
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+        float r = sy[0] * p[i];
+#pragma unroll
+        for (int j = 1; j < NW; j++) r = __builtin_fmaf(sy[j], p[i + j * PITCH], r);
+        acc = (i == 0) ? sx[0] * r : __builtin_fmaf(sx[i], r, acc);
+    }
+    return acc;
+}
+
 // exact round-to-nearest-even of c * 2^40 as int64 (|c| < 2^22)
 __device__ __forceinline__ long long fx40(float c)
 {
@@ -219,7 +245,7 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
 
     const float lox = (float)a.ngb, hix = (float)(a.ngb + a.ngx);
     const float loy = (float)a.ngb, hiy = (float)(a.ngb + a.ngy);
-    double ke = 0.0;
+    double ke = 0.0;   // sum of |v|^2 (times ke_coef after the loop)
     int drift = 0;   // live particles gathered outside their tile window
     // uniform B: btz = constB Bz (scaled by fN1 at Boris order 4/6, with
     // btbt from the unscaled btz), s = 2 btz / (1 + btbt)
@@ -285,10 +311,11 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
             if constexpr (DEP == 0) drift += inw ? 0 : 1;
             oi = i2; oj = j1;
             if (inw) {
-                Epx = gather<NW>(sEx + (i1 + 1 - NGB - X0) + (j1 - NGB - Y0) * W, W, sx0, sy0);
-                Epy = gather<NW>(sEy + (i2 - NGB - X0) + (j2 + 1 - NGB - Y0) * W, W, sx1, sy1);
+                // (window rows are non-negative here: 24-bit multiplies)
+                Epx = gather_lds<NW, W>(sEx + (i1 + 1 - NGB - X0) + rowW<W>(j1 - NGB - Y0), sx0, sy0);
+                Epy = gather_lds<NW, W>(sEy + (i2 - NGB - X0) + rowW<W>(j2 + 1 - NGB - Y0), sx1, sy1);
                 if constexpr (UNIB) Bpz = a.bz0;
-                else Bpz = gather<NW>(sBz + (i1 + 1 - NGB - X0) + (j2 + 1 - NGB - Y0) * W, W, sx0, sy1);
+                else Bpz = gather_lds<NW, W>(sBz + (i1 + 1 - NGB - X0) + rowW<W>(j2 + 1 - NGB - Y0), sx0, sy1);
             } else {
                 Epx = gather<NW>(a.Ex + (i1 + 1 - NGB) + (int64_t)(j1 - NGB) * pex, pex, sx0, sy0);
                 Epy = gather<NW>(a.Ey + (i2 - NGB) + (int64_t)(j2 + 1 - NGB) * pey, pey, sx1, sy1);
@@ -353,7 +380,7 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
         }
         if constexpr (PUSH) {
             const float vv = vx * vx + vy * vy + vz * vz;
-            ke += a.ke_coef * (double)vv;
+            ke += (double)vv;
         }
         if constexpr (DEP >= 1) {
             // charge (Particle.mm:357-361): node (i2+i-ngb, j1+j-ngb), weights
@@ -384,7 +411,7 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
             }
             if constexpr (PUSH) drift += inw ? 0 : 1;
             if (inw) {
-                unsigned long long *r = sRho + wi + wj * W;
+                unsigned long long *r = sRho + wi + rowW<W>(wj);
 #pragma unroll
                 for (int j = 0; j < NW; j++)
 #pragma unroll
@@ -410,8 +437,8 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
                 unsigned long long *jxb, *jyb;
                 int pjx, pjy;
                 if (inw) {
-                    jxb = sJx + (i1 + 1 - NGB - X0) + (j1 - NGB - Y0) * W; pjx = W;
-                    jyb = sJy + (i2 - NGB - X0) + (j2 + 1 - NGB - Y0) * W; pjy = W;
+                    jxb = sJx + (i1 + 1 - NGB - X0) + rowW<W>(j1 - NGB - Y0); pjx = W;
+                    jyb = sJy + (i2 - NGB - X0) + rowW<W>(j2 + 1 - NGB - Y0); pjy = W;
                 } else {
                     jxb = a.jx + (i1 + 1 - NGB) + (int64_t)(j1 - NGB) * pex; pjx = pex;
                     jyb = a.jy + (i2 - NGB) + (int64_t)(j2 + 1 - NGB) * pey; pjy = pey;
@@ -430,6 +457,7 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
 
     if constexpr (PUSH) {
         // deterministic workgroup sum of the kinetic energy
+        ke *= a.ke_coef;
         for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off, 64);
         if ((tid & 63) == 0) sKE[tid >> 6] = ke;
         for (int off = 32; off > 0; off >>= 1) drift += __shfl_down(drift, off, 64);
