@@ -1288,11 +1288,12 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
     const int ge = std::min(1024, gp_cdiv(n, 256));
     const double cx = s.lv[0].hx2i, cy = s.lv[0].hy2i;
     auto t0 = std::chrono::steady_clock::now();
-    // a singular (all-Neumann/periodic) system: b and every iterate are kept
-    // at zero mean.  The FP32 corrections carry a constant component that S
-    // does not see; left in x it would grow over the refinements until its
-    // FP64 rounding (ulp of the constant times the stencil) sets a floor on
-    // the residual
+    // a singular (all-Neumann/periodic) system: b is projected to zero mean,
+    // and so is the solution.  The FP32 corrections carry a constant
+    // component that S does not see; left in x over many refinements it would
+    // grow until its FP64 rounding (ulp of the constant times the stencil)
+    // sets a floor on the residual, so from the third refinement on every
+    // iterate is projected too
     auto sub_mean = [&](double *a) {
         k_sum64<<<nb, RB, 0, st>>>(a, n, s.red);
         k_final<<<1, RB, 0, st>>>(s.red, nb, s.scal, S_MEAN, 0);
@@ -1300,7 +1301,6 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
     };
     if (singular) {
         sub_mean(s.b64);
-        sub_mean(s.x64);
         GP_HIP(hipGetLastError());
     }
     int total_it = 0, outer = 0, nn_it = 0, fell_back = 0, pcg_round = 0;
@@ -1329,7 +1329,7 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
             int rc = hk->warm(hk->ctx, s.e, s.r);
             if (rc) return rc;
             k_axpy64<<<ge, 256, 0, st>>>(s.x64, s.e, n, 1.0 / sc);
-            if (singular) sub_mean(s.x64);
+            if (singular && outer >= 2) sub_mean(s.x64);
             GP_HIP(hipGetLastError());
             warm_pending = false;
             outer++;
@@ -1352,9 +1352,13 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
         }
         total_it += it;
         k_axpy64<<<ge, 256, 0, st>>>(s.x64, s.e, n, 1.0 / sc);
-        if (singular) sub_mean(s.x64);
+        if (singular && outer >= 2) sub_mean(s.x64);
         GP_HIP(hipGetLastError());
         outer++;
+    }
+    if (singular) {
+        sub_mean(s.x64);
+        GP_HIP(hipGetLastError());
     }
     GP_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();

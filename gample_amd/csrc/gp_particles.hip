@@ -520,6 +520,10 @@ __global__ __launch_bounds__(GP_BLOCK) void k_stage_reduce(const unsigned long l
     if (imgy && comp == 0) {
         if (gy >= ngb && gy < ngb + ngy) { py[0] = gy - ngy; py[1] = gy; py[2] = gy + ngy; npy = 3; }
     } else { py[0] = gy; npy = 1; }
+    // at most MT windows cover a node along an axis; the loads of all of
+    // them are issued together (a window of an empty tile holds stale data
+    // and is read, then masked out)
+    constexpr int MT = (W + GP_TILE - 1) / GP_TILE;
     unsigned long long sum = 0;
     for (int b = 0; b < npy; b++) {
         const int y = py[b];
@@ -527,13 +531,24 @@ __global__ __launch_bounds__(GP_BLOCK) void k_stage_reduce(const unsigned long l
         for (int a = 0; a < npx; a++) {
             const int x = px[a];
             const int txl = max(fdiv16(x - c0 - W) + 1, 0), txh = min(fdiv16(x - c0), ntx - 1);
-            for (int ty = tyl; ty <= tyh; ty++)
-                for (int tx = txl; tx <= txh; tx++) {
-                    const int t = ty * ntx + tx;
-                    if (!on[t]) continue;
-                    const int lx = x - c0 - GP_TILE * tx, ly = y - c0 - GP_TILE * ty;
-                    sum += stage[((int64_t)t * ncomp + comp) * (W * W) + lx + ly * W];
+            unsigned long long v[MT * MT];
+            bool use[MT * MT];
+#pragma unroll
+            for (int qy = 0; qy < MT; qy++)
+#pragma unroll
+                for (int qx = 0; qx < MT; qx++) {
+                    const int ty = tyl + qy, tx = txl + qx, q = qx + qy * MT;
+                    use[q] = ty <= tyh && tx <= txh;
+                    v[q] = 0;
+                    if (use[q]) {
+                        const int t = ty * ntx + tx;
+                        const int lx = x - c0 - GP_TILE * tx, ly = y - c0 - GP_TILE * ty;
+                        use[q] = on[t] != 0;
+                        v[q] = stage[((int64_t)t * ncomp + comp) * (W * W) + lx + ly * W];
+                    }
                 }
+#pragma unroll
+            for (int q = 0; q < MT * MT; q++) sum += use[q] ? v[q] : 0ull;
         }
     }
     if (sum) acc[gx + (int64_t)gy * ext_x] += sum;
