@@ -208,3 +208,46 @@ def test_periodic_wrap_is_exact_fmod(orc):
     exp = np.where(exp >= ngx, np.nextafter(np.float32(ngx), np.float32(0)), exp)
     np.testing.assert_array_equal(out[0], exp)
     assert np.all((out[0] >= 0) & (out[0] < ngx))
+
+
+@pytest.mark.parametrize("kind", ["fast", "ref"])
+def test_boris_exb_drift(orc, kind):
+    """Uniform Ex and Bz: a particle moving at the E x B drift
+    v_d = -(constE / constB) Ex / Bz y (c = constE / constB in the push's
+    constants) keeps that velocity step after step, and a gyrating particle's
+    velocity averages to it over whole gyro-periods (the Boris scheme keeps
+    the drift exactly; a wrong sign or factor in E or in the rotation moves it)."""
+    ngx = ngy = 8
+    g = orc.grid(ngx, ngy, 1, pbc=("periodic",) * 4)
+    nx = ngx
+    E0, B0, cE, cB = 1.0, 1.0, 0.1, 0.2
+    Ex = np.full((nx + 1, nx + 2), E0, np.float32)
+    Ey = np.zeros((nx + 2, nx + 1), np.float32)
+    Bz = np.full((nx + 2, nx + 2), B0, np.float32)
+    vd = -(cE / cB) * E0 / B0
+    rng = np.random.default_rng(17)
+    n = 400
+    p = _particles(n, ngx, ngy, 0, seed=9)
+    p[2] = np.zeros(n, np.float32)
+    p[3] = np.full(n, vd, np.float32)
+    q = _particles(n, ngx, ngy, 0, seed=10)
+    ang = rng.uniform(0, 2 * np.pi, n)
+    q[2] = (0.3 * np.cos(ang)).astype(np.float32)
+    q[3] = (vd + 0.3 * np.sin(ang)).astype(np.float32)
+    consts = (cE, cB, 0.01, 0.01)
+    period = 2 * np.pi / (2 * math.atan(cB * B0))        # steps per gyration
+    steps = int(round(19 * period))                       # close to 19 whole periods
+    mean_vx, mean_vy = np.zeros(n), np.zeros(n)
+    for _ in range(steps):
+        if kind == "fast":
+            p = list(orc.push_fast(g, Ex, Ey, Bz, consts, p, uniform_bz=True, bz0=B0))
+            q = list(orc.push_fast(g, Ex, Ey, Bz, consts, q, uniform_bz=True, bz0=B0))
+        else:
+            p = list(orc.push_ref(g, Ex, Ey, Bz, consts, p)[:5])
+            q = list(orc.push_ref(g, Ex, Ey, Bz, consts, q)[:5])
+        mean_vx += q[2]
+        mean_vy += q[3]
+    np.testing.assert_allclose(p[2], 0.0, atol=1e-4)
+    np.testing.assert_allclose(p[3], vd, atol=1e-4)
+    assert abs(mean_vx.mean() / steps) < 5e-3
+    assert abs(mean_vy.mean() / steps - vd) < 5e-3
