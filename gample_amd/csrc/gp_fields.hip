@@ -46,27 +46,33 @@ __global__ void k_final(const double *part, int nb, double *out, int slot, int)
 }
 // ---------------- charge / current / rhs / reconstruction ----------------
 struct AccSet {
-    const int64_t *a[GP_MAX_SPECIES];
+    int64_t *a[GP_MAX_SPECIES];
     double f[GP_MAX_SPECIES];
     int ns;
 };
+// out = sum over species of acc * f, and the accumulators cleared for the
+// next deposit
 __global__ void k_scale_sum(AccSet s, double *out, int64_t n)
 {
     const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (k >= n) return;
     double v = 0.0;
-    for (int q = 0; q < s.ns; q++) v += (double)s.a[q][k] * s.f[q];
+    for (int q = 0; q < s.ns; q++) {
+        v += (double)s.a[q][k] * s.f[q];
+        s.a[q][k] = 0;
+    }
     out[k] = v;
 }
 // fold (EMField.mm:749-850), one thread per row (x) or column (y); applied
 // to each species' int64 accumulator before the conversion to FP64, so the
 // folded charge is exact and independent of where a deposit landed
+// (blockIdx.y selects the species' accumulator)
 template <typename T>
-__global__ void k_fold_x(T *rho, int ngb, int ngx, int nx, int ny, int bmin, int bmax, int periodic)
+__global__ void k_fold_x(AccSet as, int ngb, int ngx, int nx, int ny, int bmin, int bmax, int periodic)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j > ny) return;
-    T *r = rho + (int64_t)j * (nx + 1);
+    T *r = (T *)as.a[blockIdx.y] + (int64_t)j * (nx + 1);
     if (!periodic) {
         if (bmin == GP_FBC_DIRICHLET) for (int i = 0; i <= ngb; i++) r[ngb + 1] += r[i];
         else if (bmin == GP_FBC_NEUMANN) for (int i = 0; i < ngb; i++) r[ngb] += r[i];
@@ -78,12 +84,12 @@ __global__ void k_fold_x(T *rho, int ngb, int ngx, int nx, int ny, int bmin, int
     }
 }
 template <typename T>
-__global__ void k_fold_y(T *rho, int ngb, int ngy, int nx, int bmin, int bmax, int periodic)
+__global__ void k_fold_y(AccSet as, int ngb, int ngy, int nx, int bmin, int bmax, int periodic)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i > nx) return;
     const int P = nx + 1;
-    T *r = rho + i;
+    T *r = (T *)as.a[blockIdx.y] + i;
     if (!periodic) {
         if (bmin == GP_FBC_DIRICHLET) for (int j = 0; j <= ngb; j++) r[(ngb + 1) * P] += r[j * P];
         else if (bmin == GP_FBC_NEUMANN) for (int j = 0; j < ngb; j++) r[ngb * P] += r[j * P];
@@ -435,27 +441,22 @@ int gp_finalize_grids(gp_ctx *c)
     // periodic x, periodic y
     const int *b = c->cfg.fbc;
     const int perx = b[0] == GP_FBC_PERIODIC, pery = b[2] == GP_FBC_PERIODIC;
-    for (auto &S : c->sp) {
-        long long *r = (long long *)S.rho_acc;
-        if (!perx) k_fold_x<long long><<<gp_cdiv(c->ny + 1, 64), 64, 0, st>>>(r, c->ngb, c->ngx, c->nx, c->ny, b[0], b[1], 0);
-        if (!pery) k_fold_y<long long><<<gp_cdiv(c->nx + 1, 64), 64, 0, st>>>(r, c->ngb, c->ngy, c->nx, b[2], b[3], 0);
-        if (perx) k_fold_x<long long><<<gp_cdiv(c->ny + 1, 64), 64, 0, st>>>(r, c->ngb, c->ngx, c->nx, c->ny, b[0], b[1], 1);
-        if (pery) k_fold_y<long long><<<gp_cdiv(c->nx + 1, 64), 64, 0, st>>>(r, c->ngb, c->ngy, c->nx, b[2], b[3], 1);
+    const int ns = (int)c->sp.size();
+    if (ns > 0) {   // every species in one launch per fold
+        const dim3 gx(gp_cdiv(c->ny + 1, 64), ns), gy(gp_cdiv(c->nx + 1, 64), ns);
+        if (!perx) k_fold_x<long long><<<gx, 64, 0, st>>>(a, c->ngb, c->ngx, c->nx, c->ny, b[0], b[1], 0);
+        if (!pery) k_fold_y<long long><<<gy, 64, 0, st>>>(a, c->ngb, c->ngy, c->nx, b[2], b[3], 0);
+        if (perx) k_fold_x<long long><<<gx, 64, 0, st>>>(a, c->ngb, c->ngx, c->nx, c->ny, b[0], b[1], 1);
+        if (pery) k_fold_y<long long><<<gy, 64, 0, st>>>(a, c->ngb, c->ngy, c->nx, b[2], b[3], 1);
         GP_HIP(hipGetLastError());
     }
+    // the sums also clear the accumulators for the next deposit
     k_scale_sum<<<gp_cdiv(nN, 256), 256, 0, st>>>(a, c->rho, nN);
     GP_HIP(hipGetLastError());
     if (c->cfg.cur_dens) {
         k_scale_sum<<<gp_cdiv(nEx, 256), 256, 0, st>>>(jxs, c->jx, nEx);
         k_scale_sum<<<gp_cdiv(nEy, 256), 256, 0, st>>>(jys, c->jy, nEy);
         GP_HIP(hipGetLastError());
-    }
-    for (auto &S : c->sp) {
-        GP_HIP(hipMemsetAsync(S.rho_acc, 0, nN * 8, st));
-        if (c->cfg.cur_dens) {
-            GP_HIP(hipMemsetAsync(S.jx_acc, 0, nEx * 8, st));
-            GP_HIP(hipMemsetAsync(S.jy_acc, 0, nEy * 8, st));
-        }
     }
     return GP_OK;
 }
