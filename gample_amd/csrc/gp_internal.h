@@ -95,6 +95,10 @@ struct gp_solver {
     double *red = nullptr;     // reduction partials
     double *scal = nullptr;    // device scalars
     float *dinv = nullptr;     // dense inverse of the coarsest level (column-major), or null
+    // fast diagonalisation of the coarsest level when it is too large for a
+    // dense inverse: per-axis eigenvectors V (row-major m x m) and the
+    // per-mode inverse eigenvalues 1 / (cx lx_i + cy ly_j) (0 for a null mode)
+    float *fd_vx = nullptr, *fd_vy = nullptr, *fd_inv = nullptr;
     double *hscal = nullptr;   // pinned host mirror
     int red_blocks = 0;
     int last_it = 4;           // inner iterations of the previous solve

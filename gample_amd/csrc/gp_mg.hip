@@ -51,7 +51,10 @@ constexpr int NSW = MG_SWEEPS;            // smoothing sweeps per side of the V-
 #endif
 constexpr int BY = MG_BY;                 // rows of the (32, BY) workgroup of the level kernels
 constexpr int NT = 32 * BY;
-constexpr int BOT_MAX = 4096;          // unknowns of the largest LDS level
+// unknowns of the largest LDS level: 8192 takes the coarsest levels that
+// an odd interval count stops (124 x 50 for the 500 x 200 Dirichlet/periodic
+// grids of the reference's thruster conditions) into the bottom kernel
+constexpr int BOT_MAX = 8192;
 constexpr int BOT_LEVELS = 12;
 constexpr int DENSE_MAX = 64;          // coarsest level solved by a dense inverse
 constexpr int RB = 256;                // workgroup of the reduction kernels
@@ -440,6 +443,9 @@ struct Bottom {
     int singular;
     int sweeps;            // symmetric sweep pairs on the coarsest level (no dense inverse)
     const float *dense;    // inverse of the coarsest operator, column-major (or null)
+    // fast diagonalisation of the coarsest level (or null): eigenvector
+    // matrices of the two 1D operators and the inverse eigenvalues
+    const float *fvx, *fvy, *finv;
 };
 
 __device__ __forceinline__ float gs_point(const float *u, const float *f, const Lev &L, int i, int j, float invd)
@@ -563,8 +569,9 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         const Lev L = fixed<TX, TY>(B.lv[B.nl - 1]);
         const int n = L.mx * L.my;
         float *u = lds + B.off[B.nl - 1], *f = u + n;
-        // (the dense inverse of a singular level has the projections folded in)
-        if (B.singular && !B.dense) bot_project(f, L, scratch);
+        // (the dense inverse of a singular level has the projections folded
+        // in; the fast diagonalisation drops the null mode)
+        if (B.singular && !B.dense && !B.fvx) bot_project(f, L, scratch);
         if (B.dense) {
             // u = M f with the precomputed inverse (n <= 256): thread t sums
             // quarter t/256 of row t%256, the four quarters add in LDS
@@ -588,13 +595,48 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
             __syncthreads();
             if (tid < n) u[tid] = (scratch[tid] + scratch[tid + 256]) + (scratch[tid + 512] + scratch[tid + 768]);
             __syncthreads();
+        } else if (B.fvx) {
+            // exact solve by fast diagonalisation: with the weighted operator
+            // S = (Wx (x) Wy)(cx Kx (x) I + cy I (x) Ky) and K V = V Lambda,
+            // V' W V = I per axis,  u = (Vx (x) Vy) D^-1 (Vx' (x) Vy') f
+            // (res holds the intermediate transforms; a null mode has 1/0 -> 0)
+            const int mx = L.mx, my = L.my;
+            float *t = res;
+            for (int k = tid; k < n; k += 1024) {        // along x: t = Vx' f
+                const int ip = k % mx, j = k / mx;
+                float acc = 0.0f;
+                for (int i = 0; i < mx; i++) acc = __builtin_fmaf(B.fvx[i * mx + ip], f[i + j * mx], acc);
+                t[k] = acc;
+            }
+            __syncthreads();
+            for (int k = tid; k < n; k += 1024) {        // along y, scaled: u = D^-1 (t Vy)
+                const int ip = k % mx, jp = k / mx;
+                float acc = 0.0f;
+                for (int j = 0; j < my; j++) acc = __builtin_fmaf(B.fvy[j * my + jp], t[ip + j * mx], acc);
+                u[k] = acc * B.finv[k];
+            }
+            __syncthreads();
+            for (int k = tid; k < n; k += 1024) {        // back along y: t = u Vy'
+                const int ip = k % mx, j = k / mx;
+                float acc = 0.0f;
+                for (int jp = 0; jp < my; jp++) acc = __builtin_fmaf(B.fvy[j * my + jp], u[ip + jp * mx], acc);
+                t[k] = acc;
+            }
+            __syncthreads();
+            for (int k = tid; k < n; k += 1024) {        // back along x: u = Vx t
+                const int i = k % mx, j = k / mx;
+                float acc = 0.0f;
+                for (int ip = 0; ip < mx; ip++) acc = __builtin_fmaf(B.fvx[i * mx + ip], t[ip + j * mx], acc);
+                u[k] = acc;
+            }
+            __syncthreads();
         } else {
             bot_sweep_red0(u, f, L);
             bot_sweep(u, f, L, 1);
             for (int s = 1; s < B.sweeps; s++) { bot_sweep(u, f, L, 0); bot_sweep(u, f, L, 1); }
             for (int s = 0; s < B.sweeps; s++) { bot_sweep(u, f, L, 1); bot_sweep(u, f, L, 0); }
         }
-        if (B.singular && !B.dense) bot_project(u, L, scratch);
+        if (B.singular && !B.dense && !B.fvx) bot_project(u, L, scratch);
     }
     for (int l = B.nl - 2; l >= 0; l--) {
         const Lev L = fixed<TX, TY>(B.lv[l]), C = fixed<TX, TY>(B.lv[l + 1]);
@@ -941,6 +983,7 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
         B.singular = s.singular;
         B.sweeps = std::max(2, std::max(Lc.mx, Lc.my) / 2);
         B.dense = s.dinv;
+        B.fvx = s.fd_vx; B.fvy = s.fd_vy; B.finv = s.fd_inv;
         const bool top = lb == 0;
         double *bp = top ? part : nullptr;
         const Gate bg = take();
@@ -1124,6 +1167,80 @@ int gp_mg_scale(hipStream_t st, gp_solver &s, float *e, const float *r)
     return GP_OK;
 }
 
+// largest coarsest-level extent solved by fast diagonalisation
+constexpr int FD_MAX = 256;
+
+// eigenvectors of the symmetric matrix a (n x n, row-major; destroyed) by
+// cyclic Jacobi rotations: a = q diag(lam) q', q row-major with the
+// eigenvectors as columns
+static void jacobi_eig(std::vector<double> &a, int n, std::vector<double> &q, std::vector<double> &lam)
+{
+    q.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) q[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < n; j++) {
+                const double v = a[(size_t)i * n + j] * a[(size_t)i * n + j];
+                tot += v;
+                if (i != j) off += v;
+            }
+        if (off <= 1e-30 * tot) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int r = p + 1; r < n; r++) {
+                const double apr = a[(size_t)p * n + r];
+                if (std::fabs(apr) < 1e-300) continue;
+                const double app = a[(size_t)p * n + p], arr = a[(size_t)r * n + r];
+                const double th = 0.5 * (arr - app) / apr;
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int k = 0; k < n; k++) {   // columns p, r
+                    const double akp = a[(size_t)k * n + p], akr = a[(size_t)k * n + r];
+                    a[(size_t)k * n + p] = c * akp - sn * akr;
+                    a[(size_t)k * n + r] = sn * akp + c * akr;
+                }
+                for (int k = 0; k < n; k++) {   // rows p, r
+                    const double apk = a[(size_t)p * n + k], ark = a[(size_t)r * n + k];
+                    a[(size_t)p * n + k] = c * apk - sn * ark;
+                    a[(size_t)r * n + k] = sn * apk + c * ark;
+                }
+                for (int k = 0; k < n; k++) {
+                    const double qkp = q[(size_t)k * n + p], qkr = q[(size_t)k * n + r];
+                    q[(size_t)k * n + p] = c * qkp - sn * qkr;
+                    q[(size_t)k * n + r] = sn * qkp + c * qkr;
+                }
+            }
+    }
+    lam.resize(n);
+    for (int i = 0; i < n; i++) lam[i] = a[(size_t)i * n + i];
+}
+
+// the 1D operator of an axis of m unknowns, K u_i = 2 u_i - u_(i-1) - u_(i+1)
+// (zero beyond a Dirichlet wall, mirrored across a Neumann boundary, wrapped
+// on a periodic axis), with row weights w (1/2 on Neumann boundary rows):
+// W K is symmetric, and K V = V diag(lam) with V' W V = I, V row-major
+static void axis_eig(int t, int m, std::vector<double> &v, std::vector<double> &lam)
+{
+    std::vector<double> k((size_t)m * m, 0.0), w(m, 1.0);
+    for (int i = 0; i < m; i++) {
+        k[(size_t)i * m + i] += 2.0;
+        int l = i - 1, r = i + 1;
+        if (t == GP_AX_PER) { l = (l + m) % m; r = r % m; }
+        if (t == GP_AX_NEU) { if (l < 0) l = 1; if (r >= m) r = m - 2; }
+        if (l >= 0 && l < m) k[(size_t)i * m + l] -= 1.0;
+        if (r >= 0 && r < m) k[(size_t)i * m + r] -= 1.0;
+        if (t == GP_AX_NEU && (i == 0 || i == m - 1)) w[i] = 0.5;
+    }
+    // symmetric form W^(1/2) K W^(-1/2) = W^(-1/2) (W K) W^(-1/2)
+    std::vector<double> a((size_t)m * m), q;
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) a[(size_t)i * m + j] = std::sqrt(w[i] / w[j]) * k[(size_t)i * m + j];
+    jacobi_eig(a, m, q, lam);
+    v.resize((size_t)m * m);
+    for (int i = 0; i < m; i++)
+        for (int j = 0; j < m; j++) v[(size_t)i * m + j] = q[(size_t)i * m + j] / std::sqrt(w[i]);
+}
+
 int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double cy)
 {
     s.lv.clear();
@@ -1152,6 +1269,16 @@ int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double 
     for (int k = 0; k < nl; k++)
         if ((int64_t)s.lv[k].mx * s.lv[k].my <= BOT_MAX) { s.bottom = k; break; }
     if (s.bottom < nl && nl - s.bottom > BOT_LEVELS) return gp_fail(GP_ERR_UNSUPPORTED, "too many multigrid levels");
+    if (s.bottom < nl) {
+        // the bottom kernel's dynamic LDS (u and f of every level it holds)
+        size_t bytes = 0;
+        for (int k = s.bottom; k < nl; k++) bytes += 2 * sizeof(float) * (size_t)s.lv[k].mx * s.lv[k].my;
+        if (bytes > 64 * 1024) {
+#define MG_ATTR(X, Y) GP_HIP(hipFuncSetAttribute((const void *)k_mg_bottom<X, Y>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))
+            MG_AXES(tx, ty, MG_ATTR);
+#undef MG_ATTR
+        }
+    }
     const int n = mx * my;
     s.n = n;
     if (!mklev(s.lv[0]).rb)
@@ -1256,6 +1383,31 @@ int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double 
             GP_HIP(hipMemcpy(s.dinv, cm.data(), (size_t)m * m * sizeof(float), hipMemcpyHostToDevice));
         }
     }
+    // a coarsest level in the bottom but too large for the dense inverse (an
+    // odd interval count stopped the coarsening) is solved exactly by fast
+    // diagonalisation instead of Gauss-Seidel sweeps
+    {
+        const gp_mg_level &c = s.lv[nl - 1];
+        if (s.bottom < nl && !s.dinv && c.mx <= FD_MAX && c.my <= FD_MAX) {
+            std::vector<double> vx, lx, vy, ly;
+            axis_eig(tx, c.mx, vx, lx);
+            axis_eig(ty, c.my, vy, ly);
+            const double cxl = c.hx2i, cyl = c.hy2i;
+            std::vector<float> fvx(vx.begin(), vx.end()), fvy(vy.begin(), vy.end());
+            std::vector<float> finv((size_t)c.mx * c.my);
+            for (int j = 0; j < c.my; j++)
+                for (int i = 0; i < c.mx; i++) {
+                    const double ev = cxl * lx[i] + cyl * ly[j];
+                    finv[i + (size_t)j * c.mx] = std::fabs(ev) > 1e-9 * (cxl + cyl) ? (float)(1.0 / ev) : 0.0f;
+                }
+            GP_HIP(hipMalloc((void **)&s.fd_vx, fvx.size() * sizeof(float)));
+            GP_HIP(hipMalloc((void **)&s.fd_vy, fvy.size() * sizeof(float)));
+            GP_HIP(hipMalloc((void **)&s.fd_inv, finv.size() * sizeof(float)));
+            GP_HIP(hipMemcpy(s.fd_vx, fvx.data(), fvx.size() * sizeof(float), hipMemcpyHostToDevice));
+            GP_HIP(hipMemcpy(s.fd_vy, fvy.data(), fvy.size() * sizeof(float), hipMemcpyHostToDevice));
+            GP_HIP(hipMemcpy(s.fd_inv, finv.data(), finv.size() * sizeof(float), hipMemcpyHostToDevice));
+        }
+    }
 
     GP_HIP(hipHostMalloc((void **)&s.hscal, S_NS * sizeof(double), hipHostMallocDefault));
     return GP_OK;
@@ -1268,7 +1420,8 @@ void gp_mg_free(gp_solver &s)
         (void)hipFree(s.lv[k].r);
     }
     s.lv.clear();
-    void *ps[] = {s.b64, s.x64, s.r64, s.rhs_bc, s.r, s.z, s.p, s.q, s.q2, s.e, s.red, s.scal, s.dinv};
+    void *ps[] = {s.b64, s.x64, s.r64, s.rhs_bc, s.r, s.z, s.p, s.q, s.q2, s.e, s.red, s.scal, s.dinv,
+                  s.fd_vx, s.fd_vy, s.fd_inv};
     for (void *p : ps) if (p) (void)hipFree(p);
     if (s.hscal) (void)hipHostFree(s.hscal);
     s = gp_solver();
