@@ -51,6 +51,23 @@ __device__ __forceinline__ bf16s f2b(float f)
     return (bf16s)(u >> 16);
 }
 __device__ __forceinline__ float b2f(bf16s h) { return __uint_as_float((uint32_t)h << 16); }
+// the 8 BF16 values of a 16-byte word, in memory (channel) order
+__device__ __forceinline__ void unpack8(const uint4 &v, float *f)
+{
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        f[2 * k] = __uint_as_float(w[k] << 16);
+        f[2 * k + 1] = __uint_as_float(w[k] & 0xFFFF0000u);
+    }
+}
+__device__ __forceinline__ uint4 pack8(const float *f)
+{
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) w[k] = (uint32_t)f2b(f[2 * k]) | ((uint32_t)f2b(f[2 * k + 1]) << 16);
+    return make_uint4(w[0], w[1], w[2], w[3]);
+}
 
 struct Geo {
     int B, H, W;      // batch, rows, columns
@@ -221,14 +238,32 @@ __global__ void k_dw_reduce(const float *part, int nchunk, int CO, int CI, float
     for (int k = 0; k < nchunk; k++) s += part[(int64_t)k * n + e];
     grad[((int64_t)co * 9 + t) * CI + ci] = s;
 }
-// bias gradient partials: part[blk][c] = sum over the block's pixels of d[p][c]
-__global__ void k_colsum(const bf16s *d, int64_t npix, int C, int64_t chunk, float *part)
+// bias gradient partials: part[blk][c] = sum over the block's pixels of d[p][c].
+// Each thread reads 8 channels (16 bytes) of every (256 / (C / 8))-th pixel;
+// the pixel lanes are added in lane order.
+__global__ __launch_bounds__(NB) void k_colsum(const bf16s *d, int64_t npix, int C, int64_t chunk, float *part)
 {
-    const int c = threadIdx.x;   // blockDim = C
+    __shared__ float sh[NB * 8];
+    const int C8 = C / 8, PL = NB / C8;
+    const int cg = threadIdx.x % C8, pl = threadIdx.x / C8;
     const int64_t p0 = blockIdx.x * chunk, p1 = std::min(npix, p0 + chunk);
-    float s = 0.0f;
-    for (int64_t p = p0; p < p1; p++) s += b2f(d[p * C + c]);
-    part[(int64_t)blockIdx.x * C + c] = s;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (pl < PL)
+        for (int64_t p = p0 + pl; p < p1; p += PL) {
+            float v[8];
+            unpack8(*(const uint4 *)(d + p * C + 8 * cg), v);
+#pragma unroll
+            for (int k = 0; k < 8; k++) s[k] += v[k];
+        }
+    if (pl < PL)
+#pragma unroll
+        for (int k = 0; k < 8; k++) sh[pl * C + 8 * cg + k] = s[k];
+    __syncthreads();
+    for (int c = threadIdx.x; c < C; c += NB) {
+        float t = 0.0f;
+        for (int l = 0; l < PL; l++) t += sh[l * C + c];
+        part[(int64_t)blockIdx.x * C + c] = t;
+    }
 }
 __global__ void k_colsum_f(const float *d, int64_t npix, int64_t chunk, float *part)
 {
@@ -244,13 +279,45 @@ __global__ void k_colsum_f(const float *d, int64_t npix, int64_t chunk, float *p
     }
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
-__global__ void k_part_reduce(const float *part, int nblk, int n, float *out)
+// out[e] = sum over k < nblk of part[k][e]: a block takes 32 outputs with 8
+// lanes each; lane l sums k = l, l + 8, ... (four loads in flight, added in
+// k order) and the lanes add in lane order
+__global__ __launch_bounds__(NB) void k_part_reduce(const float *part, int nblk, int n, float *out)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
+    __shared__ float sh[8][33];
+    const int ex = threadIdx.x & 31, kl = threadIdx.x >> 5;
+    const int e = blockIdx.x * 32 + ex;
     float s = 0.0f;
-    for (int k = 0; k < nblk; k++) s += part[(int64_t)k * n + e];
-    out[e] = s;
+    if (e < n) {
+        int k = kl;
+        for (; k + 24 < nblk; k += 32) {
+            const float a0 = part[(int64_t)k * n + e], a1 = part[(int64_t)(k + 8) * n + e];
+            const float a2 = part[(int64_t)(k + 16) * n + e], a3 = part[(int64_t)(k + 24) * n + e];
+            s += a0; s += a1; s += a2; s += a3;
+        }
+        for (; k < nblk; k += 8) s += part[(int64_t)k * n + e];
+    }
+    sh[kl][ex] = s;
+    __syncthreads();
+    if (kl == 0 && e < n) {
+        float t = 0.0f;
+        for (int l = 0; l < 8; l++) t += sh[l][ex];
+        out[e] = t;
+    }
+}
+// out[0] = sum of part[0 .. nblk): strided thread sums, then a fixed tree
+__global__ __launch_bounds__(NB) void k_part_sum1(const float *part, int nblk, float *out)
+{
+    __shared__ float sh[NB];
+    float s = 0.0f;
+    for (int k = threadIdx.x; k < nblk; k += NB) s += part[k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = NB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -281,25 +348,6 @@ __global__ void k_conv_in(const float *r, const float *w, const float *bias, bf1
         const uint32_t pk = (uint32_t)f2b(fmaxf(s0, 0.0f)) | ((uint32_t)f2b(fmaxf(s1, 0.0f)) << 16);
         *(uint32_t *)(out + p * C0 + c) = pk;
     }
-}
-// weight-gradient partials of conv_in: part[blk][c][t] = sum_p d[p][c] r[p + off_t]
-template <int C0>
-__global__ void k_conv_in_dw(const bf16s *d, const float *r, Geo g, int64_t chunk, float *part)
-{
-    // blockDim = C0 * 9 / ... : one thread per (c, t)
-    const int e = threadIdx.x;
-    if (e >= C0 * 9) return;
-    const int c = e / 9, t = e % 9;
-    const int64_t np = (int64_t)g.B * g.H * g.W;
-    const int64_t p0 = blockIdx.x * chunk, p1 = std::min(np, p0 + chunk);
-    const int HW = g.H * g.W;
-    float s = 0.0f;
-    for (int64_t p = p0; p < p1; p++) {
-        const int b = (int)(p / HW), rem = (int)(p % HW);
-        const int sp = src_pix(g, b, rem / g.W + t / 3 - 1, rem % g.W + t % 3 - 1);
-        if (sp >= 0) s = __builtin_fmaf(b2f(d[p * C0 + c]), r[sp], s);
-    }
-    part[(int64_t)blockIdx.x * C0 * 9 + e] = s;
 }
 // y[p] = sum_t sum_c w[t][c] u[p + off_t][c] + b
 template <int C0>
@@ -342,80 +390,134 @@ __global__ void k_conv_out_dx(const float *dy, const float *w, bf16s *du, Geo g)
         du[p * C0 + c] = f2b(s);
     }
 }
-// weight-gradient partials of conv_out: part[blk][t][c] = sum_p dy[p] u[p + off_t][c]
+// Weight-gradient partials of the two one-channel convolutions at level 0:
+//   part[blk][idx(c, t)] = sum over the block's pixels q of A[q][c] v[src(q + sg off_t)]
+//   conv_in:  A = d (gradient of the pre-activation), v = r (input), sg = +1, idx = 9 c + t
+//   conv_out: A = u (input activation), v = dy,                     sg = -1, idx = C0 t + c
+// (sum_p dy[p] u[src(p + off)][c] = sum_q u[q][c] dy[src(q - off)]: both run
+// over the pixel pairs inside the image, or wrap on a periodic axis.)
+// Tiles of 8 x 32 pixels, taken by the blocks in a grid-stride loop: A of
+// the tile and the 10 x 34 window of v are staged in LDS, and thread (c, t)
+// accumulates its product over the tile's pixels in row order.
 template <int C0>
-__global__ void k_conv_out_dw(const float *dy, const bf16s *u, Geo g, int64_t chunk, float *part)
+__global__ __launch_bounds__(9 * C0) void k_dw1(const bf16s *A, const float *v, Geo g, int sg, int ct, float *part)
 {
-    const int e = threadIdx.x;   // t * C0 + c
-    if (e >= 9 * C0) return;
-    const int t = e / C0, c = e % C0;
-    const int64_t np = (int64_t)g.B * g.H * g.W;
-    const int64_t p0 = blockIdx.x * chunk, p1 = std::min(np, p0 + chunk);
-    const int HW = g.H * g.W;
+    constexpr int TH = 8, TW = 32, NT1 = 9 * C0, VW = TW + 2;
+    __shared__ __attribute__((aligned(16))) bf16s sA[TH * TW * C0];
+    __shared__ float sV[(TH + 2) * VW];
+    const int tid = threadIdx.x, c = tid / 9, t = tid % 9;
+    const int dy = sg * (t / 3 - 1), dx = sg * (t % 3 - 1);
+    const int tyn = (g.H + TH - 1) / TH, txn = (g.W + TW - 1) / TW;
+    const int64_t ntile = (int64_t)g.B * tyn * txn;
     float s = 0.0f;
-    for (int64_t p = p0; p < p1; p++) {
-        const int b = (int)(p / HW), rem = (int)(p % HW);
-        const int sp = src_pix(g, b, rem / g.W + t / 3 - 1, rem % g.W + t % 3 - 1);
-        if (sp >= 0) s = __builtin_fmaf(dy[p], b2f(u[(int64_t)sp * C0 + c]), s);
+    for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const int b = (int)(tile / (tyn * txn)), rem = (int)(tile % (tyn * txn));
+        const int ty0 = (rem / txn) * TH, tx0 = (rem % txn) * TW;
+        __syncthreads();   // the previous tile is consumed
+        for (int e = tid; e < TH * TW * (C0 / 8); e += NT1) {
+            const int pix = e / (C0 / 8), c8 = e % (C0 / 8);
+            const int yy = ty0 + pix / TW, xx = tx0 + pix % TW;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (yy < g.H && xx < g.W) val = *(const uint4 *)(A + (((int64_t)b * g.H + yy) * g.W + xx) * C0 + 8 * c8);
+            *(uint4 *)(sA + pix * C0 + 8 * c8) = val;
+        }
+        for (int e = tid; e < (TH + 2) * VW; e += NT1) {
+            const int sp = src_pix(g, b, ty0 + e / VW - 1, tx0 + e % VW - 1);
+            sV[e] = sp >= 0 ? v[sp] : 0.0f;
+        }
+        __syncthreads();
+        const float *vrow = sV + (1 + dy) * VW + 1 + dx;
+        for (int yy = 0; yy < TH; yy++) {
+#pragma unroll 8
+            for (int xx = 0; xx < TW; xx++)
+                s = __builtin_fmaf(b2f(sA[(yy * TW + xx) * C0 + c]), vrow[yy * VW + xx], s);
+        }
     }
-    part[(int64_t)blockIdx.x * 9 * C0 + e] = s;
+    part[(int64_t)blockIdx.x * NT1 + (ct ? t * C0 + c : 9 * c + t)] = s;
 }
 
 // ---------------------------------------------------------------------------
 // pooling, upsampling and the fused backward combinations (NHWC bf16)
 // ---------------------------------------------------------------------------
+// One thread per 8 channels (16 bytes) of a pixel in the four kernels below;
+// every channel is combined exactly as in a scalar loop.
 // out (H/2 x W/2) = mean of 2x2 blocks of in (H x W); g = geometry of in
 __global__ void k_pool(const bf16s *in, bf16s *out, Geo g, int C)
 {
     const int64_t e = blockIdx.x * (int64_t)NB + threadIdx.x;
-    const int H2 = g.H / 2, W2 = g.W / 2;
-    const int64_t n = (int64_t)g.B * H2 * W2 * C;
+    const int C8 = C / 8, H2 = g.H / 2, W2 = g.W / 2;
+    const int64_t n = (int64_t)g.B * H2 * W2 * C8;
     if (e >= n) return;
-    const int c = (int)(e % C);
-    const int64_t q = e / C;
+    const int c8 = (int)(e % C8);
+    const int64_t q = e / C8;
     const int x = (int)(q % W2), y = (int)((q / W2) % H2), b = (int)(q / ((int64_t)W2 * H2));
-    const int64_t s = (((int64_t)b * g.H + 2 * y) * g.W + 2 * x) * C + c;
-    const float v = (b2f(in[s]) + b2f(in[s + C])) + (b2f(in[s + (int64_t)g.W * C]) + b2f(in[s + (int64_t)g.W * C + C]));
-    out[e] = f2b(0.25f * v);
+    const int64_t s = (((int64_t)b * g.H + 2 * y) * g.W + 2 * x) * C + 8 * c8, row = (int64_t)g.W * C;
+    float v0[8], v1[8], v2[8], v3[8], o[8];
+    unpack8(*(const uint4 *)(in + s), v0);
+    unpack8(*(const uint4 *)(in + s + C), v1);
+    unpack8(*(const uint4 *)(in + s + row), v2);
+    unpack8(*(const uint4 *)(in + s + row + C), v3);
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = 0.25f * ((v0[k] + v1[k]) + (v2[k] + v3[k]));
+    *(uint4 *)(out + 8 * e) = pack8(o);
 }
 // out (H x W) = up(gin (H/2 x W/2)) + skip; g = geometry of out
 __global__ void k_up_add(const bf16s *gin, const bf16s *skip, bf16s *out, Geo g, int C)
 {
     const int64_t e = blockIdx.x * (int64_t)NB + threadIdx.x;
-    const int64_t n = (int64_t)g.B * g.H * g.W * C;
+    const int C8 = C / 8;
+    const int64_t n = (int64_t)g.B * g.H * g.W * C8;
     if (e >= n) return;
-    const int c = (int)(e % C);
-    const int64_t q = e / C;
+    const int c8 = (int)(e % C8);
+    const int64_t q = e / C8;
     const int x = (int)(q % g.W), y = (int)((q / g.W) % g.H), b = (int)(q / ((int64_t)g.W * g.H));
-    const int64_t s = (((int64_t)b * (g.H / 2) + y / 2) * (g.W / 2) + x / 2) * C + c;
-    out[e] = f2b(b2f(gin[s]) + b2f(skip[e]));
+    const int64_t s = (((int64_t)b * (g.H / 2) + y / 2) * (g.W / 2) + x / 2) * C + 8 * c8;
+    float vg[8], vs[8], o[8];
+    unpack8(*(const uint4 *)(gin + s), vg);
+    unpack8(*(const uint4 *)(skip + 8 * e), vs);
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = vg[k] + vs[k];
+    *(uint4 *)(out + 8 * e) = pack8(o);
 }
 // decoder backward: dpre (H/2) = (sum of the 2x2 block of du (H)) * [act > 0]; g = geometry of du
 __global__ void k_sum2x2_mask(const bf16s *du, const bf16s *act, bf16s *dpre, Geo g, int C)
 {
     const int64_t e = blockIdx.x * (int64_t)NB + threadIdx.x;
-    const int H2 = g.H / 2, W2 = g.W / 2;
-    const int64_t n = (int64_t)g.B * H2 * W2 * C;
+    const int C8 = C / 8, H2 = g.H / 2, W2 = g.W / 2;
+    const int64_t n = (int64_t)g.B * H2 * W2 * C8;
     if (e >= n) return;
-    const int c = (int)(e % C);
-    const int64_t q = e / C;
+    const int c8 = (int)(e % C8);
+    const int64_t q = e / C8;
     const int x = (int)(q % W2), y = (int)((q / W2) % H2), b = (int)(q / ((int64_t)W2 * H2));
-    const int64_t s = (((int64_t)b * g.H + 2 * y) * g.W + 2 * x) * C + c;
-    const float v = (b2f(du[s]) + b2f(du[s + C])) + (b2f(du[s + (int64_t)g.W * C]) + b2f(du[s + (int64_t)g.W * C + C]));
-    dpre[e] = f2b(b2f(act[e]) > 0.0f ? v : 0.0f);
+    const int64_t s = (((int64_t)b * g.H + 2 * y) * g.W + 2 * x) * C + 8 * c8, row = (int64_t)g.W * C;
+    float v0[8], v1[8], v2[8], v3[8], va[8], o[8];
+    unpack8(*(const uint4 *)(du + s), v0);
+    unpack8(*(const uint4 *)(du + s + C), v1);
+    unpack8(*(const uint4 *)(du + s + row), v2);
+    unpack8(*(const uint4 *)(du + s + row + C), v3);
+    unpack8(*(const uint4 *)(act + 8 * e), va);
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = va[k] > 0.0f ? (v0[k] + v1[k]) + (v2[k] + v3[k]) : 0.0f;
+    *(uint4 *)(dpre + 8 * e) = pack8(o);
 }
 // encoder backward: dpre (H) = (dskip + 0.25 up(dpool (H/2))) * [act > 0]; g = geometry of dskip
 __global__ void k_skip_pool_mask(const bf16s *dskip, const bf16s *dpool, const bf16s *act, bf16s *dpre, Geo g, int C)
 {
     const int64_t e = blockIdx.x * (int64_t)NB + threadIdx.x;
-    const int64_t n = (int64_t)g.B * g.H * g.W * C;
+    const int C8 = C / 8;
+    const int64_t n = (int64_t)g.B * g.H * g.W * C8;
     if (e >= n) return;
-    const int c = (int)(e % C);
-    const int64_t q = e / C;
+    const int c8 = (int)(e % C8);
+    const int64_t q = e / C8;
     const int x = (int)(q % g.W), y = (int)((q / g.W) % g.H), b = (int)(q / ((int64_t)g.W * g.H));
-    const int64_t s = (((int64_t)b * (g.H / 2) + y / 2) * (g.W / 2) + x / 2) * C + c;
-    const float v = b2f(dskip[e]) + 0.25f * b2f(dpool[s]);
-    dpre[e] = f2b(b2f(act[e]) > 0.0f ? v : 0.0f);
+    const int64_t s = (((int64_t)b * (g.H / 2) + y / 2) * (g.W / 2) + x / 2) * C + 8 * c8;
+    float vs[8], vp[8], va[8], o[8];
+    unpack8(*(const uint4 *)(dskip + 8 * e), vs);
+    unpack8(*(const uint4 *)(dpool + s), vp);
+    unpack8(*(const uint4 *)(act + 8 * e), va);
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = va[k] > 0.0f ? vs[k] + 0.25f * vp[k] : 0.0f;
+    *(uint4 *)(dpre + 8 * e) = pack8(o);
 }
 
 // loss partials: dy = 2 (y - t) / n_total; part[blk] = sum (y - t)^2
@@ -738,14 +840,50 @@ int dw(hipStream_t st, int CI, int CO, const bf16s *d, const bf16s *x, float *pa
 #undef GP_DW_CASE
     return gp_fail(GP_ERR_UNSUPPORTED, "weight gradient shape " + std::to_string(CI) + " -> " + std::to_string(CO) + " not instantiated");
 }
+// out[0 .. n) = column sums of the nblk x n partials (fixed order)
+void part_reduce(hipStream_t st, const float *part, int nblk, int n, float *out)
+{
+    if (n == 1) k_part_sum1<<<1, NB, 0, st>>>(part, nblk, out);
+    else k_part_reduce<<<gp_cdiv(n, 32), NB, 0, st>>>(part, nblk, n, out);
+}
 int bias_grad(hipStream_t st, const bf16s *d, int64_t npix, int C, float *part, float *out)
 {
+    if (C % 8 || NB % (C / 8)) return gp_fail(GP_ERR_UNSUPPORTED, "bias gradient of " + std::to_string(C) + " channels");
     const int nblk = (int)std::min<int64_t>(1024, (npix + 255) / 256);
     const int64_t chunk = (npix + nblk - 1) / nblk;
-    k_colsum<<<nblk, C, 0, st>>>(d, npix, C, chunk, part);
-    k_part_reduce<<<gp_cdiv(C, NB), NB, 0, st>>>(part, nblk, C, out);
+    k_colsum<<<nblk, NB, 0, st>>>(d, npix, C, chunk, part);
+    part_reduce(st, part, nblk, C, out);
     GP_HIP(hipGetLastError());
     return GP_OK;
+}
+// weight gradient of a one-channel convolution at level 0 (k_dw1) into grad
+int dw1(hipStream_t st, const gp_nn *n, const bf16s *A, const float *v, const Geo &g, int sg, int ct, float *grad)
+{
+    const int64_t ntile = (int64_t)g.B * ((g.H + 7) / 8) * ((g.W + 31) / 32);
+    const int nblk = (int)std::min<int64_t>(2048, ntile);
+    if ((int64_t)nblk * 9 * 32 > n->part_n) return gp_fail(GP_ERR_STATE, "weight-gradient scratch too small");
+    k_dw1<32><<<nblk, 9 * 32, 0, st>>>(A, v, g, sg, ct, n->part);
+    part_reduce(st, n->part, nblk, 9 * 32, grad);
+    GP_HIP(hipGetLastError());
+    return GP_OK;
+}
+// launch shapes of the elementwise kernels (one thread per 8 channels)
+void pool(hipStream_t st, const bf16s *in, bf16s *out, const Geo &g, int C)
+{
+    k_pool<<<g1((int64_t)g.B * (g.H / 2) * (g.W / 2) * (C / 8)), NB, 0, st>>>(in, out, g, C);
+}
+void up_add(hipStream_t st, const bf16s *gin, const bf16s *skip, bf16s *out, const Geo &g, int C)
+{
+    k_up_add<<<g1((int64_t)g.B * g.H * g.W * (C / 8)), NB, 0, st>>>(gin, skip, out, g, C);
+}
+void sum2x2_mask(hipStream_t st, const bf16s *du, const bf16s *act, bf16s *dpre, const Geo &g, int C)
+{
+    k_sum2x2_mask<<<g1((int64_t)g.B * (g.H / 2) * (g.W / 2) * (C / 8)), NB, 0, st>>>(du, act, dpre, g, C);
+}
+void skip_pool_mask(hipStream_t st, const bf16s *dskip, const bf16s *dpool, const bf16s *act, bf16s *dpre,
+                    const Geo &g, int C)
+{
+    k_skip_pool_mask<<<g1((int64_t)g.B * g.H * g.W * (C / 8)), NB, 0, st>>>(dskip, dpool, act, dpre, g, C);
 }
 
 void nn_layout(gp_nn *n)
@@ -786,18 +924,18 @@ int nn_forward(hipStream_t st, gp_nn *n, int B)
     auto WBl = [&](int l) { return n->WB + n->bfoff[l]; };
     int rc;
     k_conv_in<32><<<g1(P0), NB, 0, st>>>(n->x_in, P + n->woff[0], P + n->boff[0], n->a0, G0);
-    k_pool<<<g1(P0 / 4 * c0), NB, 0, st>>>(n->a0, n->p1, G0, c0);
+    pool(st, n->a0, n->p1, G0, c0);
     if ((rc = conv(st, c0, c1, n->p1, WBl(1), P + n->boff[1], nullptr, n->a1, G1, 1))) return rc;
-    k_pool<<<g1(P0 / 16 * c1), NB, 0, st>>>(n->a1, n->p2, G1, c1);
+    pool(st, n->a1, n->p2, G1, c1);
     if ((rc = conv(st, c1, c2, n->p2, WBl(2), P + n->boff[2], nullptr, n->a2, G2, 1))) return rc;
-    k_pool<<<g1(P0 / 64 * c2), NB, 0, st>>>(n->a2, n->p3, G2, c2);
+    pool(st, n->a2, n->p3, G2, c2);
     if ((rc = conv(st, c2, c2, n->p3, WBl(3), P + n->boff[3], nullptr, n->a3, G3, 1))) return rc;
     if ((rc = conv(st, c2, c2, n->a3, WBl(4), P + n->boff[4], nullptr, n->g2, G3, 1))) return rc;
-    k_up_add<<<g1(P0 / 16 * c2), NB, 0, st>>>(n->g2, n->a2, n->u2, G2, c2);
+    up_add(st, n->g2, n->a2, n->u2, G2, c2);
     if ((rc = conv(st, c2, c1, n->u2, WBl(5), P + n->boff[5], nullptr, n->g1, G2, 1))) return rc;
-    k_up_add<<<g1(P0 / 4 * c1), NB, 0, st>>>(n->g1, n->a1, n->u1, G1, c1);
+    up_add(st, n->g1, n->a1, n->u1, G1, c1);
     if ((rc = conv(st, c1, c0, n->u1, WBl(6), P + n->boff[6], nullptr, n->g0, G1, 1))) return rc;
-    k_up_add<<<g1(P0 * c0), NB, 0, st>>>(n->g0, n->a0, n->u0, G0, c0);
+    up_add(st, n->g0, n->a0, n->u0, G0, c0);
     k_conv_out<32><<<g1(P0), NB, 0, st>>>(n->u0, P + n->woff[7], P + n->boff[7], n->y_out, G0);
     GP_HIP(hipGetLastError());
     return GP_OK;
@@ -816,24 +954,23 @@ int nn_backward(hipStream_t st, gp_nn *n, int B)
     {
         const int nblk = (int)std::min<int64_t>(2048, (P0 + 255) / 256);
         const int64_t chunk = (P0 + nblk - 1) / nblk;
-        k_conv_out_dw<32><<<nblk, 9 * 32, 0, st>>>(n->dy, n->u0, G0, chunk, n->part);
-        k_part_reduce<<<gp_cdiv(9 * c0, NB), NB, 0, st>>>(n->part, nblk, 9 * c0, Gr + n->woff[7]);
+        if ((rc = dw1(st, n, n->u0, n->dy, G0, -1, 1, Gr + n->woff[7]))) return rc;
         k_colsum_f<<<nblk, NB, 0, st>>>(n->dy, P0, chunk, n->part);
-        k_part_reduce<<<1, NB, 0, st>>>(n->part, nblk, 1, Gr + n->boff[7]);
+        part_reduce(st, n->part, nblk, 1, Gr + n->boff[7]);
         k_conv_out_dx<32><<<g1(P0), NB, 0, st>>>(n->dy, n->P + n->woff[7], n->d0a, G0);   // d_u0
     }
     // u0 = up(g0) + a0 -> dpre6 at level 1
-    k_sum2x2_mask<<<g1(P1 * c0), NB, 0, st>>>(n->d0a, n->g0, n->d1a, G0, c0);
+    sum2x2_mask(st, n->d0a, n->g0, n->d1a, G0, c0);
     if ((rc = dw(st, c1, c0, n->d1a, n->u1, n->part, n->part_n, Gr + n->woff[6], G1))) return rc;
     if ((rc = bias_grad(st, n->d1a, P1, c0, n->part, Gr + n->boff[6]))) return rc;
     if ((rc = conv(st, c0, c1, n->d1a, WTl(6), nullptr, nullptr, n->d1b, G1, 0))) return rc;   // d_u1
     // u1 = up(g1) + a1 -> dpre5 at level 2
-    k_sum2x2_mask<<<g1(P2 * c1), NB, 0, st>>>(n->d1b, n->g1, n->d2a, G1, c1);
+    sum2x2_mask(st, n->d1b, n->g1, n->d2a, G1, c1);
     if ((rc = dw(st, c2, c1, n->d2a, n->u2, n->part, n->part_n, Gr + n->woff[5], G2))) return rc;
     if ((rc = bias_grad(st, n->d2a, P2, c1, n->part, Gr + n->boff[5]))) return rc;
     if ((rc = conv(st, c1, c2, n->d2a, WTl(5), nullptr, nullptr, n->d2b, G2, 0))) return rc;   // d_u2
     // u2 = up(g2) + a2 -> dpre4 at level 3
-    k_sum2x2_mask<<<g1(P3 * c2), NB, 0, st>>>(n->d2b, n->g2, n->d3a, G2, c2);
+    sum2x2_mask(st, n->d2b, n->g2, n->d3a, G2, c2);
     if ((rc = dw(st, c2, c2, n->d3a, n->a3, n->part, n->part_n, Gr + n->woff[4], G3))) return rc;
     if ((rc = bias_grad(st, n->d3a, P3, c2, n->part, Gr + n->boff[4]))) return rc;
     // d_a3 masked by a3 > 0 -> dpre3
@@ -842,23 +979,18 @@ int nn_backward(hipStream_t st, gp_nn *n, int B)
     if ((rc = bias_grad(st, n->d3b, P3, c2, n->part, Gr + n->boff[3]))) return rc;
     if ((rc = conv(st, c2, c2, n->d3b, WTl(3), nullptr, nullptr, n->d3a, G3, 0))) return rc;   // d_p3
     // a2: skip (d_u2) + pool path -> dpre2
-    k_skip_pool_mask<<<g1(P2 * c2), NB, 0, st>>>(n->d2b, n->d3a, n->a2, n->d2c, G2, c2);
+    skip_pool_mask(st, n->d2b, n->d3a, n->a2, n->d2c, G2, c2);
     if ((rc = dw(st, c1, c2, n->d2c, n->p2, n->part, n->part_n, Gr + n->woff[2], G2))) return rc;
     if ((rc = bias_grad(st, n->d2c, P2, c2, n->part, Gr + n->boff[2]))) return rc;
     if ((rc = conv(st, c2, c1, n->d2c, WTl(2), nullptr, nullptr, n->d2a, G2, 0))) return rc;   // d_p2
     // a1: skip (d_u1) + pool path -> dpre1
-    k_skip_pool_mask<<<g1(P1 * c1), NB, 0, st>>>(n->d1b, n->d2a, n->a1, n->d1c, G1, c1);
+    skip_pool_mask(st, n->d1b, n->d2a, n->a1, n->d1c, G1, c1);
     if ((rc = dw(st, c0, c1, n->d1c, n->p1, n->part, n->part_n, Gr + n->woff[1], G1))) return rc;
     if ((rc = bias_grad(st, n->d1c, P1, c1, n->part, Gr + n->boff[1]))) return rc;
     if ((rc = conv(st, c1, c0, n->d1c, WTl(1), nullptr, nullptr, n->d1a, G1, 0))) return rc;   // d_p1
     // a0: skip (d_u0) + pool path -> dpre0
-    k_skip_pool_mask<<<g1(P0 * c0), NB, 0, st>>>(n->d0a, n->d1a, n->a0, n->d0b, G0, c0);
-    {
-        const int nblk = (int)std::min<int64_t>(2048, (P0 + 255) / 256);
-        const int64_t chunk = (P0 + nblk - 1) / nblk;
-        k_conv_in_dw<32><<<nblk, 9 * 32, 0, st>>>(n->d0b, n->x_in, G0, chunk, n->part);
-        k_part_reduce<<<gp_cdiv(9 * c0, NB), NB, 0, st>>>(n->part, nblk, 9 * c0, Gr + n->woff[0]);
-    }
+    skip_pool_mask(st, n->d0a, n->d1a, n->a0, n->d0b, G0, c0);
+    if ((rc = dw1(st, n, n->d0b, n->x_in, G0, 1, 0, Gr + n->woff[0]))) return rc;
     if ((rc = bias_grad(st, n->d0b, P0, c0, n->part, Gr + n->boff[0]))) return rc;
     GP_HIP(hipGetLastError());
     return GP_OK;
@@ -888,7 +1020,7 @@ int nn_loss(hipStream_t st, gp_nn *n, int B, int kind, double *loss)
         k_energy_grad<<<nblk, NB, 0, st>>>(n->dtmp, n->dy, e, n->bw, n->lpart);
     }
     float *sum = n->lpart + nblk;
-    k_part_reduce<<<1, NB, 0, st>>>(n->lpart, nblk, 1, sum);
+    part_reduce(st, n->lpart, nblk, 1, sum);
     GP_HIP(hipGetLastError());
     GP_HIP(hipMemcpyAsync(n->hloss, sum, 4, hipMemcpyDeviceToHost, st));
     GP_HIP(hipStreamSynchronize(st));
