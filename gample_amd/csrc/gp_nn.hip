@@ -36,6 +36,7 @@
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef uint16_t bf16s;   // BF16 storage
 
 namespace {
@@ -43,13 +44,8 @@ namespace {
 constexpr int NL = 8;            // layers: in, W1..W6, out
 constexpr int NB = 256;          // threads of the elementwise kernels
 
-__device__ __forceinline__ bf16s f2b(float f)
-{
-    // round to nearest even (finite inputs)
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (bf16s)(u >> 16);
-}
+// round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ bf16s f2b(float f) { return __builtin_bit_cast(bf16s, (__bf16)f); }
 __device__ __forceinline__ float b2f(bf16s h) { return __uint_as_float((uint32_t)h << 16); }
 // the 8 BF16 values of a 16-byte word, in memory (channel) order
 __device__ __forceinline__ void unpack8(const uint4 &v, float *f)
@@ -122,13 +118,30 @@ __global__ __launch_bounds__(256) void k_conv(const bf16s *in, const bf16s *w, c
     for (int n = 0; n < CO / 16; n++) acc[n] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int i = lane & 15, kq = lane >> 4;
     const int ry = 2 * wv + (i >> 3), rx = i & 7;
+    // the next tap's weights are loaded into registers while this tap's
+    // MFMAs run (their global latency is off the critical path)
+    constexpr int WPT = (CO * CH + 255) / 256;
+    u32x4 wr[WPT];
+#pragma unroll
+    for (int k = 0; k < WPT; k++) {
+        const int e = tid + 256 * k;
+        if (e < CO * CH) wr[k] = *(const u32x4 *)(w + ((int64_t)(e / CH) * 9) * CI + (e % CH) * 8);
+    }
     for (int t = 0; t < 9; t++) {
         __syncthreads();   // previous tap's weights consumed (and the halo staged)
-        for (int e = tid; e < CO * CH; e += 256) {
-            const int co = e / CH, c8 = e % CH;
-            *(uint4 *)(sW + co * PP + c8 * 8) = *(const uint4 *)(w + ((int64_t)co * 9 + t) * CI + c8 * 8);
+#pragma unroll
+        for (int k = 0; k < WPT; k++) {
+            const int e = tid + 256 * k;
+            if (e < CO * CH) *(u32x4 *)(sW + (e / CH) * PP + (e % CH) * 8) = wr[k];
         }
         __syncthreads();
+        if (t < 8) {
+#pragma unroll
+            for (int k = 0; k < WPT; k++) {
+                const int e = tid + 256 * k;
+                if (e < CO * CH) wr[k] = *(const u32x4 *)(w + ((int64_t)(e / CH) * 9 + t + 1) * CI + (e % CH) * 8);
+            }
+        }
         const int dy = t / 3 - 1, dx = t % 3 - 1;
         const bf16s *arow = sIn + ((ry + 1 + dy) * 10 + (rx + 1 + dx)) * PP + kq * 8;
 #pragma unroll
@@ -434,6 +447,107 @@ __global__ __launch_bounds__(9 * C0) void k_dw1(const bf16s *A, const float *v, 
         }
     }
     part[(int64_t)blockIdx.x * NT1 + (ct ? t * C0 + c : 9 * c + t)] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Level 0 of the inference forward, fused: the C0-channel activations a0 and
+// u0 (64 B per pixel each) are never written.  Both kernels evaluate every
+// value exactly as the unfused chain does (conv_in's FMA order, the BF16
+// rounding of a0, of the pooled p1 and of u0 = up(g0) + a0, conv_out's tap
+// and channel order), so the output is the same to the bit.
+// ---------------------------------------------------------------------------
+// p1 = pool(relu(conv_in(r))): one thread per pooled pixel, all C0 channels
+template <int C0>
+__global__ void k_in_pool(const float *r, const float *w, const float *bias, bf16s *p1, Geo g)
+{
+    const int H2 = g.H / 2, W2 = g.W / 2;
+    const int64_t e = blockIdx.x * (int64_t)NB + threadIdx.x;
+    if (e >= (int64_t)g.B * H2 * W2) return;
+    const int x2 = (int)(e % W2), y2 = (int)((e / W2) % H2), b = (int)(e / ((int64_t)W2 * H2));
+    float v[4][4];   // r around the 2 x 2 block, rows 2 y2 - 1 .. 2 y2 + 2
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int sp = src_pix(g, b, 2 * y2 - 1 + a, 2 * x2 - 1 + c);
+            v[a][c] = sp >= 0 ? r[sp] : 0.0f;
+        }
+    for (int c8 = 0; c8 < C0 / 8; c8++) {
+        float o[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int c = 8 * c8 + k;
+            float q[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {   // (0,0) (0,1) (1,0) (1,1)
+                const int oy = j >> 1, ox = j & 1;
+                float s0 = bias[c];
+#pragma unroll
+                for (int t = 0; t < 9; t++) s0 = __builtin_fmaf(w[c * 9 + t], v[oy + t / 3][ox + t % 3], s0);
+                q[j] = b2f(f2b(fmaxf(s0, 0.0f)));
+            }
+            o[k] = 0.25f * ((q[0] + q[1]) + (q[2] + q[3]));
+        }
+        *(uint4 *)(p1 + e * C0 + 8 * c8) = pack8(o);
+    }
+}
+// y = conv_out(up(g0) + relu(conv_in(r))) on 14 x 14 output tiles: the r
+// window (18 x 18) and u0 on the 16 x 16 window (channel-major, odd pitch;
+// one window position per thread) are built in LDS, then each of 196
+// threads sums its pixel's 9 taps x C0 channels
+template <int C0>
+__global__ __launch_bounds__(256) void k_up_out(const float *r, const bf16s *g0, const float *win, const float *bin,
+                                                const float *wout, const float *bout, float *y, Geo g)
+{
+    constexpr int T = 14, UW = T + 2, RW = T + 4, UP = UW * UW + 1;
+    __shared__ float sr[RW * RW];
+    __shared__ float su[C0 * UP];
+    const int tid = threadIdx.x;
+    const int tx0 = blockIdx.x * T, ty0 = blockIdx.y * T, b = blockIdx.z;
+    for (int e = tid; e < RW * RW; e += 256) {
+        const int sp = src_pix(g, b, ty0 - 2 + e / RW, tx0 - 2 + e % RW);
+        sr[e] = sp >= 0 ? r[sp] : 0.0f;
+    }
+    __syncthreads();
+    // one window position per thread: its 3 x 3 r values once, then every
+    // channel (the weights are uniform across the block)
+    for (int pix = tid; pix < UW * UW; pix += 256) {
+        const int hy = pix / UW, hx = pix % UW;
+        // the pixel this window position reads (wrapped on a periodic axis)
+        const int sp = src_pix(g, b, ty0 - 1 + hy, tx0 - 1 + hx);
+        if (sp < 0) {
+            for (int c = 0; c < C0; c++) su[c * UP + pix] = 0.0f;
+            continue;
+        }
+        const int rem = sp % (g.H * g.W), wy = rem / g.W, wx = rem % g.W;
+        float v[9];
+#pragma unroll
+        for (int t = 0; t < 9; t++) v[t] = sr[(hy + t / 3) * RW + hx + t % 3];
+        const bf16s *gp = g0 + (((int64_t)b * (g.H / 2) + wy / 2) * (g.W / 2) + wx / 2) * C0;
+        for (int c8 = 0; c8 < C0 / 8; c8++) {
+            float gv[8];
+            unpack8(*(const uint4 *)(gp + 8 * c8), gv);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int c = 8 * c8 + k;
+                float s0 = bin[c];
+#pragma unroll
+                for (int t = 0; t < 9; t++) s0 = __builtin_fmaf(win[c * 9 + t], v[t], s0);
+                su[c * UP + pix] = b2f(f2b(gv[k] + b2f(f2b(fmaxf(s0, 0.0f)))));
+            }
+        }
+    }
+    __syncthreads();
+    const int ly = tid / T, lx = tid % T, oy = ty0 + ly, ox = tx0 + lx;
+    if (tid >= T * T || oy >= g.H || ox >= g.W) return;
+    float s = bout[0];
+    for (int t = 0; t < 9; t++) {
+        if (src_pix(g, b, oy + t / 3 - 1, ox + t % 3 - 1) < 0) continue;
+        const float *up = su + (ly + t / 3) * UW + lx + t % 3;
+#pragma unroll
+        for (int c = 0; c < C0; c++) s = __builtin_fmaf(wout[t * C0 + c], up[c * UP], s);
+    }
+    y[((int64_t)b * g.H + oy) * g.W + ox] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -915,7 +1029,7 @@ int nn_prep_weights(hipStream_t st, gp_nn *n)
 }
 
 // forward of B images already in n->x_in; output n->y_out (padded images)
-int nn_forward(hipStream_t st, gp_nn *n, int B)
+int nn_forward(hipStream_t st, gp_nn *n, int B, bool infer = false)
 {
     const int c0 = n->c0, c1 = 2 * c0, c2 = 4 * c0;
     const Geo G0 = geo(n, B, 0), G1 = geo(n, B, 1), G2 = geo(n, B, 2), G3 = geo(n, B, 3);
@@ -923,8 +1037,12 @@ int nn_forward(hipStream_t st, gp_nn *n, int B)
     float *P = n->P;
     auto WBl = [&](int l) { return n->WB + n->bfoff[l]; };
     int rc;
-    k_conv_in<32><<<g1(P0), NB, 0, st>>>(n->x_in, P + n->woff[0], P + n->boff[0], n->a0, G0);
-    pool(st, n->a0, n->p1, G0, c0);
+    if (infer) {
+        k_in_pool<32><<<g1(P0 / 4), NB, 0, st>>>(n->x_in, P + n->woff[0], P + n->boff[0], n->p1, G0);
+    } else {
+        k_conv_in<32><<<g1(P0), NB, 0, st>>>(n->x_in, P + n->woff[0], P + n->boff[0], n->a0, G0);
+        pool(st, n->a0, n->p1, G0, c0);
+    }
     if ((rc = conv(st, c0, c1, n->p1, WBl(1), P + n->boff[1], nullptr, n->a1, G1, 1))) return rc;
     pool(st, n->a1, n->p2, G1, c1);
     if ((rc = conv(st, c1, c2, n->p2, WBl(2), P + n->boff[2], nullptr, n->a2, G2, 1))) return rc;
@@ -935,8 +1053,13 @@ int nn_forward(hipStream_t st, gp_nn *n, int B)
     if ((rc = conv(st, c2, c1, n->u2, WBl(5), P + n->boff[5], nullptr, n->g1, G2, 1))) return rc;
     up_add(st, n->g1, n->a1, n->u1, G1, c1);
     if ((rc = conv(st, c1, c0, n->u1, WBl(6), P + n->boff[6], nullptr, n->g0, G1, 1))) return rc;
-    up_add(st, n->g0, n->a0, n->u0, G0, c0);
-    k_conv_out<32><<<g1(P0), NB, 0, st>>>(n->u0, P + n->woff[7], P + n->boff[7], n->y_out, G0);
+    if (infer) {
+        k_up_out<32><<<dim3((G0.W + 13) / 14, (G0.H + 13) / 14, B), 256, 0, st>>>(
+            n->x_in, n->g0, P + n->woff[0], P + n->boff[0], P + n->woff[7], P + n->boff[7], n->y_out, G0);
+    } else {
+        up_add(st, n->g0, n->a0, n->u0, G0, c0);
+        k_conv_out<32><<<g1(P0), NB, 0, st>>>(n->u0, P + n->woff[7], P + n->boff[7], n->y_out, G0);
+    }
     GP_HIP(hipGetLastError());
     return GP_OK;
 }
@@ -1257,7 +1380,7 @@ int gp_nn_forward_impl(gp_ctx *c, const float *in, int B, float *out)
     if (rc) return rc;
     const int64_t HW = (int64_t)n->H * n->W;
     GP_HIP(hipMemcpyAsync(n->x_in, in, B * HW * 4, hipMemcpyHostToDevice, st));
-    if ((rc = nn_forward(st, n, B))) return rc;
+    if ((rc = nn_forward(st, n, B, true))) return rc;
     GP_HIP(hipMemcpyAsync(out, n->y_out, B * HW * 4, hipMemcpyDeviceToHost, st));
     GP_HIP(hipStreamSynchronize(st));
     return GP_OK;
@@ -1288,9 +1411,8 @@ int gp_nn_apply(gp_ctx *c, gp_solver *s, const float *r, float *z)
     int rc = nn_set_grid(n, *s);
     if (rc) return rc;
     if ((rc = nn_alloc_batch(n, 1))) return rc;
-    if (rc) return rc;
     if ((rc = nn_embed(st, n, r, n->x_in, n->stats))) return rc;
-    if ((rc = nn_forward(st, n, 1))) return rc;
+    if ((rc = nn_forward(st, n, 1, true))) return rc;
     k_extract<<<g1((int64_t)n->mx * n->my), NB, 0, st>>>(n->y_out, n->H, n->W, n->mx, n->my, z);
     GP_HIP(hipGetLastError());
     return GP_OK;

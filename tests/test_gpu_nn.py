@@ -40,7 +40,31 @@ def rel(a, b, floor=1e-30):
 GEOMS = [(16, 16, ("periodic",) * 4), (24, 16, ("Dirichlet", "Dirichlet", "periodic", "periodic"))]
 
 
-@pytest.mark.parametrize("ngx,ngy,fbc", GEOMS)
+# the forward also on a grid whose padded image is not a multiple of the
+# fused level-0 kernels' 14 x 14 output tiles, periodic in y only
+FWD_GEOMS = GEOMS + [(40, 24, ("Dirichlet", "Dirichlet", "periodic", "periodic"))]
+
+
+@pytest.mark.parametrize("ngx,ngy,fbc", FWD_GEOMS)
+def test_inference_forward_equals_training_forward(gp, ngx, ngy, fbc):
+    """gp_nn_forward runs the inference chain (level 0 fused: a0 and u0 never
+    stored); gp_nn_grad runs the training chain that stores them.  With the
+    inference output as the target, the training chain's mean-square loss
+    is exactly 0 only if both outputs agree to the bit."""
+    say(f"[nn fused forward] {ngx}x{ngy} {fbc[0]}")
+    ctx = ctx_for(gp, ngx, ngy, fbc)
+    ctx.nn_init(32, seed=5)
+    H, W = ctx.nn_dims()
+    x = np.random.default_rng(3).standard_normal((3, H, W)).astype(np.float32)
+    y = ctx.nn_forward(x)
+    loss, _ = ctx.nn_grad(x, y, 0)
+    say(f"   H={H} W={W} loss {loss!r}")
+    assert np.all(np.isfinite(y)) and np.abs(y).max() > 0
+    assert loss == 0.0
+    ctx.close()
+
+
+@pytest.mark.parametrize("ngx,ngy,fbc", FWD_GEOMS)
 def test_forward_matches_oracle(gp, ngx, ngy, fbc):
     say(f"[nn forward] {ngx}x{ngy} {fbc[0]}")
     ctx = ctx_for(gp, ngx, ngy, fbc)
