@@ -701,41 +701,6 @@ __global__ void k_energy_grad(const float *d, float *dy, EGeo e, const float *wb
     }
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
-// stats[1] = sqrt(v' S1 v) of a (mean-removed when singular) unknown-grid
-// vector v (mx x my); stats[0] = its mean (0 unless singular)
-__global__ void k_snorm(const float *v, EGeo e, int singular, double *stats)
-{
-    __shared__ double sh[NB];
-    const int64_t n = (int64_t)e.mx * e.my;
-    double s = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += NB) s += v[k];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = NB / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    const double mean = singular ? sh[0] / (double)n : 0.0;
-    __syncthreads();
-    // S1 is blind to the constant, so v' S1 v needs no mean removal
-    EGeo u = e;
-    u.W = e.mx;
-    double q = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += NB) {
-        const int x = (int)(k % e.mx), y = (int)(k / e.mx);
-        q += (double)v[k] * (double)s1_at(v, u, x, y);
-    }
-    sh[threadIdx.x] = q;
-    __syncthreads();
-    for (int w = NB / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        stats[0] = mean;
-        stats[1] = sqrt(fmax(sh[0], 0.0));
-    }
-}
 
 // Adam (Kingma & Ba 2015) on the FP32 master parameters
 __global__ void k_adam(float *p, const float *gr, float *m, float *v, int64_t n, float lr, float b1, float b2, float eps,
@@ -763,32 +728,69 @@ __global__ void k_wprep(const float *w, bf16s *wb, bf16s *wt, int CO, int CI)
 
 // per-sample normalisation of a right-hand side: out = (in - mean) / rms of
 // the unknown grid (mx x my) placed in the padded H x W image (zeros outside);
-// stats[0] = mean (0 unless singular), stats[1] = rms after the mean removal
-__global__ void k_stats(const float *in, int64_t n, int singular, double *stats)
+// Statistics of an unknown-grid vector v (mx x my) in two passes over
+// NSB blocks, partials added in a fixed order:
+//   stats[0] = mean (0 unless the problem is singular)
+//   stats[1] = rms after the mean removal (kind 0), or sqrt(v' S1 v) (kind 1;
+//              S1 is blind to the constant, so no mean removal is needed)
+constexpr int NSB = 256;
+__device__ double block_sum(double s, double *sh)
 {
-    __shared__ double sh[NB];
-    double s = 0.0, q = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += NB) s += in[k];
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int w = NB / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    const double mean = singular ? sh[0] / (double)n : 0.0;
-    __syncthreads();
-    for (int64_t k = threadIdx.x; k < n; k += NB) { const double d = in[k] - mean; q += d * d; }
-    sh[threadIdx.x] = q;
-    __syncthreads();
-    for (int w = NB / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        stats[0] = mean;
-        stats[1] = sqrt(sh[0] / (double)n);
-    }
+    return sh[0];
 }
+// part[blk] = sum of v over the block's contiguous chunk
+__global__ __launch_bounds__(NB) void k_vsum(const float *v, int64_t n, double *part)
+{
+    __shared__ double sh[NB];
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x, p0 = blockIdx.x * chunk;
+    const int64_t p1 = std::min(n, p0 + chunk);
+    double s = 0.0;
+    for (int64_t k = p0 + threadIdx.x; k < p1; k += NB) s += v[k];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// part[blk] = sum over the chunk of (v - stats[0])^2 (kind 0) or v (S1 v) (kind 1)
+__global__ __launch_bounds__(NB) void k_vsq(const float *v, EGeo e, const double *stats, int kind, double *part)
+{
+    __shared__ double sh[NB];
+    const int64_t n = (int64_t)e.mx * e.my;
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x, p0 = blockIdx.x * chunk;
+    const int64_t p1 = std::min(n, p0 + chunk);
+    const double mean = stats[0];
+    EGeo u = e;
+    u.W = e.mx;
+    double q = 0.0;
+    for (int64_t k = p0 + threadIdx.x; k < p1; k += NB) {
+        if (kind == 0) {
+            const double d = v[k] - mean;
+            q += d * d;
+        } else {
+            const int x = (int)(k % e.mx), y = (int)(k / e.mx);
+            q += (double)v[k] * (double)s1_at(v, u, x, y);
+        }
+    }
+    q = block_sum(q, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = q;
+}
+// stats[slot] from np partials: op 0 the mean (singular ? s / n : 0),
+// op 1 sqrt(s / n), op 2 sqrt(max(s, 0))
+__global__ __launch_bounds__(NB) void k_vfin(const double *part, int np, int64_t n, int op, int singular, double *stats,
+                                             int slot)
+{
+    __shared__ double sh[NB];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < np; k += NB) s += part[k];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0)
+        stats[slot] = op == 0 ? (singular ? s / (double)n : 0.0) : op == 1 ? sqrt(s / (double)n) : sqrt(fmax(s, 0.0));
+}
+// img (H x W, zero padded) = (v - stats[0]) / stats[1] on the mx x my grid
 __global__ void k_embed(const float *in, int mx, int my, const double *stats, float *img, int H, int W)
 {
     const int64_t e = blockIdx.x * (int64_t)NB + threadIdx.x;
@@ -1159,14 +1161,24 @@ uint64_t sm64(uint64_t &s)
     return z ^ (z >> 31);
 }
 
+// stats[0..1] of v (k_vsum .. k_vfin); the partials use n->stats[64 ..]
+void vstats(hipStream_t st, gp_nn *n, const float *v, int kind, double *stats)
+{
+    const int64_t m = (int64_t)n->mx * n->my;
+    double *part = n->stats + 64;
+    k_vsum<<<NSB, NB, 0, st>>>(v, m, part);
+    k_vfin<<<1, NB, 0, st>>>(part, NSB, m, 0, n->singular, stats, 0);
+    k_vsq<<<NSB, NB, 0, st>>>(v, egeo(n, 1), stats, kind, part);
+    k_vfin<<<1, NB, 0, st>>>(part, NSB, m, kind == 0 ? 1 : 2, n->singular, stats, 1);
+}
+
 // normalise one unknown-grid vector (device FP32, mx*my) into image slot
 // img: by its rms (kind 0, network inputs) or by its S1 energy norm (kind 1,
 // training targets); the mean is removed first when the problem is singular
 int nn_embed(hipStream_t st, gp_nn *n, const float *v, float *img, double *stats_dev, int kind = 0)
 {
     const int64_t m = (int64_t)n->mx * n->my;
-    if (kind == 0) k_stats<<<1, NB, 0, st>>>(v, m, n->singular, stats_dev);
-    else k_snorm<<<1, NB, 0, st>>>(v, egeo(n, 1), n->singular, stats_dev);
+    vstats(st, n, v, kind, stats_dev);
     k_embed<<<g1((int64_t)n->H * n->W), NB, 0, st>>>(v, n->mx, n->my, stats_dev, img, n->H, n->W);
     GP_HIP(hipGetLastError());
     return GP_OK;
@@ -1220,7 +1232,7 @@ int gp_nn_create(gp_ctx *c, int base_ch, uint64_t seed)
     GP_HIP(hipMemset(n->V, 0, n->nparam * 4));
     GP_HIP(hipMalloc((void **)&n->WB, std::max<int64_t>(n->nbf, 1) * 2));
     GP_HIP(hipMalloc((void **)&n->WT, std::max<int64_t>(n->nbf, 1) * 2));
-    GP_HIP(hipMalloc((void **)&n->stats, 64 * sizeof(double)));
+    GP_HIP(hipMalloc((void **)&n->stats, (64 + NSB) * sizeof(double)));   // slots, then partials
     GP_HIP(hipHostMalloc((void **)&n->hloss, 64, hipHostMallocDefault));
     // He-uniform initialisation from a splitmix64 stream, biases zero
     std::vector<float> h(n->nparam, 0.0f);
@@ -1270,7 +1282,7 @@ int gp_nn_add_pair_dev(gp_ctx *c, const double *b, const double *x)
     if (rc) return rc;
     k_d2f<<<g1(m), NB, 0, st>>>(x, n->tmpf, m);
     if ((rc = nn_embed(st, n, n->tmpf, tg, n->stats + 2, 0))) return rc;
-    k_snorm<<<1, NB, 0, st>>>(n->tmpf, egeo(n, 1), n->singular, n->stats + 4);
+    vstats(st, n, n->tmpf, 1, n->stats + 4);
     GP_HIP(hipGetLastError());
     double hs[6];
     GP_HIP(hipMemcpyAsync(hs, n->stats, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
