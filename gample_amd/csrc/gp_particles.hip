@@ -1339,6 +1339,9 @@ int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke)
     return GP_OK;
 }
 
+#ifndef GP_CLS_MOVED
+#define GP_CLS_MOVED 0.01
+#endif
 int gp_launch_sort(gp_ctx *c, int sid)
 {
     gp_species &S = c->sp[sid];
@@ -1354,8 +1357,9 @@ int gp_launch_sort(gp_ctx *c, int sid)
     const int g = gp_cdiv(n, GP_BLOCK);
     const int W = WIN + c->order + 1;
     // interleave the bank classes unless the species' particles change cell
-    // too often for the order to last (more than 1 % of them in the last push)
-    const bool cls = S.last_moved < 0 || (double)S.last_moved < 0.01 * (double)n;
+    // too often for the order to last (more than GP_CLS_MOVED of them in the
+    // last push)
+    const bool cls = S.last_moved < 0 || (double)S.last_moved < GP_CLS_MOVED * (double)n;
     k_tile_key<<<g, GP_BLOCK, 0, s>>>(S.a.x, S.a.y, n, c->ngb, c->ngx, c->ngy, c->ntx, W, cls ? 1 : 0, S.key);
     GP_HIP(hipGetLastError());
     uint32_t *ks = nullptr, *vs = nullptr;
