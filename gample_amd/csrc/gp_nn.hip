@@ -1439,6 +1439,11 @@ int gp_nn_infer_impl(gp_ctx *c, const double *rhs, double *out)
     std::vector<float> h(m);
     for (int64_t k = 0; k < m; k++) h[k] = (float)rhs[k];
     float *r = nullptr, *z = nullptr;
+    // every exit below frees both buffers
+    struct Bufs {
+        float **a, **b;
+        ~Bufs() { if (*a) (void)hipFree(*a); if (*b) (void)hipFree(*b); }
+    } guard{&r, &z};
     GP_HIP(hipMalloc((void **)&r, m * 4));
     GP_HIP(hipMalloc((void **)&z, m * 4));
     GP_HIP(hipMemcpy(r, h.data(), m * 4, hipMemcpyHostToDevice));
@@ -1448,8 +1453,6 @@ int gp_nn_infer_impl(gp_ctx *c, const double *rhs, double *out)
         GP_HIP(hipMemcpy(h.data(), z, m * 4, hipMemcpyDeviceToHost));
         for (int64_t k = 0; k < m; k++) out[k] = h[k];
     }
-    (void)hipFree(r);
-    (void)hipFree(z);
     return rc;
 }
 

@@ -1459,13 +1459,19 @@ int gp_launch_generate(gp_ctx *c, int se, int si, const gp_gen *g, int64_t addn,
         if (addn > c->rng_cap) {
             const int64_t cap = std::max(addn, 2 * c->rng_cap);
             uint64_t *ns = nullptr, *ni = nullptr;
-            GP_HIP(hipMalloc((void **)&ns, sizeof(uint64_t) * cap));
-            GP_HIP(hipMalloc((void **)&ni, sizeof(uint64_t) * cap));
-            if (c->rng_n > 0) {
-                GP_HIP(hipMemcpyAsync(ns, c->rng_s, sizeof(uint64_t) * c->rng_n, hipMemcpyDeviceToDevice, s));
-                GP_HIP(hipMemcpyAsync(ni, c->rng_inc, sizeof(uint64_t) * c->rng_n, hipMemcpyDeviceToDevice, s));
+            hipError_t e = hipMalloc((void **)&ns, sizeof(uint64_t) * cap);
+            if (e == hipSuccess) e = hipMalloc((void **)&ni, sizeof(uint64_t) * cap);
+            if (e == hipSuccess && c->rng_n > 0) {
+                e = hipMemcpyAsync(ns, c->rng_s, sizeof(uint64_t) * c->rng_n, hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(ni, c->rng_inc, sizeof(uint64_t) * c->rng_n, hipMemcpyDeviceToDevice, s);
             }
-            GP_HIP(hipStreamSynchronize(s));
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) {   // the old states stay in place
+                if (ns) (void)hipFree(ns);
+                if (ni) (void)hipFree(ni);
+                return gp_fail(GP_ERR_NOMEM, std::string("generation streams: ") + hipGetErrorString(e));
+            }
             if (c->rng_s) GP_HIP(hipFree(c->rng_s));
             if (c->rng_inc) GP_HIP(hipFree(c->rng_inc));
             c->rng_s = ns; c->rng_inc = ni; c->rng_cap = cap;
