@@ -12,7 +12,8 @@
 //   prolongation   bilinear P, added
 //   post-smoothing NSW sweeps in the adjoint order (black, then red)
 // Each grid level costs two launches: k_mg_down (the pre-sweeps, residual
-// and restriction on a 32x32 tile whose (2 NSW + 1)-wide halo is recomputed
+// and restriction on a 32x32 tile -- 64x64 on a level with more than 512
+// such tiles -- whose (2 NSW + 1)-wide halo is recomputed
 // locally) and k_mg_up (prolongation and the post-sweeps with a 2 NSW-wide
 // halo).  Recomputing the halo reproduces global red-black Gauss-Seidel
 // exactly.  All levels with at
@@ -208,7 +209,7 @@ __device__ bool gate_converged(const Gate &g, double *sh, int tid, bool block0)
 // sweeps exactly on the tile and its 1-wide rim (needed by the restriction).
 // The window's global loads are all issued before any is consumed.
 // ---------------------------------------------------------------------------
-template <int NS, int TX, int TY>
+template <int NS, int TB, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_mg_down(const float *f, float *u, float *fc, Lev Lin, Lev Cin, const double *scal,
                                                  Gate gt)
 {
@@ -276,8 +277,10 @@ __global__ __launch_bounds__(NT) void k_mg_down(const float *f, float *u, float 
     for (int lj = ty; lj < TB; lj += BY) {
         const int gj = tj0 + lj;
         if (gj >= L.my) break;
-        const int gi = ti0 + tx;
-        if (gi < L.mx) u[gi + gj * L.mx] = su[(HD + tx) + (HD + lj) * WD];
+        for (int li = tx; li < TB; li += 32) {
+            const int gi = ti0 + li;
+            if (gi < L.mx) u[gi + gj * L.mx] = su[(HD + li) + (HD + lj) * WD];
+        }
     }
     // restriction for the coarse unknowns centred in this tile
     const int ci0 = ti0 > 0 ? ti0 / 2 - 1 : 0, cj0 = tj0 > 0 ? tj0 / 2 - 1 : 0;
@@ -316,7 +319,7 @@ __global__ __launch_bounds__(NT) void k_mg_down(const float *f, float *u, float 
 // halo.  Optional partial of f.u (the CG's r.z at the finest level).
 // blockDim (32, BY)
 // ---------------------------------------------------------------------------
-template <int NS, int TX, int TY>
+template <int NS, int TB, int TX, int TY>
 __global__ __launch_bounds__(NT) void k_mg_up(const float *upre, const float *f, const float *uc, float *u, Lev Lin,
                                                Lev Cin, double *part, const double *scal)
 {
@@ -364,14 +367,15 @@ __global__ __launch_bounds__(NT) void k_mg_up(const float *upre, const float *f,
         __syncthreads();
     }
     double acc = 0.0;
-    for (int lj = ty; lj < TB; lj += BY) {
-        const int gj = tj0 + lj, gi = ti0 + tx;
-        if (gj < L.my && gi < L.mx) {
-            const int k = (HU + tx) + (HU + lj) * WU;
-            u[gi + gj * L.mx] = su[k];
-            acc += (double)sf[k] * (double)su[k];
+    for (int lj = ty; lj < TB; lj += BY)
+        for (int li = tx; li < TB; li += 32) {
+            const int gj = tj0 + lj, gi = ti0 + li;
+            if (gj < L.my && gi < L.mx) {
+                const int k = (HU + li) + (HU + lj) * WU;
+                u[gi + gj * L.mx] = su[k];
+                acc += (double)sf[k] * (double)su[k];
+            }
         }
-    }
     if (part) {
         for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
         if ((tid & 63) == 0) red[tid >> 6] = acc;
@@ -930,7 +934,15 @@ Lev mklev(const gp_mg_level &l)
     L.rb = !((L.tx == GP_AX_PER && (l.mx & 1)) || (L.ty == GP_AX_PER && (l.my & 1)));
     return L;
 }
-inline dim3 tiles(const Lev &L) { return dim3((L.mx + TB - 1) / TB, (L.my + TB - 1) / TB); }
+inline dim3 tiles(const Lev &L, int tb = TB) { return dim3((L.mx + tb - 1) / tb, (L.my + tb - 1) / tb); }
+// tile edge of the level kernels: 64 where 32 x 32 tiles would need more
+// than one round of workgroups (two 1024-thread groups per CU), so that a
+// fine level runs in one round with a smaller share of recomputed halo
+inline int level_tile(const Lev &L)
+{
+    const dim3 g = tiles(L);
+    return (int64_t)g.x * g.y > 512 ? 2 * TB : TB;
+}
 
 // z = M r (s.lv[0].f aliases r, s.lv[0].u aliases z); part: partial r.z.
 // scal (may be null) carries the convergence flag that makes every kernel a
@@ -964,9 +976,16 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
     for (int l = 0; l < std::min(lb, nl - 1); l++) {
         const Lev L = mklev(s.lv[l]), C = mklev(s.lv[l + 1]);
         const Gate g = take();
-#define MG_DOWN(X, Y) k_mg_down<NSW, X, Y><<<tiles(L), blk, 0, st>>>(s.lv[l].f, s.lv[l].r, s.lv[l + 1].f, L, C, scal, g)
-        MG_AXES(L.tx, L.ty, MG_DOWN);
+        const int tb = level_tile(L);
+        if (tb == TB) {
+#define MG_DOWN(X, Y) k_mg_down<NSW, TB, X, Y><<<tiles(L, tb), blk, 0, st>>>(s.lv[l].f, s.lv[l].r, s.lv[l + 1].f, L, C, scal, g)
+            MG_AXES(L.tx, L.ty, MG_DOWN);
 #undef MG_DOWN
+        } else {
+#define MG_DOWN(X, Y) k_mg_down<NSW, 2 * TB, X, Y><<<tiles(L, tb), blk, 0, st>>>(s.lv[l].f, s.lv[l].r, s.lv[l + 1].f, L, C, scal, g)
+            MG_AXES(L.tx, L.ty, MG_DOWN);
+#undef MG_DOWN
+        }
     }
     GP_HIP(hipGetLastError());
     *nparts = 0;
@@ -1017,11 +1036,18 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
     }
     for (int l = std::min(lb, nl - 1) - 1; l >= 0; l--) {
         const Lev L = mklev(s.lv[l]), C = mklev(s.lv[l + 1]);
-        const dim3 g = tiles(L);
+        const int tb = level_tile(L);
+        const dim3 g = tiles(L, tb);
         double *pp = l == 0 ? part : nullptr;
-#define MG_UP(X, Y) k_mg_up<NSW, X, Y><<<g, blk, 0, st>>>(s.lv[l].r, s.lv[l].f, s.lv[l + 1].u, s.lv[l].u, L, C, pp, scal)
-        MG_AXES(L.tx, L.ty, MG_UP);
+        if (tb == TB) {
+#define MG_UP(X, Y) k_mg_up<NSW, TB, X, Y><<<g, blk, 0, st>>>(s.lv[l].r, s.lv[l].f, s.lv[l + 1].u, s.lv[l].u, L, C, pp, scal)
+            MG_AXES(L.tx, L.ty, MG_UP);
 #undef MG_UP
+        } else {
+#define MG_UP(X, Y) k_mg_up<NSW, 2 * TB, X, Y><<<g, blk, 0, st>>>(s.lv[l].r, s.lv[l].f, s.lv[l + 1].u, s.lv[l].u, L, C, pp, scal)
+            MG_AXES(L.tx, L.ty, MG_UP);
+#undef MG_UP
+        }
         if (l == 0) *nparts = (int)(g.x * g.y);
     }
     GP_HIP(hipGetLastError());
