@@ -97,7 +97,7 @@ int gp_ctx_create(const gp_config *cfg, gp_ctx **out)
         if (e != hipSuccess) { delete c; return gp_fail(GP_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
         int rc = gp_fields_init(c);
         if (rc) { gp_ctx_destroy(c); return rc; }
-        e = hipHostMalloc((void **)&c->h_drift, 2 * sizeof(unsigned long long) * GP_MAX_DRIFT_SPECIES);
+        e = hipHostMalloc((void **)&c->h_drift, 3 * sizeof(unsigned long long) * GP_MAX_DRIFT_SPECIES);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->drift_ev, hipEventDisableTiming);
         if (e != hipSuccess) { gp_ctx_destroy(c); return gp_fail(GP_ERR_HIP, std::string("drift counters: ") + hipGetErrorString(e)); }
         *out = c;
@@ -105,16 +105,22 @@ int gp_ctx_create(const gp_config *cfg, gp_ctx **out)
     })
 }
 
+static int species_setup(gp_ctx *c, const gp_species_desc *d, gp_species &S);
+
+static void species_free(gp_species &S)
+{
+    void *ps[] = {S.a.x, S.a.y, S.a.vx, S.a.vy, S.a.vz, S.b.x, S.b.y, S.b.vx, S.b.vy, S.b.vz, S.tile_off,
+                  S.rho_acc, S.jx_acc, S.jy_acc, S.dead, S.dead_tmp, S.counters, S.ke_part, S.ke_out,
+                  S.key, S.key2, S.val, S.val2};
+    for (void *p : ps) if (p) (void)hipFree(p);
+    S = gp_species();
+}
+
 int gp_ctx_destroy(gp_ctx *c)
 {
     if (!c) return GP_OK;
     (void)hipDeviceSynchronize();
-    for (auto &S : c->sp) {
-        void *ps[] = {S.a.x, S.a.y, S.a.vx, S.a.vy, S.a.vz, S.b.x, S.b.y, S.b.vx, S.b.vy, S.b.vz, S.tile_off,
-                      S.rho_acc, S.jx_acc, S.jy_acc, S.dead, S.dead_tmp, S.counters, S.ke_part, S.ke_out,
-                      S.key, S.key2, S.val, S.val2};
-        for (void *p : ps) if (p) (void)hipFree(p);
-    }
+    for (auto &S : c->sp) species_free(S);
     for (auto &v : c->tev)
         for (hipEvent_t e : v) (void)hipEventDestroy(e);
     gp_nn_destroy(c);
@@ -156,7 +162,22 @@ int gp_species_create(gp_ctx *c, const gp_species_desc *d, int *sid)
     if (!(d->m > 0)) return gp_fail(GP_ERR_ARG, "mass must be positive");
     GP_TRY({
         c->sp.emplace_back();
-        gp_species &S = c->sp.back();
+        // a failed set-up frees what it allocated and removes the species
+        const int rc = species_setup(c, d, c->sp.back());
+        if (rc) {
+            (void)hipStreamSynchronize(c->stream);
+            species_free(c->sp.back());
+            c->sp.pop_back();
+            return rc;
+        }
+        *sid = (int)c->sp.size() - 1;
+        return GP_OK;
+    })
+}
+
+static int species_setup(gp_ctx *c, const gp_species_desc *d, gp_species &S)
+{
+    {
         S.q = d->q; S.m = d->m; S.w = d->w;
         S.qf = (float)d->q; S.mf = (float)d->m; S.wf = (float)d->w;
         // Particle.mm:540: q*w in float, divided by dx*dy in double
@@ -200,9 +221,8 @@ int gp_species_create(gp_ctx *c, const gp_species_desc *d, int *sid)
         GP_HIP(hipMemsetAsync(S.counters, 0, 64, c->stream));
         GP_HIP(hipMemsetAsync(S.ke_out, 0, 8, c->stream));
         GP_HIP(hipStreamSynchronize(c->stream));
-        *sid = (int)c->sp.size() - 1;
         return GP_OK;
-    })
+    }
 }
 
 int gp_species_upload(gp_ctx *c, int sid, int64_t n, const float *x, const float *y, const float *vx,
@@ -466,8 +486,13 @@ int gp_step(gp_ctx *c, double dt, double tol, int maxiter, int solver, int sort_
         c->drift_pending = 0;
         for (int s = 0; s < nsp && s < GP_MAX_DRIFT_SPECIES; s++) {
             gp_species &S = c->sp[s];
-            S.last_drift = (int64_t)c->h_drift[2 * s];
-            S.last_moved = (int64_t)c->h_drift[2 * s + 1] * (c->ntiles >= 256 ? GP_MOVED_SAMPLE : 1);
+            // a live particle that left the domain through a side with no
+            // boundary condition (gp_reduce reports it too, when it runs)
+            if (c->h_drift[3 * s])
+                return gp_fail(GP_ERR_STATE, "particles outside the domain with no boundary condition: " +
+                                                 std::to_string(c->h_drift[3 * s]));
+            S.last_drift = (int64_t)c->h_drift[3 * s + 1];
+            S.last_moved = (int64_t)c->h_drift[3 * s + 2] * (c->ntiles >= 256 ? GP_MOVED_SAMPLE : 1);
             if (c->sort_drift > 0.0 && (double)S.last_drift > c->sort_drift * (double)S.n) {
                 int rc = gp_launch_sort(c, s);
                 if (rc) return rc;
@@ -480,7 +505,7 @@ int gp_step(gp_ctx *c, double dt, double tol, int maxiter, int solver, int sort_
     }
     if (c->h_drift) {
         for (int s = 0; s < nsp && s < GP_MAX_DRIFT_SPECIES; s++) {
-            GP_HIP(hipMemcpyAsync(c->h_drift + 2 * s, c->sp[s].counters + 6, 2 * sizeof(unsigned long long),
+            GP_HIP(hipMemcpyAsync(c->h_drift + 3 * s, c->sp[s].counters + 5, 3 * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, c->stream));
             GP_HIP(hipMemsetAsync(c->sp[s].counters + 6, 0, 2 * sizeof(unsigned long long), c->stream));
         }

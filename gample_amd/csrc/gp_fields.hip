@@ -401,6 +401,8 @@ int gp_fields_init(gp_ctx *c)
         GP_HIP(hipMalloc((void **)&c->dsol_ay, m * 8));
         GP_HIP(hipMalloc((void **)&c->last_ax, m * 8));
         GP_HIP(hipMalloc((void **)&c->last_ay, m * 8));
+        GP_HIP(hipMalloc((void **)&c->dar_bx, m * 8));
+        GP_HIP(hipMalloc((void **)&c->dar_by, m * 8));
         GP_HIP(hipMemset(c->dsol_chi, 0, m * 8));
         GP_HIP(hipMemset(c->dsol_ax, 0, m * 8));
         GP_HIP(hipMemset(c->dsol_ay, 0, m * 8));
@@ -415,7 +417,8 @@ void gp_fields_free(gp_ctx *c)
 {
     float *fs[] = {c->Ex, c->Ey, c->Ez, c->Bx, c->By, c->Bz, c->dEx, c->dEy, c->dBz};
     for (float *p : fs) if (p) (void)hipFree(p);
-    double *ds[] = {c->rho, c->jx, c->jy, c->phi, c->dsol_chi, c->dsol_ax, c->dsol_ay, c->last_ax, c->last_ay, c->red};
+    double *ds[] = {c->rho, c->jx, c->jy, c->phi, c->dsol_chi, c->dsol_ax, c->dsol_ay, c->last_ax, c->last_ay,
+                    c->dar_bx, c->dar_by, c->red};
     for (double *p : ds) if (p) (void)hipFree(p);
     if (c->hred) (void)hipHostFree(c->hred);
     gp_mg_free(c->ps);
@@ -552,6 +555,13 @@ int gp_solve_rhs_impl(gp_ctx *c, const double *b, double *x, double tol, int max
     const int64_t m = s.n;
     // keep the context's own solution (the warm start of its next step)
     double *keep = nullptr;
+    // every exit frees it (an early error exit leaves the warm start as the
+    // failed solve left it)
+    struct Keep {
+        double **p;
+        hipStream_t s;
+        ~Keep() { if (*p) (void)hipFreeAsync(*p, s); }
+    } guard{&keep, sm};
     GP_HIP(hipMallocAsync((void **)&keep, m * 8, sm));
     GP_HIP(hipMemcpyAsync(keep, s.x64, m * 8, hipMemcpyDeviceToDevice, sm));
     GP_HIP(hipMemcpyAsync(s.b64, b, m * 8, hipMemcpyHostToDevice, sm));
@@ -565,7 +575,6 @@ int gp_solve_rhs_impl(gp_ctx *c, const double *b, double *x, double tol, int max
     if (status < 0) return status;
     GP_HIP(hipMemcpyAsync(x, s.x64, m * 8, hipMemcpyDeviceToHost, sm));
     GP_HIP(hipMemcpyAsync(s.x64, keep, m * 8, hipMemcpyDeviceToDevice, sm));
-    GP_HIP(hipFreeAsync(keep, sm));
     GP_HIP(hipStreamSynchronize(sm));
     if (st) *st = local;
     return status;
@@ -598,9 +607,7 @@ int gp_solve_darwin_impl(gp_ctx *c, double dt, double tol, int maxiter, int solv
     if (s0 < 0) return s0;
     k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(c->dsol_chi, d.x64, m);
     // Ax, Ay rhs from chi; solve each
-    double *bx = nullptr, *by = nullptr;
-    GP_HIP(hipMallocAsync((void **)&bx, m * 8, sm));
-    GP_HIP(hipMallocAsync((void **)&by, m * 8, sm));
+    double *const bx = c->dar_bx, *const by = c->dar_by;
     k_darwin_a_rhs<<<gp_cdiv(m, 256), 256, 0, sm>>>(c->jx, c->jy, c->dsol_chi, bx, by, a);
     GP_HIP(hipGetLastError());
     if ((rc = gp_mg_sub_mean(sm, d, bx)) || (rc = gp_mg_sub_mean(sm, d, by))) return rc;
@@ -616,8 +623,6 @@ int gp_solve_darwin_impl(gp_ctx *c, double dt, double tol, int maxiter, int solv
     int s2 = gp_solve_spd(sm, d, tol, maxiter, 1, &st[2], hp);
     if (s2 < 0) return s2;
     k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(c->dsol_ay, d.x64, m);
-    GP_HIP(hipFreeAsync(bx, sm));
-    GP_HIP(hipFreeAsync(by, sm));
     k_darwin_fields<<<g2(d.nkx + 1, d.nky + 1), b2, 0, sm>>>(c->dsol_ax, c->dsol_ay, c->last_ax, c->last_ay, c->dEx,
                                                              c->dEy, c->dBz, a, C_LIGHT * dt);
     GP_HIP(hipGetLastError());

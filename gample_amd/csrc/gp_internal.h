@@ -124,6 +124,7 @@ struct gp_ctx {
     gp_solver ps;                   // Poisson
     gp_solver ds;                   // Darwin (Neumann/periodic, singular)
     double *dsol_chi = nullptr, *dsol_ax = nullptr, *dsol_ay = nullptr, *last_ax = nullptr, *last_ay = nullptr;
+    double *dar_bx = nullptr, *dar_by = nullptr;   // Ax, Ay right-hand sides
     gp_solve_stats last_darwin[3] = {};   // chi, Ax, Ay of the last Darwin solve
     double *red = nullptr;          // generic reduction scratch
     double *hred = nullptr;         // pinned
@@ -145,7 +146,7 @@ struct gp_ctx {
     // previous push gathered more than sort_drift * n of its particles
     // outside their tile windows (0: off)
     double sort_drift = 0.0;
-    unsigned long long *h_drift = nullptr;   // pinned, two per species: drifted, moved
+    unsigned long long *h_drift = nullptr;   // pinned, three per species: escaped, drifted, moved
     hipEvent_t drift_ev = nullptr;
     int drift_pending = 0;
     // generation streams (Collision.mm:189-336, 464-678): thread id of every

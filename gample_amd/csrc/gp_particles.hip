@@ -890,8 +890,15 @@ struct Pcg {
         const uint32_t rot = (uint32_t)(old >> 59u);
         return (xs >> rot) | (xs << ((-rot) & 31u));
     }
-    // (u >> 9) + 1/2 is exact in FP32: the result lies strictly inside (0, 1)
-    __device__ __forceinline__ float u01() { return ((float)(next() >> 9) + 0.5f) * 0x1p-23f; }
+    // rand01 of Collision.mm:28-30, float(u) 2^-32 (round to nearest even),
+    // with u = 0 mapped to 2^-33 (the draws feed log) and a conversion that
+    // rounds to 2^32 mapped to the largest float below 1
+    __device__ __forceinline__ float u01()
+    {
+        const uint32_t u = next();
+        const float r = (float)u * 0x1p-32f;
+        return u == 0u ? 0x1p-33f : (r < 1.0f ? r : 0x1.fffffep-1f);
+    }
 };
 __device__ __forceinline__ Pcg seed_stream(uint64_t master, int64_t id)
 {
@@ -1021,15 +1028,16 @@ __global__ void k_load(LoadArgs a)
     const float vy = g.ele_u[1] + am * sinf(2.0f * PIf * rv2);
     rv1 = r.u01(); rv2 = r.u01();
     const float vz = g.ele_u[2] + g.ele_vth * sqrtf(-logf(rv1)) * cosf(2.0f * PIf * rv2);
-    // a position inside the domain (cm) that rounds onto the upper face in
-    // grid units is moved just below it (the push takes [ngb, ngb+ng))
+    // a position inside the domain or on its upper face (cm) that lands on
+    // the upper face in grid units is moved just below it (the push takes
+    // [ngb, ngb+ng))
     {
         const float hx = a.ngb + a.xdom, hy = a.ngb + a.ydom;
         const float xc = x, yc = y;
         x = x / a.dx + a.ngb;
         y = y / a.dy + a.ngb;
-        if (xc < a.xdom * a.dx && x >= hx) x = __int_as_float(__float_as_int(hx) - 1);
-        if (yc < a.ydom * a.dy && y >= hy) y = __int_as_float(__float_as_int(hy) - 1);
+        if (xc <= a.xdom * a.dx && x >= hx) x = __int_as_float(__float_as_int(hx) - 1);
+        if (yc <= a.ydom * a.dy && y >= hy) y = __int_as_float(__float_as_int(hy) - 1);
     }
     const int64_t je = a.ne + k;
     a.e.x[je] = x; a.e.y[je] = y; a.e.vx[je] = vx; a.e.vy[je] = vy; a.e.vz[je] = vz;

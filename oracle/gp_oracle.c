@@ -590,10 +590,17 @@ uint32_t gpo_pcg32(uint64_t *state, uint64_t inc)
     uint32_t rot = (uint32_t)(old >> 59u);
     return (xs >> rot) | (xs << ((-rot) & 31));
 }
-/* Open-interval uniform: the top 23 bits plus one half, times 2^-23, so the
- * result is exact in FP32 and never 0 (log) nor 1 (upper wall).  The
- * reference's float(u)*2^-32 (src/Collision.mm:28-30) can return both. */
-float gpo_rand01(uint32_t u) { return ((float)(u >> 9) + 0.5f) * 0x1p-23f; }
+/* src/Collision.mm:28-30: float(u) * 2^-32, the conversion rounding to
+ * nearest even.  Two guarded cases: u = 0 gives 2^-33 instead of 0 (the
+ * loader and the sources take log of the draw), and the 128 largest u,
+ * whose conversion rounds to 2^32, give the largest float below 1 instead
+ * of 1 (a position drawn at the upper face would leave the domain). */
+float gpo_rand01(uint32_t u)
+{
+    const float r = (float)u * 0x1p-32f;
+    if (u == 0u) return 0x1p-33f;
+    return r < 1.0f ? r : 0x1.fffffep-1f;
+}
 
 #define PIF ((float)PI_D)
 
@@ -676,8 +683,8 @@ static void gen_one(const gpo_gen *p, int64_t id, int64_t ntot, uint64_t *stp, u
             const float xc = x, yc = y;
             x = x / p->dx + p->ngb;
             y = y / p->dy + p->ngb;
-            if (p->xdom > 0 && xc < p->xdom * p->dx && x >= hx) x = nextafterf(hx, 0.0f);
-            if (p->ydom > 0 && yc < p->ydom * p->dy && y >= hy) y = nextafterf(hy, 0.0f);
+            if (p->xdom > 0 && xc <= p->xdom * p->dx && x >= hx) x = nextafterf(hx, 0.0f);
+            if (p->ydom > 0 && yc <= p->ydom * p->dy && y >= hy) y = nextafterf(hy, 0.0f);
         }
         ex[k] = x; ey[k] = y; evx[k] = vx; evy[k] = vy; evz[k] = vz;
         if (ix && p->gen_type != 3) {

@@ -490,8 +490,9 @@ def test_poisson_large_level_tiles(gp, orc, fbc, ngx, ngy):
 def test_darwin_solves(gp, orc):
     """Darwin chain (EMField.mm:1138-1329) on a Dirichlet-x / periodic-y grid:
     the three node-grid solves reach 1e-8 on the symmetrised system D*A of
-    the reference's CSR with the reference's right-hand sides, and the delta
-    fields follow the reference's difference formulas."""
+    the reference's CSR with the reference's right-hand sides, and delEx,
+    delEy and delBz follow the reference's difference formulas on the device's
+    A, on a first call (A_prev = 0) and on a second one after a push."""
     say("[darwin]")
     ngx, ngy, dx, dt = 64, 64, 0.005, 5e-12
     fbc = ("Dirichlet", "Dirichlet", "periodic", "periodic")
@@ -528,13 +529,38 @@ def test_darwin_solves(gp, orc):
     rx, ry = orc.darwin_rhs_a(f, jx, jy, chi)
     check(rx, ax)
     check(ry, ay)
-    # delta fields from A (last A = 0 at the first call)
-    dEx = ctx.field(gp.F_DEL_EX)
-    AX = ax.reshape(nky + 1, nkx + 1)
-    nx = ngx + 2
-    exp = -(AX[:nky, :nkx]) / (CL * dt)
-    np.testing.assert_allclose(dEx[1:1 + nky, 2:2 + nkx], exp.astype(np.float32), rtol=1e-6,
-                               atol=1e-6 * np.abs(exp).max())
+    ngb = 1
+    dxf = float(np.float32(dx))
+
+    def check_delta(ax, ay, lax, lay):
+        """EMField.mm:1283-1297 on the node grid, i < nkx, j < nky: delEx at
+        Ex[j+ngb, i+ngb+1], delEy at Ey[j+ngb+2, i+ngb] (the reference's row
+        offset), delBz = dxAy - dyAx at Bz[j+ngb+1, i+ngb+1]; FP64, then FP32"""
+        AX, AY = ax.reshape(nky + 1, nkx + 1), ay.reshape(nky + 1, nkx + 1)
+        LX, LY = lax.reshape(nky + 1, nkx + 1), lay.reshape(nky + 1, nkx + 1)
+        got = ctx.field(gp.F_DEL_EX)[ngb:ngb + nky, ngb + 1:ngb + 1 + nkx]
+        exp = (-(AX[:nky, :nkx] - LX[:nky, :nkx]) / (CL * dt)).astype(np.float32)
+        np.testing.assert_allclose(got, exp, rtol=1e-6, atol=1e-6 * np.abs(exp).max())
+        got = ctx.field(gp.F_DEL_EY)[ngb + 2:ngb + 2 + nky, ngb:ngb + nkx]
+        exp = (-(AY[:nky, :nkx] - LY[:nky, :nkx]) / (CL * dt)).astype(np.float32)
+        np.testing.assert_allclose(got, exp, rtol=1e-6, atol=1e-6 * np.abs(exp).max())
+        got = ctx.field(gp.F_DEL_BZ)[ngb + 1:ngb + 1 + nky, ngb + 1:ngb + 1 + nkx]
+        exp = ((AY[:nky, 1:nkx + 1] - AY[:nky, :nkx]) / dxf - (AX[1:nky + 1, :nkx] - AX[:nky, :nkx]) / dxf)
+        exp = exp.astype(np.float32)
+        assert np.abs(exp).max() > 0
+        np.testing.assert_allclose(got, exp, rtol=1e-6, atol=1e-6 * np.abs(exp).max())
+
+    # first call: the previous A is zero
+    check_delta(ax, ay, np.zeros_like(ax), np.zeros_like(ay))
+    # second call after a step of the particles: delta E from A - A_prev
+    ctx.push(se, dt, deposit=1)
+    ctx.solve_fields(dt, tol=1e-8)
+    sts, rc = ctx.solve_darwin(dt, tol=1e-8, maxiter=400)
+    for st in sts:
+        assert st.relres <= 1e-8, [x.relres for x in sts]
+    ax2, ay2 = ctx.field(gp.F_AX).ravel(), ctx.field(gp.F_AY).ravel()
+    assert np.abs(ax2 - ax).max() > 1e-6 * np.abs(ax).max()
+    check_delta(ax2, ay2, ax, ay)
     ctx.close()
 
 

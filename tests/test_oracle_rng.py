@@ -31,9 +31,25 @@ def test_pcg32_kat(orc):
     assert got == KAT["pcg32_srandom_42_54"]
 
 
-def test_rand01_open_interval(orc):
-    assert orc.lib().gpo_rand01(0) == 2.0 ** -24                 # (0 + 1/2) * 2^-23
-    assert orc.lib().gpo_rand01(0xFFFFFFFF) == 1.0 - 2.0 ** -24  # never rounds to 1
+def test_rand01_is_the_reference_draw(orc):
+    """rand01 = float(u) * 2^-32 (src/Collision.mm:28-30): NumPy's uint32 ->
+    float32 conversion (round to nearest even) times 2^-32, for random u and
+    for the edges of the rounding, outside the two guarded cases."""
+    f = orc.lib().gpo_rand01
+    rng = np.random.default_rng(5)
+    top = 2 ** 32 - 128          # from here the conversion rounds to 2^32
+    us = np.concatenate([rng.integers(1, top, 20000, dtype=np.uint64),
+                         np.arange(1, 600, dtype=np.uint64),                  # exact small values
+                         np.arange(2 ** 24 - 8, 2 ** 24 + 40, dtype=np.uint64),  # first rounding
+                         np.arange(top - 400, top, dtype=np.uint64)]).astype(np.uint32)
+    exp = us.astype(np.float32) * np.float32(2.0 ** -32)
+    got = np.array([f(int(u)) for u in us], dtype=np.float32)
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    assert f(top - 129) == 1.0 - 2.0 ** -24      # 2^32 - 256 exactly: a reference value
+    # guarded: log(0), and the 128 values whose conversion rounds to 1.0
+    assert f(0) == 2.0 ** -33
+    for u in (top, top + 1, 2 ** 32 - 1):
+        assert f(u) == 1.0 - 2.0 ** -24
 
 
 def _gen(orc, gt=0, vth_e=1e8, vth_i=1e5, L=1.0):
