@@ -1,8 +1,7 @@
 // XmlLogger.cpp -- see XmlLogger.h.
 #include "XmlLogger.h"
 
-#include <iomanip>
-#include <sstream>
+#include <cstdio>
 #include <stdexcept>
 
 namespace gample {
@@ -10,10 +9,10 @@ namespace gample {
 XmlLogger::XmlLogger(const std::string &logName, const std::string &csvHeader)
 {
     xml_.open(logName + "_log.xml");
-    if (!xml_) throw std::runtime_error("Cannot open log file: " + logName);
+    if (!xml_) throw std::runtime_error("XmlLogger: " + logName + "_log.xml could not be created");
     xml_ << "<?xml version=\"1.0\" encoding=\"UTF-8\"?>\n<SimulationLog>\n";
     csv_.open(logName + "_timeseries.csv");
-    if (!csv_) throw std::runtime_error("Cannot open csv file: " + logName);
+    if (!csv_) throw std::runtime_error("XmlLogger: " + logName + "_timeseries.csv could not be created");
     csv_ << csvHeader << "\n";
 }
 
@@ -51,11 +50,12 @@ void XmlLogger::logCycleEnd()
 
 void XmlLogger::logComment(const std::string &text) { xml_ << "<![CDATA[\n" << text << "]]>\n"; }
 
+// d.ddde+XX with prec digits after the point (the log's number format)
 std::string fmtSci(float v, int prec)
 {
-    std::ostringstream o;
-    o << std::scientific << std::setprecision(prec) << v;
-    return o.str();
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.*e", prec, (double)v);
+    return std::string(buf);
 }
 
 }  // namespace gample

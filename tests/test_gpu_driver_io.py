@@ -140,3 +140,55 @@ def test_outputs_and_restart(tmp_path):
         for k in ("vx", "vy"):
             np.testing.assert_allclose(a[k][oa], b[k][ob], rtol=0, atol=1e-5 * np.abs(a[k]).max())
     assert not os.path.exists(os.path.join(tmp, "bin", "io2_ProgressData_" + names[0] + "_00000005.bin"))
+
+
+def test_moment_normalisation_hand_case(tmp_path):
+    """The driver's moment output (host/Modules.cpp Moment::integrateMoments,
+    FileIO outputMoments) against the normalisation of Moment.mm:275-316 on a
+    hand case restarted from a progress file: two electrons at one point
+    give, on every node of their stencil, u = (v1 + v2)/2 and
+    P_ab = m (<v_a v_b> - u_a u_b) n (written in Pa: x 0.1); a third electron
+    alone elsewhere gives u = v3 and P = 0."""
+    tmp = str(tmp_path)
+    os.makedirs(os.path.join(tmp, "bin"))
+    cfg, raw = condition(tmp, "mom", start=5, restart="hand")
+    raw["ParamForTimeIntegration"].update(End=6, FieldOutput=100, ProgressOutput=0, ParticleOutput=0)
+    json.dump(raw, open(cfg, "w"))
+    names = [p["ParticleName"] for p in raw["ParamForParticle"]]
+    v1, v2, v3 = (1.0e7, 2.0e7, -3.0e6), (-5.0e6, 4.0e6, 8.0e6), (2.5e6, -7.0e6, 1.5e6)
+    rec_t = np.dtype([("x", "<f4"), ("y", "<f4"), ("vx", "<f4"), ("vy", "<f4"), ("vz", "<f4"), ("flag", "<i4")])
+    e = np.zeros(3, rec_t)
+    e["x"], e["y"] = [20.25, 20.25, 45.5], [17.75, 17.75, 40.125]        # grid units incl. ngb = 1
+    for k, v in enumerate((v1, v2, v3)):
+        e["vx"][k], e["vy"][k], e["vz"][k] = v
+    i = np.zeros(1, rec_t)
+    i["x"], i["y"] = 8.5, 8.5
+    e.tofile(os.path.join(tmp, "bin", f"hand_ProgressData_{names[0]}_00000005.bin"))
+    i.tofile(os.path.join(tmp, "bin", f"hand_ProgressData_{names[1]}_00000005.bin"))
+    run(tmp, cfg)
+    mesh, m = read_fields(os.path.join(tmp, "bin", f"mom_Moments_{names[0]}_00000005.bin"))
+    get = lambda k: m[names[0] + "_" + k][1].astype(np.float64)
+    n = get("n")
+    dx = float(np.float32(raw["ParamForField"]["GridSizeOfX"]))
+    w = raw["ParamForParticle"][0]["WeightValue"] * dx * dx / PPC
+    mass = float(np.float32(raw["ParamForParticle"][0]["Mass"]))
+    assert n.sum() * dx * dx == pytest.approx(3 * w, rel=1e-5)
+    V = np.array([v1, v2], np.float64)
+    u_ab = V.mean(axis=0)
+    comps = ("x", "y", "z")
+    near = np.zeros_like(n, bool)
+    near[14:23, 17:26] = True                       # rows y, columns x around (20.25, 17.75)
+    for node in zip(*np.nonzero((n > 0) & near)):
+        for a in range(3):
+            assert get("u" + comps[a])[node] == pytest.approx(u_ab[a], rel=1e-5)
+            for b in range(a, 3):
+                p = mass * ((V[:, a] * V[:, b]).mean() - u_ab[a] * u_ab[b]) * n[node] * 0.1
+                got = get("P" + comps[a] + comps[b])[node]
+                assert got == pytest.approx(p, rel=1e-4, abs=1e-6 * mass * 4e14 * n[node] * 0.1)
+    far = (n > 0) & ~near
+    assert far.sum() >= 9                           # the third electron's stencil
+    for node in zip(*np.nonzero(far)):
+        for a in range(3):
+            assert get("u" + comps[a])[node] == pytest.approx(v3[a], rel=1e-5)
+            for b in range(a, 3):
+                assert abs(get("P" + comps[a] + comps[b])[node]) <= 1e-5 * mass * 1e14 * n[node]

@@ -147,7 +147,8 @@ def test_boris_pure_e(orc):
 def test_fast_vs_reference_order(orc, order, boris):
     """Kernel op order vs the reference's literal FP32 expression order: one
     step agrees to 1e-5 relative; cell indices agree except for particles
-    within 1e-5 of a cell face."""
+    within 1e-4 cell of a cell face (the two orders differ in the last bits,
+    which can move a particle that close to a face across it)."""
     ngx, ngy = 32, 24
     ngb = order // 2
     Ex, Ey, Bz = _fields(ngx, ngy, order, seed=order + boris)
