@@ -1108,14 +1108,14 @@ __global__ void k_moments(MomArgs a)
     wts<ORDER>(x - (float)i1, sx);
     wts<ORDER>(y - (float)j1, sy);
     const float v[3] = {a.vx[p] * a.inv_vnorm, a.vy[p] * a.inv_vnorm, a.vz[p] * a.inv_vnorm};
-    unsigned qx[NW], qy[NW];
-    for (int k = 0; k < NW; k++) { qx[k] = q23(sx[k]); qy[k] = q23(sy[k]); }
     const int pit = a.nx + 1;
     for (int j = 0; j < NW; j++)
         for (int i = 0; i < NW; i++) {
             const int64_t node = (i1 + i - NGB) + (int64_t)(j1 + j - NGB) * pit;
+            // the density takes the same FP32 node weight as the velocity
+            // sums, so that n v / n at a node of few particles is their mean
             const float s = sx[i] * sy[j];
-            atomicAdd(&a.acc[node], (unsigned long long)qx[i] * qy[j]);
+            atomicAdd(&a.acc[node], (unsigned long long)fx40(s));
             atomicAdd(&a.acc[1 * a.ng + node], (unsigned long long)fx40(s * v[0]));
             atomicAdd(&a.acc[2 * a.ng + node], (unsigned long long)fx40(s * v[1]));
             atomicAdd(&a.acc[3 * a.ng + node], (unsigned long long)fx40(s * v[2]));
@@ -1548,7 +1548,7 @@ int gp_launch_moments(gp_ctx *c, int sid, double *host10)
         GP_HIP(hipGetLastError());
     }
     // Moment.mm:242 scale = w/(dx*dy) on n; velocity sums are unscaled
-    const double sn = ldexp(1.0, -GP_RHO_FRAC) * ((double)S.wf / ((double)c->dxf * (double)c->dyf));
+    const double sn = ldexp(1.0, -GP_CUR_FRAC) * ((double)S.wf / ((double)c->dxf * (double)c->dyf));
     const double sv = ldexp(1.0, -GP_CUR_FRAC) * S.vnorm;
     const double svv = ldexp(1.0, -GP_CUR_FRAC) * S.vnorm * S.vnorm;
     k_mom_scale<<<gp_cdiv(10 * ng, GP_BLOCK), GP_BLOCK, 0, s>>>(acc, ng, sn, sv, svv, out);
