@@ -88,7 +88,7 @@ EXPORTS = [
     "gp_ctx_create", "gp_ctx_destroy", "gp_last_error", "gp_sync", "gp_dims",
     "gp_species_create", "gp_species_upload", "gp_species_download", "gp_species_count",
     "gp_species_grid_get", "gp_species_read", "gp_load_pair", "gp_generate", "gp_set_cathode", "gp_inject_wall", "gp_reset_grids", "gp_deposit",
-    "gp_solve_fields", "gp_solve_darwin", "gp_darwin_stats", "gp_push", "gp_reduce", "gp_sort", "gp_set_sort_drift", "gp_sort_stats", "gp_moments",
+    "gp_solve_fields", "gp_solve_darwin", "gp_darwin_stats", "gp_push", "gp_reduce", "gp_reduce_noalloc", "gp_sort", "gp_set_sort_drift", "gp_sort_stats", "gp_moments",
     "gp_step", "gp_field_get", "gp_field_set", "gp_field_energy", "gp_timing", "gp_kernel_time",
     "gp_nn_init", "gp_nn_train", "gp_nn_infer", "gp_nn_save", "gp_nn_load", "gp_nn_add_pair", "gp_nn_params",
     "gp_nn_dims", "gp_nn_forward", "gp_nn_grad", "gp_solve_rhs", "gp_nn_config",
@@ -128,6 +128,7 @@ def lib() -> C.CDLL:
         L.gp_darwin_stats.argtypes = [P, C.POINTER(SolveStats)]
         L.gp_push.argtypes = [P, I, D, I]
         L.gp_reduce.argtypes = [P, I, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.gp_reduce_noalloc.argtypes = [P, I, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         L.gp_sort.argtypes = [P, I]
         L.gp_set_sort_drift.argtypes = [P, D]
         L.gp_sort_stats.argtypes = [P, I, C.POINTER(C.c_int64)]
@@ -307,6 +308,13 @@ class Context:
         cnt = (C.c_int64 * 4)()
         ke = C.c_double()
         _check(lib().gp_reduce(self.h, sid, cnt, C.byref(ke)))
+        return list(cnt), ke.value
+
+    def reduce_noalloc(self, sid):
+        """gp_reduce with its scratch in the library's static device array"""
+        cnt = (C.c_int64 * 4)()
+        ke = C.c_double()
+        _check(lib().gp_reduce_noalloc(self.h, sid, cnt, C.byref(ke)))
         return list(cnt), ke.value
 
     def sort(self, sid):

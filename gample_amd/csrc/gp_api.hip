@@ -95,6 +95,9 @@ int gp_ctx_create(const gp_config *cfg, gp_ctx **out)
         if (e != hipSuccess) { delete c; return gp_fail(GP_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e)); }
         e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
         if (e != hipSuccess) { delete c; return gp_fail(GP_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+        // this device's copy of the static reduce scratch (gp_reduce_noalloc)
+        int rs = gp_noalloc_setup(k.device);
+        if (rs) { gp_ctx_destroy(c); return rs; }
         int rc = gp_fields_init(c);
         if (rc) { gp_ctx_destroy(c); return rc; }
         e = hipHostMalloc((void **)&c->h_drift, 3 * sizeof(unsigned long long) * GP_MAX_DRIFT_SPECIES);
@@ -444,6 +447,17 @@ int gp_reduce(gp_ctx *c, int sid, int64_t counts[4], double *kinetic)
     GP_CHECK_SID(c, sid);
     int64_t tmp[4];
     return gp_launch_reduce(c, sid, counts ? counts : tmp, kinetic);
+}
+
+int gp_reduce_noalloc(gp_ctx *c, int sid, int64_t counts[4], double *kinetic)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    gp_arena ar;
+    const int rc = gp_noalloc_arena(c->cfg.device, &ar);
+    if (rc) return rc;
+    int64_t tmp[4];
+    return gp_launch_reduce(c, sid, counts ? counts : tmp, kinetic, &ar);
 }
 
 int gp_sort(gp_ctx *c, int sid)

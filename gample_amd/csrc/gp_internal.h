@@ -39,6 +39,24 @@ int gp_fail(int code, const std::string &msg);
             return gp_fail(GP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+static inline size_t gp_align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bump allocator over a caller-provided device buffer: the scratch of the
+// allocation-free entry points.  take() returns null when the request does
+// not fit; a user restores `used` to release what it took.
+struct gp_arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    void *take(size_t bytes)
+    {
+        const size_t b = gp_align256(bytes);
+        if (b > cap - used) return nullptr;
+        void *p = base + used;
+        used += b;
+        return p;
+    }
+};
+
 struct gp_soa {
     float *x = nullptr, *y = nullptr, *vx = nullptr, *vy = nullptr, *vz = nullptr;
 };
@@ -171,7 +189,10 @@ struct gp_ctx {
 // ---- particle side (gp_particles.hip) ----
 int gp_launch_push(gp_ctx *c, int sid, double dt, int deposit);
 int gp_launch_deposit(gp_ctx *c, int sid, int64_t first, int64_t count);
-int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke);
+int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke, gp_arena *ar = nullptr);
+// the device's static reduce scratch (set up once per device by gp_ctx_create)
+int gp_noalloc_setup(int device);
+int gp_noalloc_arena(int device, gp_arena *ar);
 int gp_launch_sort(gp_ctx *c, int sid);
 int gp_launch_load(gp_ctx *c, int se, int si, const gp_gen *g, uint64_t seed, int64_t first_id, int64_t n);
 int gp_launch_generate(gp_ctx *c, int se, int si, const gp_gen *g, int64_t addn, int deposit);
@@ -179,7 +200,8 @@ int gp_launch_inject_wall(gp_ctx *c, int sid, int64_t n, float vth, const float 
                           uint64_t seed, uint64_t stream, int deposit);
 int gp_launch_moments(gp_ctx *c, int sid, double *host10);
 int gp_radix_sort_u32(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp,
-                      int64_t n, int bits, int init_vals, uint32_t **keys_out, uint32_t **vals_out);
+                      int64_t n, int bits, int init_vals, uint32_t **keys_out, uint32_t **vals_out,
+                      gp_arena *ar = nullptr);
 
 // ---- field side (gp_fields.hip) ----
 int gp_fields_init(gp_ctx *c);

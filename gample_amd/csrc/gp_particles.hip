@@ -22,6 +22,7 @@
 #include "gp_internal.h"
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 namespace {
 
@@ -645,7 +646,17 @@ __global__ void k_scan_add(uint32_t *d, int64_t n, const uint32_t *sums)
     }
 }
 
-int scan_u32(hipStream_t s, uint32_t *d, int64_t n)
+// block sums of a scan of n values: one uint32 per block, recursively
+size_t scan_scratch_bytes(int64_t n)
+{
+    const int64_t nb = n > 0 ? (n + SC_TILE - 1) / SC_TILE : 0;
+    if (nb <= 1) return 0;
+    return gp_align256(sizeof(uint32_t) * (size_t)nb) + scan_scratch_bytes(nb);
+}
+
+// Exclusive scan in place.  The block sums come from the arena when one is
+// given (and are released to it on return), else from the stream allocator.
+int scan_u32(hipStream_t s, uint32_t *d, int64_t n, gp_arena *ar)
 {
     if (n <= 0) return GP_OK;
     const int nb = gp_cdiv(n, SC_TILE);
@@ -655,14 +666,21 @@ int scan_u32(hipStream_t s, uint32_t *d, int64_t n)
         return GP_OK;
     }
     uint32_t *sums = nullptr;
-    GP_HIP(hipMallocAsync((void **)&sums, sizeof(uint32_t) * nb, s));
+    const size_t mark = ar ? ar->used : 0;
+    if (ar) {
+        sums = (uint32_t *)ar->take(sizeof(uint32_t) * nb);
+        if (!sums) return gp_fail(GP_ERR_CAPACITY, "scan: scratch arena exhausted");
+    } else {
+        GP_HIP(hipMallocAsync((void **)&sums, sizeof(uint32_t) * nb, s));
+    }
     k_scan_block<<<nb, GP_BLOCK, 0, s>>>(d, n, sums);
     GP_HIP(hipGetLastError());
-    int rc = scan_u32(s, sums, nb);
+    int rc = scan_u32(s, sums, nb, ar);
     if (rc) return rc;
     k_scan_add<<<nb, GP_BLOCK, 0, s>>>(d, n, sums);
     GP_HIP(hipGetLastError());
-    GP_HIP(hipFreeAsync(sums, s));
+    if (ar) ar->used = mark;
+    else GP_HIP(hipFreeAsync(sums, s));
     return GP_OK;
 }
 
@@ -1253,24 +1271,47 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
 // ===========================================================================
 // host entry points used by gp_api.hip
 // ===========================================================================
+// digit width of a sort of `bits`-bit keys: the fewest passes of at most 9
+// bits, 8-bit digits when they suffice
+static int rs_digit(int bits, int *passes)
+{
+    if (bits < 1) bits = 1;
+    *passes = (bits + 8) / 9;
+    return bits <= 8 * *passes ? 8 : 9;
+}
+
+// scratch of a radix sort of n keys: the digit histograms and their scan
+size_t radix_scratch_bytes(int64_t n, int bits)
+{
+    if (n <= 0) return 0;
+    int passes;
+    const int64_t hl = ((int64_t)1 << rs_digit(bits, &passes)) * gp_cdiv(n, RS_TILE);
+    return gp_align256(sizeof(uint32_t) * (size_t)hl) + scan_scratch_bytes(hl);
+}
+
 int gp_radix_sort_u32(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *keys_tmp, uint32_t *vals_tmp,
-                      int64_t n, int bits, int init_vals, uint32_t **keys_out, uint32_t **vals_out)
+                      int64_t n, int bits, int init_vals, uint32_t **keys_out, uint32_t **vals_out, gp_arena *ar)
 {
     uint32_t *ki = keys, *vi = vals, *ko = keys_tmp, *vo = vals_tmp;
     if (n <= 0) { *keys_out = ki; *vals_out = vi; return GP_OK; }
     const int nblk = gp_cdiv(n, RS_TILE);
-    // fewest passes of at most 9 bits; 8-bit digits when they suffice
-    if (bits < 1) bits = 1;
-    const int passes = (bits + 8) / 9;
-    const int dig = bits <= 8 * passes ? 8 : 9;
+    int passes;
+    const int dig = rs_digit(bits, &passes);
     uint32_t *hist = nullptr;
-    GP_HIP(hipMallocAsync((void **)&hist, sizeof(uint32_t) * ((size_t)1 << dig) * (size_t)nblk, s));
+    const size_t hbytes = sizeof(uint32_t) * ((size_t)1 << dig) * (size_t)nblk;
+    const size_t mark = ar ? ar->used : 0;
+    if (ar) {
+        hist = (uint32_t *)ar->take(hbytes);
+        if (!hist) return gp_fail(GP_ERR_CAPACITY, "radix sort: scratch arena exhausted");
+    } else {
+        GP_HIP(hipMallocAsync((void **)&hist, hbytes, s));
+    }
     for (int ps = 0; ps < passes; ps++) {
         const int shift = dig * ps;
         if (dig == 8) k_rs_hist<8><<<nblk, GP_BLOCK, 0, s>>>(ki, n, shift, hist, nblk);
         else k_rs_hist<9><<<nblk, GP_BLOCK, 0, s>>>(ki, n, shift, hist, nblk);
         GP_HIP(hipGetLastError());
-        int rc = scan_u32(s, hist, ((int64_t)1 << dig) * (int64_t)nblk);
+        int rc = scan_u32(s, hist, ((int64_t)1 << dig) * (int64_t)nblk, ar);
         if (rc) return rc;
         if (dig == 8)
             k_rs_scatter<8><<<nblk, GP_BLOCK, 0, s>>>(ki, vi, ko, vo, n, shift, (ps == 0) ? init_vals : 0, hist, nblk);
@@ -1280,7 +1321,8 @@ int gp_radix_sort_u32(hipStream_t s, uint32_t *keys, uint32_t *vals, uint32_t *k
         std::swap(ki, ko);
         std::swap(vi, vo);
     }
-    GP_HIP(hipFreeAsync(hist, s));
+    if (ar) ar->used = mark;
+    else GP_HIP(hipFreeAsync(hist, s));
     *keys_out = ki;
     *vals_out = vi;
     return GP_OK;
@@ -1303,7 +1345,15 @@ int gp_launch_deposit(gp_ctx *c, int sid, int64_t first, int64_t count)
     return run_particle_kernel(c, sid, 0.0, dep, false, first, count);
 }
 
-int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke)
+// key bits of the absorbed-index sort of a species of n particles
+static int dead_bits(int64_t n)
+{
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) < n) bits++;
+    return bits;
+}
+
+int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke, gp_arena *ar)
 {
     gp_species &S = c->sp[sid];
     hipStream_t s = c->stream;
@@ -1312,6 +1362,15 @@ int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke)
     GP_HIP(hipStreamSynchronize(s));
     if (h[5]) return gp_fail(GP_ERR_STATE, "particles outside the domain with no boundary condition: " + std::to_string(h[5]));
     const int64_t nd = (int64_t)h[0];
+    if (ar && nd > 0) {
+        // the arena must hold the larger of the two phases' scratch (the
+        // sort's is released before the scan of the tail flags)
+        const size_t need = std::max(radix_scratch_bytes(nd, dead_bits(S.n)), scan_scratch_bytes(nd));
+        if (need > ar->cap - ar->used)
+            return gp_fail(GP_ERR_CAPACITY, "gp_reduce_noalloc: " + std::to_string(nd) + " absorbed particles need " +
+                                                std::to_string(need) + " B of scratch, more than the " +
+                                                std::to_string(ar->cap - ar->used) + " B it has");
+    }
     for (int w = 0; w < 4; w++) counts[w] = (int64_t)h[1 + w];
     if (ke) {
         GP_HIP(hipMemcpyAsync(ke, S.ke_out, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1320,10 +1379,8 @@ int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke)
         if (nd > S.n) return gp_fail(GP_ERR_STATE, "absorbed count exceeds particle count");
         const int64_t n_new = S.n - nd;
         // sort the absorbed indices ascending
-        int bits = 1;
-        while (bits < 32 && ((int64_t)1 << bits) < S.n) bits++;
         uint32_t *ks = nullptr, *vs = nullptr;
-        int rc = gp_radix_sort_u32(s, S.dead, S.val, S.dead_tmp, S.val2, nd, bits, 1, &ks, &vs);
+        int rc = gp_radix_sort_u32(s, S.dead, S.val, S.dead_tmp, S.val2, nd, dead_bits(S.n), 1, &ks, &vs, ar);
         if (rc) return rc;
         uint32_t *nh = S.key2;         // device scalar: number of holes
         uint32_t *flags = S.key;       // tail flags -> positions (nd)
@@ -1333,7 +1390,7 @@ int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke)
         const int g = gp_cdiv(nd, GP_BLOCK);
         k_tail_flags<<<g, GP_BLOCK, 0, s>>>(S.a.x, n_new, nd, flags);
         GP_HIP(hipGetLastError());
-        rc = scan_u32(s, flags, nd);
+        rc = scan_u32(s, flags, nd, ar);
         if (rc) return rc;
         k_donors<<<g, GP_BLOCK, 0, s>>>(S.a.x, n_new, nd, flags, donor);
         GP_HIP(hipGetLastError());
@@ -1568,5 +1625,51 @@ int gp_launch_ke(gp_ctx *c, int sid)
     GP_HIP(hipGetLastError());
     k_sum_partials<<<1, GP_BLOCK, 0, c->stream>>>(S.ke_part, g, S.ke_out);
     GP_HIP(hipGetLastError());
+    return GP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Static scratch of gp_reduce_noalloc: the radix histograms and scan block
+// sums of the compaction, for callers that cannot allocate device memory.
+// Each device holds its own copy for the life of the process; host code
+// reaches it only through the address hipGetSymbolAddress returned for that
+// device (the array's name is a host-side stand-in).  Every call on a device
+// shares the array: calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+#define GP_NOALLOC_SCRATCH (12u << 20)
+static __device__ __align__(256) unsigned char scratch[GP_NOALLOC_SCRATCH];
+
+namespace {
+constexpr int GP_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[GP_MAX_DEVICES] = {};
+}  // namespace
+
+// resolve the device's copy of the scratch array, with that device current
+int gp_noalloc_setup(int device)
+{
+    if (device < 0 || device >= GP_MAX_DEVICES) return gp_fail(GP_ERR_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return GP_OK;
+    int cur = -1;
+    GP_HIP(hipGetDevice(&cur));
+    if (cur != device) GP_HIP(hipSetDevice(device));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !ptr)
+        return gp_fail(GP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return GP_OK;
+}
+
+int gp_noalloc_arena(int device, gp_arena *ar)
+{
+    if (device < 0 || device >= GP_MAX_DEVICES) return gp_fail(GP_ERR_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) return gp_fail(GP_ERR_STATE, "no reduce scratch on this device (gp_ctx_create sets it up)");
+    ar->base = (char *)g_scratch_ptr[device];
+    ar->cap = GP_NOALLOC_SCRATCH;
+    ar->used = 0;
     return GP_OK;
 }

@@ -237,6 +237,18 @@ int  gp_push(gp_ctx *ctx, int sid, double dt, int deposit);
 /* -[Particle reduce:] (Particle.mm:893-1002): compacts absorbed particles
  * (holes filled from the tail), per-wall counts, FP64 kinetic energy. */
 int  gp_reduce(gp_ctx *ctx, int sid, int64_t counts[4], double *kinetic);
+/* gp_reduce without any device allocation (no hipMalloc, hipFree or
+ * allocating library call): the radix histograms and scan block sums of the
+ * compaction live in a static 12 MiB device array that each GPU holds for
+ * the life of the process.  No counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation).  Same results as
+ * gp_reduce.  Fails with GP_ERR_CAPACITY, changing nothing, when the
+ * absorbed particles of this call need more scratch than the array holds
+ * (at most 2^24 absorbed particles per call are always accepted).  Every
+ * call on one GPU shares its array: calls that overlap on one GPU (other
+ * contexts' streams, other host threads) race. */
+int  gp_reduce_noalloc(gp_ctx *ctx, int sid, int64_t counts[4], double *kinetic);
 /* cell/tile sort of the SoA store by a GPU LSD radix sort (no reference
  * counterpart: the reference keeps insertion order).  Particles are grouped
  * by 16x16-cell tile; inside a tile they are interleaved over the 64 LDS
