@@ -188,7 +188,9 @@ struct PushArgs {
     double ke_coef;
 };
 
-template <int ORDER, bool UNIB, int DEP, bool PUSH>
+// BCK: 0 any particle boundary codes (read from the arguments), 1 periodic
+// on all four sides (the wrap only: no absorbed or reflected particles)
+template <int ORDER, bool UNIB, int DEP, bool PUSH, int BCK>
 __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
 {
     constexpr int PB = push_block(ORDER);
@@ -353,16 +355,21 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
             x = __builtin_fmaf(vx, a.constX, x);
             y = __builtin_fmaf(vy, a.constY, y);
             // particle BCs (Particle.mm:215-269)
-            if (a.bc0 == GP_PBC_PERIODIC) x = wrap(x, a.ngx, a.ngb);
-            if (a.bc2 == GP_PBC_PERIODIC) y = wrap(y, a.ngy, a.ngb);
-            if (a.bc0 == GP_PBC_DELETE && x < lox) flag = 1;
-            if (a.bc1 == GP_PBC_DELETE && x >= hix) flag = 2;
-            if (a.bc2 == GP_PBC_DELETE && y < loy) flag = 3;
-            if (a.bc3 == GP_PBC_DELETE && y >= hiy) flag = 4;
-            if (a.bc0 == GP_PBC_REFLECT && x < lox) { x = lox + lox - x; vx = -vx; }
-            if (a.bc1 == GP_PBC_REFLECT && x >= hix) { x = hix + hix - x; vx = -vx; }
-            if (a.bc2 == GP_PBC_REFLECT && y < loy) { y = loy + loy - y; vy = -vy; }
-            if (a.bc3 == GP_PBC_REFLECT && y >= hiy) { y = hiy + hiy - y; vy = -vy; }
+            if constexpr (BCK == 1) {
+                x = wrap(x, a.ngx, a.ngb);
+                y = wrap(y, a.ngy, a.ngb);
+            } else {
+                if (a.bc0 == GP_PBC_PERIODIC) x = wrap(x, a.ngx, a.ngb);
+                if (a.bc2 == GP_PBC_PERIODIC) y = wrap(y, a.ngy, a.ngb);
+                if (a.bc0 == GP_PBC_DELETE && x < lox) flag = 1;
+                if (a.bc1 == GP_PBC_DELETE && x >= hix) flag = 2;
+                if (a.bc2 == GP_PBC_DELETE && y < loy) flag = 3;
+                if (a.bc3 == GP_PBC_DELETE && y >= hiy) flag = 4;
+                if (a.bc0 == GP_PBC_REFLECT && x < lox) { x = lox + lox - x; vx = -vx; }
+                if (a.bc1 == GP_PBC_REFLECT && x >= hix) { x = hix + hix - x; vx = -vx; }
+                if (a.bc2 == GP_PBC_REFLECT && y < loy) { y = loy + loy - y; vy = -vy; }
+                if (a.bc3 == GP_PBC_REFLECT && y >= hiy) { y = hiy + hiy - y; vy = -vy; }
+            }
             // a wrapped or reflected position can round onto the upper wall
             if (!flag) {
                 if (x >= hix) x = __int_as_float(__float_as_int(hix) - 1);
@@ -1158,7 +1165,10 @@ __global__ void k_mom_scale(const unsigned long long *acc, int64_t ng, double sn
 template <int ORDER, bool UNIB, int DEP, bool PUSH>
 void launch_push_t(int grid, hipStream_t s, const PushArgs &a)
 {
-    k_push<ORDER, UNIB, DEP, PUSH><<<grid, push_block(ORDER), 0, s>>>(a);
+    const bool all_periodic = a.bc0 == GP_PBC_PERIODIC && a.bc1 == GP_PBC_PERIODIC && a.bc2 == GP_PBC_PERIODIC &&
+                              a.bc3 == GP_PBC_PERIODIC;
+    if (PUSH && all_periodic) k_push<ORDER, UNIB, DEP, PUSH, 1><<<grid, push_block(ORDER), 0, s>>>(a);
+    else k_push<ORDER, UNIB, DEP, PUSH, 0><<<grid, push_block(ORDER), 0, s>>>(a);
 }
 template <int ORDER>
 int launch_push_o(int grid, hipStream_t s, const PushArgs &a, bool unib, int dep, bool push)
