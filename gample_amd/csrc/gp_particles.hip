@@ -533,12 +533,16 @@ __global__ __launch_bounds__(GP_BLOCK) void k_stage_reduce(const unsigned long l
     // and is read, then masked out)
     constexpr int MT = (W + GP_TILE - 1) / GP_TILE;
     unsigned long long sum = 0;
+    // (an image position beyond every window, the usual case away from the
+    // periodic edges, is skipped before its loads are set up)
     for (int b = 0; b < npy; b++) {
         const int y = py[b];
         const int tyl = max(fdiv16(y - c0 - W) + 1, 0), tyh = min(fdiv16(y - c0), nty - 1);
+        if (tyl > tyh) continue;
         for (int a = 0; a < npx; a++) {
             const int x = px[a];
             const int txl = max(fdiv16(x - c0 - W) + 1, 0), txh = min(fdiv16(x - c0), ntx - 1);
+            if (txl > txh) continue;
             unsigned long long v[MT * MT];
             bool use[MT * MT];
 #pragma unroll
