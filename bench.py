@@ -102,7 +102,7 @@ def gpu_run(args):
     cyc += cycle_rep.pop("steps_run")
     _, dg, _ = ctx.solve_fields(dt, tol=tol, diag=True)
     nn = ctx.count(se) + ctx.count(si)
-    nnrep = None if args.no_nn else nn_poisson(ctx, args, dt, tol, cyc)
+    nnrep = None if args.no_nn else nn_poisson(ctx, args, dt, tol, cyc, float(np.float32(P["dx"])))
     ctx.close()
     return dict(n_total=nn, seconds=t1 - t0, push_ms=push_ms, push_n=push_n, sort_ms=sort_ms, sort_n=sort_n,
                 solve_ms=solve_ms, solve_n=solve_n, iters=iters, P=P, n_species=n, nn=nnrep, sorts=sorts,
@@ -145,20 +145,34 @@ def sort_cycle(ctx, se, dt, tol, sort_every, cyc, max_steps):
                     "cycle, with its sort, solves and pushes"}
 
 
-def nn_poisson(ctx, args, dt, tol, cyc):
-    """Learned Poisson warm start / preconditioner on the workload's own grid:
-    train on (b, x) pairs from further steps of the run plus random power-law
-    right-hand sides, then solve held-out fields of the run cold (x0 = 0) with
-    MG-CG and with each network mode, and warm (previous solution) with
-    MG-CG as the time loop does."""
-    from gample_amd.nn_data import add_random_pairs
+def nn_poisson(ctx, args, dt, tol, cyc, h):
+    """Learned warm start on the time loop's own increments.  After the timed
+    steps the run continues; step n's right-hand side b_n and solution x_n
+    give the pair (b_n - b_{n-1}, x_n - x_{n-1}): the first residual of a solve
+    warm-started from x_{n-1}, and the correction it needs.  The network is
+    trained on these pairs with its relative energy-norm loss; then held-out
+    steps are solved from x_{n-1} with MG-CG and with each network mode, and
+    from zero with MG-CG.  Also reported, on the held-out increments: the
+    network's relative energy-norm error (its output at the S-optimal scale)
+    and the residual it leaves at the residual-optimal scale, against the
+    warm start's."""
+    import gample_amd as gp
     t0 = time.perf_counter()
-    ctx.nn_init(32, seed=1)
-    for _ in range(args.nn_sim_pairs):
+    npair, nev = args.nn_pairs, args.nn_eval_fields
+    my, mx = ctx.field_shape(gp.F_SOL)[0]
+    db = np.empty((npair + nev, my * mx)); dx = np.empty_like(db)
+    bprev = xprev = None
+    for k in range(-1, npair + nev):
         cyc += 1
         ctx.step(dt, tol=tol, maxiter=400, sort_interval=args.sort_every, cycle=cyc)
-        ctx.nn_add_pair()
-    add_random_pairs(ctx, args.nn_random_pairs, np.random.default_rng(7), True, True, tol=tol)
+        b, x = ctx.field(gp.F_RHS).ravel().copy(), ctx.field(gp.F_SOL).ravel().copy()
+        if k >= 0:
+            db[k] = b - bprev
+            dx[k] = x - xprev
+        bprev, xprev = b, x
+    ctx.nn_init(32, seed=1)
+    ctx.nn_config(1, 2, 0)
+    ctx.nn_train(epochs=0, rho=db[:npair], phi=dx[:npair])
     t1 = time.perf_counter()
     loss0 = ctx.nn_train(epochs=1, batch=4, lr=0.0)
     epochs, loss = 0, loss0
@@ -166,25 +180,42 @@ def nn_poisson(ctx, args, dt, tol, cyc):
         loss = ctx.nn_train(epochs=1, batch=4, lr=1e-3)
         epochs += 1
     t2 = time.perf_counter()
-    import gample_amd as gp
-    rows = {k: [] for k in ("mg_cold", "mg_warm", "nn_warm_fcg", "nn_warm_mg", "nn_fcg")}
-    for _ in range(args.nn_eval_fields):
-        xprev = ctx.field(gp.F_SOL).ravel().copy()
-        cyc += 1
-        ctx.step(dt, tol=tol, maxiter=400, sort_interval=args.sort_every, cycle=cyc)
-        b = ctx.field(gp.F_RHS).ravel().copy()
-        for key, solver, x0 in (("mg_cold", 0, None), ("mg_warm", 0, xprev), ("nn_warm_fcg", 1, None),
-                                ("nn_warm_mg", 2, None), ("nn_fcg", 3, None)):
+
+    def S(v):      # the solver's operator on the periodic unknown grid
+        u = v.reshape(my, mx)
+        return ((4 * u - np.roll(u, 1, 0) - np.roll(u, -1, 0) - np.roll(u, 1, 1) - np.roll(u, -1, 1)) / h ** 2).ravel()
+
+    rows = {k: [] for k in ("mg_cold", "mg_warm", "nn_warm_mg", "nn_warm_fcg")}
+    err, res_warm, res_nn = [], [], []
+    # held-out steps: b_n = b_{n-1} + db, x_{n-1} = x_n - dx, counted back from the last step
+    b, x = bprev, xprev
+    for k in range(npair + nev - 1, npair - 1, -1):
+        bp, xp = b - db[k], x - dx[k]
+        for key, solver, x0 in (("mg_cold", 0, None), ("mg_warm", 0, xp), ("nn_warm_mg", 2, xp),
+                                ("nn_warm_fcg", 1, xp)):
             _, st, _ = ctx.solve_rhs(b, x0=x0, tol=tol, maxiter=400, solver=solver)
             rows[key].append((st.iterations, 1e3 * st.seconds, st.relres, st.fell_back))
-    out = {"train_pairs": args.nn_sim_pairs + args.nn_random_pairs, "harvest_s": t1 - t0, "train_s": t2 - t1,
-           "epochs": epochs, "loss_untrained": loss0, "loss_trained": loss,
+        z = ctx.nn_infer(db[k])
+        sz = S(z)
+        a = (z @ db[k]) / (z @ sz)            # S dx = db: the S-optimal multiple of z
+        e = dx[k] - a * z
+        err.append(np.sqrt((e @ S(e)) / (dx[k] @ S(dx[k]))))
+        ar = (db[k] @ sz) / (sz @ sz)
+        nb = np.linalg.norm(b)
+        res_warm.append(np.linalg.norm(db[k]) / nb)
+        res_nn.append(np.linalg.norm(db[k] - ar * sz) / nb)
+        b, x = bp, xp
+    out = {"train_pairs": npair, "pairs": "time-loop increments (b_n - b_{n-1}, x_n - x_{n-1})",
+           "harvest_s": t1 - t0, "train_s": t2 - t1, "epochs": epochs, "loss_untrained": loss0, "loss_trained": loss,
            "network": "4-level conv encoder-decoder 32-64-128-128, 3x3, BF16 MFMA / FP32 accumulate",
-           "fields": args.nn_eval_fields, "tol": tol}
+           "fields": nev, "tol": tol,
+           "heldout_energy_error": float(np.mean(err)), "heldout_relres_warm_start": float(np.mean(res_warm)),
+           "heldout_relres_after_network": float(np.mean(res_nn))}
     for k, v in rows.items():
         a = np.array(v, dtype=float)
         out[k] = {"iterations": float(a[:, 0].mean()), "ms": float(a[:, 1].mean()),
                   "max_relres": float(a[:, 2].max()), "fell_back": int(a[:, 3].sum())}
+    out["nn_warm_mg_over_mg_warm_time"] = out["nn_warm_mg"]["ms"] / out["mg_warm"]["ms"]
     out["nn_warm_fcg_over_mg_cold_time"] = out["nn_warm_fcg"]["ms"] / out["mg_cold"]["ms"]
     return out
 
@@ -286,11 +317,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-nn", action="store_true", help="skip the learned Poisson preconditioner report")
-    ap.add_argument("--nn-sim-pairs", type=int, default=16)
-    ap.add_argument("--nn-random-pairs", type=int, default=48)
+    ap.add_argument("--nn-pairs", type=int, default=150, help="time-loop increments to train the network on")
     ap.add_argument("--nn-train-seconds", type=float, default=30.0)
-    ap.add_argument("--nn-max-epochs", type=int, default=60)
-    ap.add_argument("--nn-eval-fields", type=int, default=3)
+    ap.add_argument("--nn-max-epochs", type=int, default=15)
+    ap.add_argument("--nn-eval-fields", type=int, default=4)
     ap.add_argument("--cycle-max-steps", type=int, default=60,
                     help="steps allowed to reach each of the two electron sorts that bound the sort cycle")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
