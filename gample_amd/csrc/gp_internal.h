@@ -119,8 +119,7 @@ struct gp_solver {
     float *fd_vx = nullptr, *fd_vy = nullptr, *fd_inv = nullptr;
     double *hscal = nullptr;   // pinned host mirror
     int red_blocks = 0;
-    int last_it = 4;           // inner iterations of the previous solve
-    int last_it_r[4] = {4, 4, 4, 4};   // the same per refinement round (first batch sizes)
+    int last_it = 4;           // MG-PCG iterations of the previous solve (its first batch)
     int mg_ref_it = 0;         // iterations of the last plain MG-PCG solve
 };
 

@@ -17,10 +17,12 @@
 // locally) and k_mg_up (prolongation and the post-sweeps with a 2 NSW-wide
 // halo).  Recomputing the halo reproduces global red-black Gauss-Seidel
 // exactly.  All levels with at
-// most 4096 unknowns run inside one workgroup in LDS (k_mg_bottom).
+// most 8192 unknowns run inside one workgroup in LDS (k_mg_bottom).
 //
-// Outer loop in FP64: r = b - S x, stop at ||r||/||b|| <= tol; inner
-// correction in FP32 by MG-PCG (dot products accumulated in FP64), x += e.
+// CG in mixed precision: the iterate in FP64, the residual recursion, the
+// V-cycle and the search directions in FP32 (dot products accumulated in
+// FP64), with the recursive residual replaced by the true FP64 one every few
+// iterations; the solve stops at ||b - S x|| / ||b|| <= tol, tested in FP64.
 #include "gp_internal.h"
 #include <algorithm>
 #include <chrono>
@@ -35,17 +37,19 @@ constexpr int HU = 2, WU = TB + 2 * HU;   // window of k_rb_sweep (one full swee
 #define MG_SWEEPS 1
 #endif
 constexpr int NSW = MG_SWEEPS;            // smoothing sweeps per side of the V-cycle
-// smallest reduction asked of one FP32 inner solve (its own rounding limits it)
+// smallest reduction asked of one FP32 flexible-CG solve inside the FP64
+// refinement of the network solvers (its own rounding limits it)
 #ifndef MG_INNER_FLOOR
 #define MG_INNER_FLOOR 2e-6
 #endif
-// most FP32 iterations of one inner solve: where FP32 cannot reach the
-// inner target (a correction dominated by the operator's smoothest modes
-// carries rounding errors of ~eps32 cond(S) into the residual), the FP64
-// refinement restarts it from the true residual instead of letting the
-// recurrence drift
-#ifndef MG_INNER_CAP
-#define MG_INNER_CAP 20
+// MG-PCG: iterations between replacements of the recursive residual by the
+// true one, and the recursion's target as a fraction of tol (the true
+// residual is tested after the recursion meets it)
+#ifndef MG_REPLACE
+#define MG_REPLACE 4
+#endif
+#ifndef MG_TARGET
+#define MG_TARGET 0.8
 #endif
 #ifndef MG_BY
 #define MG_BY 32
@@ -151,7 +155,10 @@ __device__ __forceinline__ Lev fixed(Lev L)
     return L;
 }
 
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_TMP, S_BB, S_MEAN, S_DONE, S_TGT, S_IT, S_MAXIT, S_RZ2, S_NS };
+// device scalars; S_SC scales the true residual to unit RMS for the FP32
+// recursion (S_ISC = 1 / S_SC), S_TRR is the true residual's r.r
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_TMP, S_BB, S_MEAN, S_DONE, S_TGT, S_IT, S_MAXIT, S_RZ2, S_SC, S_ISC,
+       S_TRR, S_NS };
 // every kernel of an iteration returns at once when the device-side
 // convergence flag is set (the host enqueues iterations in batches)
 __device__ __forceinline__ bool done(const double *scal) { return scal && scal[S_DONE] != 0.0; }
@@ -710,9 +717,10 @@ __global__ void k_gate(Gate g)
     (void)gate_converged(g, shr, threadIdx.x, true);
 }
 // p_new = z + beta p_old (beta from scal), q = S p_new, partial p_new.q
-// Iteration it of the PCG: rz_it = r.z from the partials of the V-cycle
-// (it > 1), beta = rz_it / rz_(it-1); block 0 stores rz_it in slot cur
-// (the previous value stays in slot prev, which the other blocks read).
+// Iteration it of the PCG: rz_it = r.z from the partials of the V-cycle,
+// beta = rz_it / rz_(it-1) (0 on the first iteration, which ignores p_old);
+// block 0 stores rz_it in slot cur (the previous value stays in slot prev,
+// which the other blocks read).
 template <int TX, int TY>
 __global__ __launch_bounds__(RB) void k_pupdate_spmv(const float *z, const float *pold, float *pnew, float *q, Lev Lin,
                                                     double *scal, double *part, const double *rz_part, int nrz,
@@ -723,10 +731,12 @@ __global__ __launch_bounds__(RB) void k_pupdate_spmv(const float *z, const float
     __shared__ double shr[256];
     if (done(scal)) return;
     float beta = 0.0f;
-    if (!first) {
+    {
         const double rz = sum_parts(rz_part, nrz, shr, threadIdx.x);
-        const double ro = scal[prev];
-        beta = ro != 0.0 ? (float)(rz / ro) : 0.0f;
+        if (!first) {
+            const double ro = scal[prev];
+            beta = ro != 0.0 ? (float)(rz / ro) : 0.0f;
+        }
         if (blockIdx.x == 0 && threadIdx.x == 0) scal[cur] = rz;
     }
     const int n = L.mx * L.my;
@@ -834,7 +844,21 @@ __global__ void k_scale_opt(float *e, int n, const double *scal)
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) e[k] *= a;
 }
 
-// FP64 refinement kernels
+// FP64 kernels of the solve
+// (S x)_k in FP64 at unknown k = i + j mx
+__device__ __forceinline__ double apply64(const double *x, const Lev &L, double cx, double cy, int k)
+{
+    const int i = k % L.mx, j = k / L.mx;
+    const double c = x[k];
+    double xl, xr, yl, yr;
+    if (i > 0) xl = x[k - 1]; else xl = L.tx == GP_AX_PER ? x[k + L.mx - 1] : (L.tx == GP_AX_NEU ? x[k + 1] : 0.0);
+    if (i < L.mx - 1) xr = x[k + 1]; else xr = L.tx == GP_AX_PER ? x[k - (L.mx - 1)] : (L.tx == GP_AX_NEU ? x[k - 1] : 0.0);
+    if (j > 0) yl = x[k - L.mx]; else yl = L.ty == GP_AX_PER ? x[k + (L.my - 1) * L.mx] : (L.ty == GP_AX_NEU ? x[k + L.mx] : 0.0);
+    if (j < L.my - 1) yr = x[k + L.mx]; else yr = L.ty == GP_AX_PER ? x[k - (L.my - 1) * L.mx] : (L.ty == GP_AX_NEU ? x[k - L.mx] : 0.0);
+    const double w = (double)wgt(L.tx, i, L.mx) * (double)wgt(L.ty, j, L.my);
+    return w * (cx * (2.0 * c - xl - xr) + cy * (2.0 * c - yl - yr));
+}
+// r = b - S x (FP64), partials of r.r and b.b
 __global__ __launch_bounds__(RB) void k_resid64(const double *x, const double *b, double *r, Lev L, double cx,
                                                double cy, double *part_rr, double *part_bb)
 {
@@ -842,16 +866,7 @@ __global__ __launch_bounds__(RB) void k_resid64(const double *x, const double *b
     const int n = L.mx * L.my;
     double arr = 0.0, abb = 0.0;
     for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
-        const int i = k % L.mx, j = k / L.mx;
-        const double c = x[k];
-        double xl, xr, yl, yr;
-        if (i > 0) xl = x[k - 1]; else xl = L.tx == GP_AX_PER ? x[k + L.mx - 1] : (L.tx == GP_AX_NEU ? x[k + 1] : 0.0);
-        if (i < L.mx - 1) xr = x[k + 1]; else xr = L.tx == GP_AX_PER ? x[k - (L.mx - 1)] : (L.tx == GP_AX_NEU ? x[k - 1] : 0.0);
-        if (j > 0) yl = x[k - L.mx]; else yl = L.ty == GP_AX_PER ? x[k + (L.my - 1) * L.mx] : (L.ty == GP_AX_NEU ? x[k + L.mx] : 0.0);
-        if (j < L.my - 1) yr = x[k + L.mx]; else yr = L.ty == GP_AX_PER ? x[k - (L.my - 1) * L.mx] : (L.ty == GP_AX_NEU ? x[k - L.mx] : 0.0);
-        const double w = (double)wgt(L.tx, i, L.mx) * (double)wgt(L.ty, j, L.my);
-        const double sx = w * (cx * (2.0 * c - xl - xr) + cy * (2.0 * c - yl - yr));
-        const double rv = b[k] - sx;
+        const double rv = b[k] - apply64(x, L, cx, cy, k);
         r[k] = rv;
         arr += rv * rv;
         abb += b[k] * b[k];
@@ -879,23 +894,93 @@ __global__ __launch_bounds__(RB) void k_inner_start(const double *r, float *rf, 
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-// both reductions of the PCG start (r.z of the first V-cycle, r.r of the
-// start kernel) and the inner solve's scalars, in one launch
-__global__ void k_pcg_start(const double *rz_part, int nrz, const double *rr_part, int nrr, double *scal,
-                            double inner_tol, int maxit)
+// start of the MG-PCG (rpcg): the true residual's r.r and b.b from the
+// partials of k_resid64, the scale sc that brings r to unit RMS, and the
+// target of the scaled FP32 recursion, ||r|| <= gf tol ||b|| sc.  The flag
+// is raised at once when the iterate already meets tol.
+__global__ void k_rstart(const double *prr, const double *pbb, int nb, double *scal, double tol, double gf, int n,
+                         int maxit)
 {
     __shared__ double sh[256];
-    const int tid = threadIdx.x;
-    const double rz = sum_parts(rz_part, nrz, sh, tid);
-    const double rr = sum_parts(rr_part, nrr, sh, tid);
-    if (tid == 0) {
-        scal[S_RZ] = rz;
-        scal[S_RR] = rr;
-        scal[S_TGT] = inner_tol * sqrt(rr);
-        scal[S_DONE] = rr > 0.0 ? 0.0 : 1.0;
+    const double rr = sum_parts(prr, nb, sh, threadIdx.x);
+    const double bb = sum_parts(pbb, nb, sh, threadIdx.x);
+    if (threadIdx.x == 0) {
+        const double rms = sqrt(rr / (double)n);
+        const double sc = rms > 0.0 ? 1.0 / rms : 1.0;
+        scal[S_TRR] = rr;
+        scal[S_BB] = bb;
+        scal[S_SC] = sc;
+        scal[S_ISC] = 1.0 / sc;
+        scal[S_TGT] = gf * tol * sqrt(bb) * sc;
+        scal[S_DONE] = (rr <= tol * tol * bb || bb == 0.0) ? 1.0 : 0.0;
         scal[S_IT] = 0.0;
         scal[S_MAXIT] = (double)maxit;
     }
+}
+// rf = FP32 of sc r (sc = scal[S_SC]); p = 0 when given; partials of rf.rf
+__global__ __launch_bounds__(RB) void k_rscale(const double *r, float *rf, float *p, int n, const double *scal,
+                                               double *part)
+{
+    __shared__ double sh[RB / 64];
+    if (done(scal)) return;
+    const double sc = scal[S_SC];
+    double acc = 0.0;
+    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
+        const float v = (float)(r[k] * sc);
+        rf[k] = v;
+        if (p) p[k] = 0.0f;
+        acc += (double)v * (double)v;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// x += (alpha / sc) p in FP64, r -= alpha q in FP32, partials of r.r;
+// alpha = rz (slot rzs) / (sum of the p.q partials), reduced by every block
+__global__ __launch_bounds__(RB) void k_update_xr64(double *x, float *r, const float *p, const float *q, int n,
+                                                    const double *scal, double *part, const double *pq_part, int npq,
+                                                    int rzs)
+{
+    __shared__ double sh[RB / 64];
+    __shared__ double shr[256];
+    if (done(scal)) return;
+    const double pq = sum_parts(pq_part, npq, shr, threadIdx.x);
+    const double ad = pq != 0.0 ? scal[rzs] / pq : 0.0;
+    const float alpha = (float)ad;
+    const double ax = ad * scal[S_ISC];
+    double acc = 0.0;
+    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
+        x[k] = __builtin_fma(ax, (double)p[k], x[k]);
+        const float rv = __builtin_fmaf(-alpha, q[k], r[k]);
+        r[k] = rv;
+        acc += (double)rv * (double)rv;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// residual replacement: rf = FP32 of sc (b - S x) from the FP64 iterate,
+// partials of rf.rf
+__global__ __launch_bounds__(RB) void k_replace(const double *x, const double *b, float *rf, Lev L, double cx, double cy,
+                                                const double *scal, double *part)
+{
+    __shared__ double sh[RB / 64];
+    if (done(scal)) return;
+    const double sc = scal[S_SC];
+    const int n = L.mx * L.my;
+    double acc = 0.0;
+    for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
+        const float v = (float)((b[k] - apply64(x, L, cx, cy, k)) * sc);
+        rf[k] = v;
+        acc += (double)v * (double)v;
+    }
+    const double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// the recursion met its target but the true residual did not: lower the
+// target by f and clear the flag
+__global__ void k_retarget(double *scal, double f)
+{
+    scal[S_TGT] *= f;
+    scal[S_DONE] = 0.0;
 }
 // two reductions into two slots in one launch (the refinement's r.r, b.b)
 __global__ void k_final2(const double *pa, const double *pb, int nb, double *scal, int sa, int sb)
@@ -1054,58 +1139,85 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
     return GP_OK;
 }
 
-// FP32 PCG on S e = r (s.r holds r and is overwritten); e in s.e.  The
-// iterations are enqueued in batches; a device-side flag (set by the
-// residual reduction at the target or the iteration limit) turns the rest of
-// a batch into no-ops, so the host synchronises once per batch.
-int pcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int *iters, int round = 0)
+// MG-PCG on S x = b64 from the FP64 iterate s.x64, to ||b - S x|| <= tol ||b||.
+// The iterate is accumulated in FP64 at every iteration; the residual
+// recursion, the V-cycle and the search directions are FP32 (dot products
+// in FP64), on the residual scaled to unit RMS.  Every MG_REPLACE
+// iterations the recursive residual is replaced by the true one computed
+// from the FP64 iterate, so the recursion's rounding (about eps32 cond(S)
+// of the starting residual) never accumulates into the stopping test, and
+// the CG keeps its search direction instead of restarting.
+// Iterations are enqueued in batches (the previous solve's count first); a
+// device-side flag, raised by the residual test in the V-cycle's first
+// kernel at the target or the iteration limit, turns the rest of a batch
+// into no-ops.  Each batch ends with the true FP64 residual, the one host
+// synchronisation; when the recursion met its target and the true residual
+// did not, the solve continues from the true residual with a lower target.
+// iters: iterations made; checks: true-residual tests; rel: the last one.
+int rpcg(hipStream_t st, gp_solver &s, double tol, int maxit, int *iters, int *checks, double *relo)
 {
-    int &last = s.last_it_r[round < 3 ? round : 3];
     const Lev L0 = mklev(s.lv[0]);
     const int n = s.n, nb = s.red_blocks;
-    const int ge = std::min(1024, gp_cdiv(n, 256));
-    // e = p = 0 and the partials of r.r come from the start kernel
-    // (k_inner_start, launched by the caller with these buffers)
+    const double cx = s.lv[0].hx2i, cy = s.lv[0].hy2i;
     float *p = s.p, *p2 = s.q2;
     double *red_pq = s.red, *red_rr = s.red + 1024, *red_rz = s.red + 2048;
-    (void)ge;
-    int np = 0;
-    int rc = vcycle(st, s, red_rz, &np, nullptr);          // z = M r, partial r.z
-    if (rc) return rc;
-    k_pcg_start<<<1, 256, 0, st>>>(red_rz, np, red_rr, nb, s.scal, inner_tol, maxit);
+    k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb);
+    k_rstart<<<1, 256, 0, st>>>(s.red, s.red + nb, nb, s.scal, tol, MG_TARGET, n, maxit);
+    k_rscale<<<nb, RB, 0, st>>>(s.r64, s.r, p, n, s.scal, red_rr);
     GP_HIP(hipGetLastError());
-    int enq = 0;
-    // first batch: the previous solve's iteration count (one more if it
-    // ran short), then single iterations.  Each kernel finishes the
-    // reduction it consumes (beta in the p update, alpha in the x/r update,
-    // the residual test in the V-cycle's first kernel), so an iteration is
-    // one launch per kernel with no separate reduction launches.
-    for (int batch = std::max(1, last);; batch = 1) {
-        for (int b = 0; b < batch && enq < maxit; b++, enq++) {
-            const int it = enq + 1;
+    int np = 0;
+    int rc = vcycle(st, s, red_rz, &np, s.scal);            // z = M r, partials of r.z
+    if (rc) return rc;
+    int it = 0, first = 1, nchk = 0, status = GP_OK;
+    double rel = 0.0;
+    // each kernel finishes the reduction it consumes (r.z in the p update,
+    // p.q in the x/r update, r.r in the V-cycle's first kernel), so an
+    // iteration is one launch per kernel
+    for (int batch = std::max(1, s.last_it);; batch = 1) {
+        for (int b = 0; b < batch && it < maxit; b++) {
+            it++;
             const int cur = (it & 1) ? S_RZ : S_RZ2, prev = (it & 1) ? S_RZ2 : S_RZ;
-#define MG_PUP(X, Y) k_pupdate_spmv<X, Y><<<nb, RB, 0, st>>>(s.z, p, p2, s.q, L0, s.scal, red_pq, red_rz, np, it == 1, cur, prev)
+#define MG_PUP(X, Y) k_pupdate_spmv<X, Y><<<nb, RB, 0, st>>>(s.z, p, p2, s.q, L0, s.scal, red_pq, red_rz, np, first, cur, prev)
             MG_AXES(L0.tx, L0.ty, MG_PUP);
 #undef MG_PUP
             std::swap(p, p2);
-            k_update_xr<<<nb, RB, 0, st>>>(s.e, s.r, p, s.q, n, s.scal, red_rr, red_pq, nb, cur);
+            first = 0;
+            k_update_xr64<<<nb, RB, 0, st>>>(s.x64, s.r, p, s.q, n, s.scal, red_rr, red_pq, nb, cur);
+            if (it % MG_REPLACE == 0) k_replace<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r, L0, cx, cy, s.scal, red_rr);
             const Gate g{red_rr, nb, it, s.scal};
             rc = vcycle(st, s, red_rz, &np, s.scal, &g);
             if (rc) return rc;
         }
+        k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb);
+        k_final2<<<1, 256, 0, st>>>(s.red, s.red + nb, nb, s.scal, S_TRR, S_BB);
         GP_HIP(hipGetLastError());
         GP_HIP(hipMemcpyAsync(s.hscal, s.scal, S_NS * sizeof(double), hipMemcpyDeviceToHost, st));
         GP_HIP(hipStreamSynchronize(st));
-        if (!(s.hscal[S_RR] == s.hscal[S_RR])) return gp_fail(GP_ERR_NONFINITE, "non-finite residual in the elliptic solve");
-        if (s.hscal[S_DONE] != 0.0 || enq >= maxit) break;
+        nchk++;
+        const double rr = s.hscal[S_TRR], bb = s.hscal[S_BB];
+        rel = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+        if (!(rel == rel)) return gp_fail(GP_ERR_NONFINITE, "non-finite residual in the elliptic solve");
+        if (rel <= tol || bb == 0.0) break;
+        if (it >= maxit) { status = GP_NOT_CONVERGED; break; }
+        if (s.hscal[S_DONE] != 0.0) {
+            // the V-cycle of the last iteration was skipped: z = M r of the
+            // true residual (k_resid64 just wrote it) completes it
+            k_retarget<<<1, 1, 0, st>>>(s.scal, 0.5 * tol / rel);
+            k_rscale<<<nb, RB, 0, st>>>(s.r64, s.r, nullptr, n, s.scal, red_rr);
+            GP_HIP(hipGetLastError());
+            const Gate g{red_rr, nb, it, s.scal};
+            rc = vcycle(st, s, red_rz, &np, s.scal, &g);
+            if (rc) return rc;
+        }
     }
-    *iters = (int)s.hscal[S_IT];
-    // the next solve's same round starts with this many iterations in one
-    // batch (one more when this one ran out before converging); a solve
-    // stopped by its iteration limit leaves the count as it was
-    if (*iters < maxit) last = *iters + (*iters >= enq ? 1 : 0);
-    s.last_it = last;
-    return GP_OK;
+    const int made = s.hscal[S_DONE] != 0.0 ? (int)s.hscal[S_IT] : it;
+    *iters = made;
+    *checks = nchk;
+    *relo = rel;
+    // the next solve's first batch: this count, one more when this solve
+    // needed a second batch; a solve stopped by its limit leaves it as it was
+    if (status == GP_OK) s.last_it = std::max(1, made + (nchk > 1 ? 1 : 0));
+    return status;
 }
 
 // FP32 flexible CG on S e = r with a caller-supplied (possibly nonlinear)
@@ -1119,7 +1231,7 @@ int fcg32(hipStream_t st, gp_solver &s, double inner_tol, int maxit, int (*M)(vo
     const int ge = std::min(1024, gp_cdiv(n, 256));
     double *red_a = s.red, *red_b = s.red + 1024;
     // k_update_xr skips its work once the convergence flag is up; the flag
-    // belongs to pcg32's batched iterations and may still be set
+    // belongs to rpcg's batched iterations and may still be set
     k_set_scalar<<<1, 1, 0, st>>>(s.scal, S_DONE, 0.0);
     k_zero_f<<<ge, 256, 0, st>>>(s.e, n);
     k_dot<<<nb, RB, 0, st>>>(s.r, s.r, n, red_a);
@@ -1453,11 +1565,12 @@ void gp_mg_free(gp_solver &s)
     s = gp_solver();
 }
 
-// Solve S x = b64 (already in s.b64) starting from s.x64, FP64 refinement.
+// Solve S x = b64 (already in s.b64) starting from s.x64 by MG-PCG (rpcg).
 // Hooks (may be null): hk->warm supplies the first correction (a learned
-// warm start) in place of an inner solve; hk->precond makes the inner solves
-// flexible CG with that preconditioner, which falls back to MG-PCG once its
-// iterations exceed those of the last plain MG-PCG solve on this hierarchy.
+// warm start) before the MG-PCG; hk->precond first runs FP64 refinement
+// rounds whose corrections come from a flexible CG with that preconditioner,
+// and hands over to the MG-PCG once its iterations exceed those of the last
+// plain MG-PCG solve on this hierarchy.
 int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int singular, gp_solve_stats *stats,
                  const gp_hooks *hk)
 {
@@ -1471,8 +1584,8 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
     // and so is the solution.  The FP32 corrections carry a constant
     // component that S does not see; left in x over many refinements it would
     // grow until its FP64 rounding (ulp of the constant times the stencil)
-    // sets a floor on the residual, so from the third refinement on every
-    // iterate is projected too
+    // sets a floor on the residual, so from the network stage's third
+    // refinement on every iterate is projected too
     auto sub_mean = [&](double *a) {
         k_sum64<<<nb, RB, 0, st>>>(a, n, s.red);
         k_final<<<1, RB, 0, st>>>(s.red, nb, s.scal, S_MEAN, 0);
@@ -1482,13 +1595,14 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
         sub_mean(s.b64);
         GP_HIP(hipGetLastError());
     }
-    int total_it = 0, outer = 0, nn_it = 0, fell_back = 0, pcg_round = 0;
+    int total_it = 0, outer = 0, nn_it = 0, fell_back = 0;
     const bool warm = hk && hk->warm, flex = hk && hk->precond;
-    bool warm_pending = warm;
     const int cap = (hk && hk->cap > 0) ? hk->cap : (s.mg_ref_it > 0 ? s.mg_ref_it : 100);
     double rel = 0.0;
     int status = GP_OK;
-    for (;;) {
+    bool finished = false;
+    // the network stage: FP64 refinement rounds
+    for (bool warm_pending = warm; warm_pending || (flex && !fell_back);) {
         k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb);
         k_final2<<<1, 256, 0, st>>>(s.red, s.red + nb, nb, s.scal, S_RR, S_BB);
         GP_HIP(hipGetLastError());
@@ -1497,43 +1611,41 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
         const double rr = s.hscal[S_RR], bb = s.hscal[S_BB];
         rel = bb > 0.0 ? std::sqrt(rr / bb) : std::sqrt(rr);
         if (!(rel == rel)) return gp_fail(GP_ERR_NONFINITE, "non-finite residual in the FP64 refinement");
-        if (rel <= tol || bb == 0.0) break;
-        if (total_it >= maxiter || outer >= 40) { status = GP_NOT_CONVERGED; break; }
+        if (rel <= tol || bb == 0.0) { finished = true; break; }
+        if (total_it >= maxiter || outer >= 40) { status = GP_NOT_CONVERGED; finished = true; break; }
         const double rms = std::sqrt(rr / n);
         const double sc = rms > 0 ? 1.0 / rms : 1.0;
-        // FP32 residual, e = p = 0, partials of r.r for the PCG start
+        // FP32 residual, e = p = 0
         k_inner_start<<<nb, RB, 0, st>>>(s.r64, s.r, s.e, s.p, n, sc, s.red + 1024);
         GP_HIP(hipGetLastError());
         if (warm_pending) {
             int rc = hk->warm(hk->ctx, s.e, s.r);
             if (rc) return rc;
-            k_axpy64<<<ge, 256, 0, st>>>(s.x64, s.e, n, 1.0 / sc);
-            if (singular && outer >= 2) sub_mean(s.x64);
-            GP_HIP(hipGetLastError());
             warm_pending = false;
-            outer++;
-            continue;
-        }
-        // ask the FP32 solve for just the reduction still needed (FP32 cannot
-        // usefully go below ~1e-6 of its own starting residual)
-        const double inner = std::max(MG_INNER_FLOOR, std::min(1e-2, 0.3 * tol / rel));
-        int it = 0;
-        if (flex && !fell_back) {
-            int conv = 0;
+        } else {
+            // ask the FP32 solve for just the reduction still needed (FP32
+            // cannot usefully go below ~1e-6 of its own starting residual)
+            const double inner = std::max(MG_INNER_FLOOR, std::min(1e-2, 0.3 * tol / rel));
+            int it = 0, conv = 0;
             const int lim = std::max(1, std::min(cap - nn_it, maxiter - total_it));
             int rc = fcg32(st, s, inner, lim, hk->precond, hk->ctx, &it, &conv);
             if (rc) return rc;
             nn_it += it;
+            total_it += it;
             if (!conv && nn_it >= cap) fell_back = 1;
-        } else {
-            int rc = pcg32(st, s, inner, std::max(1, std::min(MG_INNER_CAP, maxiter - total_it)), &it, pcg_round++);
-            if (rc) return rc;
         }
-        total_it += it;
         k_axpy64<<<ge, 256, 0, st>>>(s.x64, s.e, n, 1.0 / sc);
         if (singular && outer >= 2) sub_mean(s.x64);
         GP_HIP(hipGetLastError());
         outer++;
+    }
+    if (!finished) {
+        int it = 0, chk = 0;
+        const int rc = rpcg(st, s, tol, std::max(1, maxiter - total_it), &it, &chk, &rel);
+        if (rc < 0) return rc;
+        status = rc;
+        total_it += it;
+        outer += chk;
     }
     if (singular) {
         sub_mean(s.x64);

@@ -1177,7 +1177,6 @@ void vstats(hipStream_t st, gp_nn *n, const float *v, int kind, double *stats)
 // training targets); the mean is removed first when the problem is singular
 int nn_embed(hipStream_t st, gp_nn *n, const float *v, float *img, double *stats_dev, int kind = 0)
 {
-    const int64_t m = (int64_t)n->mx * n->my;
     vstats(st, n, v, kind, stats_dev);
     k_embed<<<g1((int64_t)n->H * n->W), NB, 0, st>>>(v, n->mx, n->my, stats_dev, img, n->H, n->W);
     GP_HIP(hipGetLastError());

@@ -90,7 +90,8 @@ typedef struct {
  * solvePoisson (EMField.mm:866, 1119-1127). */
 typedef struct {
     int    iterations;        /* inner preconditioned-CG iterations */
-    int    refinements;       /* FP64 outer corrections */
+    int    refinements;       /* FP64 true-residual tests of the MG-PCG, plus the
+                                 network's FP64 corrections when it ran */
     double relres;            /* ||b - A phi||_2 / ||b||_2 in FP64 */
     double seconds;           /* device time of the solve */
     int    used_nn;           /* 1 if the network warm start / FPCG ran */
