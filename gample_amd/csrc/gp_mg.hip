@@ -714,6 +714,7 @@ __global__ void k_final(const double *part, int nb, double *scal, int slot, int 
 __global__ void k_gate(Gate g)
 {
     __shared__ double shr[256];
+    if (done(g.scal)) return;
     (void)gate_converged(g, shr, threadIdx.x, true);
 }
 // p_new = z + beta p_old (beta from scal), q = S p_new, partial p_new.q
@@ -858,24 +859,41 @@ __device__ __forceinline__ double apply64(const double *x, const Lev &L, double 
     const double w = (double)wgt(L.tx, i, L.mx) * (double)wgt(L.ty, j, L.my);
     return w * (cx * (2.0 * c - xl - xr) + cy * (2.0 * c - yl - yr));
 }
-// r = b - S x (FP64), partials of r.r and b.b
-__global__ __launch_bounds__(RB) void k_resid64(const double *x, const double *b, double *r, Lev L, double cx,
-                                               double cy, double *part_rr, double *part_bb)
+// r = b - S x (FP64), partials of r.r and b.b.
+// bsum (may be null): the nbs partials of sum(b); b is first centred in
+// place (b -= sum / n), every block finishing the sum itself.
+// part_x (may be null): partials of sum(x)
+__global__ __launch_bounds__(RB) void k_resid64(const double *x, double *b, double *r, Lev L, double cx, double cy,
+                                               double *part_rr, double *part_bb, const double *bsum = nullptr,
+                                               int nbs = 0, double *part_x = nullptr)
 {
     __shared__ double sh[RB / 64];
+    __shared__ double shr[256];
     const int n = L.mx * L.my;
-    double arr = 0.0, abb = 0.0;
+    const double m = bsum ? sum_parts(bsum, nbs, shr, threadIdx.x) / (double)n : 0.0;
+    double arr = 0.0, abb = 0.0, ax = 0.0;
     for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
-        const double rv = b[k] - apply64(x, L, cx, cy, k);
+        double bk = b[k];
+        if (bsum) {
+            bk -= m;
+            b[k] = bk;
+        }
+        const double rv = bk - apply64(x, L, cx, cy, k);
         r[k] = rv;
         arr += rv * rv;
-        abb += b[k] * b[k];
+        abb += bk * bk;
+        if (part_x) ax += x[k];
     }
     double s = block_sum(arr, sh);
     if (threadIdx.x == 0) part_rr[blockIdx.x] = s;
     __syncthreads();
     s = block_sum(abb, sh);
     if (threadIdx.x == 0) part_bb[blockIdx.x] = s;
+    if (part_x) {
+        __syncthreads();
+        s = block_sum(ax, sh);
+        if (threadIdx.x == 0) part_x[blockIdx.x] = s;
+    }
 }
 // start of an inner solve: rf = FP32 of scale * r, e = p = 0, and the block
 // partials of rf.rf (FP64, one per block, grid-stride order)
@@ -894,36 +912,41 @@ __global__ __launch_bounds__(RB) void k_inner_start(const double *r, float *rf, 
     const double s = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-// start of the MG-PCG (rpcg): the true residual's r.r and b.b from the
-// partials of k_resid64, the scale sc that brings r to unit RMS, and the
-// target of the scaled FP32 recursion, ||r|| <= gf tol ||b|| sc.  The flag
-// is raised at once when the iterate already meets tol.
-__global__ void k_rstart(const double *prr, const double *pbb, int nb, double *scal, double tol, double gf, int n,
-                         int maxit)
-{
-    __shared__ double sh[256];
-    const double rr = sum_parts(prr, nb, sh, threadIdx.x);
-    const double bb = sum_parts(pbb, nb, sh, threadIdx.x);
-    if (threadIdx.x == 0) {
-        const double rms = sqrt(rr / (double)n);
-        const double sc = rms > 0.0 ? 1.0 / rms : 1.0;
-        scal[S_TRR] = rr;
-        scal[S_BB] = bb;
-        scal[S_SC] = sc;
-        scal[S_ISC] = 1.0 / sc;
-        scal[S_TGT] = gf * tol * sqrt(bb) * sc;
-        scal[S_DONE] = (rr <= tol * tol * bb || bb == 0.0) ? 1.0 : 0.0;
-        scal[S_IT] = 0.0;
-        scal[S_MAXIT] = (double)maxit;
-    }
-}
-// rf = FP32 of sc r (sc = scal[S_SC]); p = 0 when given; partials of rf.rf
-__global__ __launch_bounds__(RB) void k_rscale(const double *r, float *rf, float *p, int n, const double *scal,
-                                               double *part)
+// rf = FP32 of sc r, p = 0 when given, partials of rf.rf.
+// Start of the MG-PCG (rpcg) when prr is given: every block finishes the true
+// residual's r.r and b.b from the nb partials of k_resid64 itself and takes
+// the scale sc that brings r to unit RMS; block 0 records the solve's
+// scalars, among them the target of the scaled FP32 recursion,
+// ||r|| <= gf tol ||b|| sc, and raises the flag at once when the iterate
+// already meets tol.  Otherwise sc = scal[S_SC].
+__global__ __launch_bounds__(RB) void k_rscale(const double *r, float *rf, float *p, int n, double *scal,
+                                               double *part, const double *prr = nullptr, const double *pbb = nullptr,
+                                               int nb = 0, double tol = 0.0, double gf = 0.0, int maxit = 0)
 {
     __shared__ double sh[RB / 64];
-    if (done(scal)) return;
-    const double sc = scal[S_SC];
+    __shared__ double shr[256];
+    double sc;
+    if (prr) {
+        const double rr = sum_parts(prr, nb, shr, threadIdx.x);
+        const double bb = sum_parts(pbb, nb, shr, threadIdx.x);
+        const double rms = sqrt(rr / (double)n);
+        sc = rms > 0.0 ? 1.0 / rms : 1.0;
+        const bool conv = rr <= tol * tol * bb || bb == 0.0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            scal[S_TRR] = rr;
+            scal[S_BB] = bb;
+            scal[S_SC] = sc;
+            scal[S_ISC] = 1.0 / sc;
+            scal[S_TGT] = gf * tol * sqrt(bb) * sc;
+            scal[S_DONE] = conv ? 1.0 : 0.0;
+            scal[S_IT] = 0.0;
+            scal[S_MAXIT] = (double)maxit;
+        }
+        if (conv) return;
+    } else {
+        if (done(scal)) return;
+        sc = scal[S_SC];
+    }
     double acc = 0.0;
     for (int k = blockIdx.x * RB + threadIdx.x; k < n; k += gridDim.x * RB) {
         const float v = (float)(r[k] * sc);
@@ -982,13 +1005,20 @@ __global__ void k_retarget(double *scal, double f)
     scal[S_TGT] *= f;
     scal[S_DONE] = 0.0;
 }
-// two reductions into two slots in one launch (the refinement's r.r, b.b)
-__global__ void k_final2(const double *pa, const double *pb, int nb, double *scal, int sa, int sb)
+// two reductions into two slots in one launch (the refinement's r.r, b.b),
+// and a third when pc is given
+__global__ void k_final2(const double *pa, const double *pb, int nb, double *scal, int sa, int sb,
+                         const double *pc = nullptr, int scs = 0)
 {
     __shared__ double sh[256];
     const double a = sum_parts(pa, nb, sh, threadIdx.x);
     const double b = sum_parts(pb, nb, sh, threadIdx.x);
-    if (threadIdx.x == 0) { scal[sa] = a; scal[sb] = b; }
+    const double c = pc ? sum_parts(pc, nb, sh, threadIdx.x) : 0.0;
+    if (threadIdx.x == 0) {
+        scal[sa] = a;
+        scal[sb] = b;
+        if (pc) scal[scs] = c;
+    }
 }
 __global__ void k_axpy64(double *x, const float *e, int n, double scale)
 {
@@ -1020,6 +1050,15 @@ Lev mklev(const gp_mg_level &l)
     return L;
 }
 inline dim3 tiles(const Lev &L, int tb = TB) { return dim3((L.mx + tb - 1) / tb, (L.my + tb - 1) / tb); }
+// regions of the solver's reduction buffer s.red (doubles): partials of p.q,
+// of r.r (both at most 1024 blocks), of r.z (one per fine-level tile, at
+// most the 32 x 32 tiles of level 0), then sums of b or x
+constexpr int RED_PQ = 0, RED_RR = 1024, RED_RZ = 2048;
+inline int RED_X(const gp_solver &s)
+{
+    const int64_t t = (int64_t)gp_cdiv(s.lv[0].mx, TB) * gp_cdiv(s.lv[0].my, TB);
+    return RED_RZ + (int)std::max<int64_t>(t, 1024);
+}
 // tile edge of the level kernels: 64 where 32 x 32 tiles would need more
 // than one round of workgroups (two 1024-thread groups per CU), so that a
 // fine level runs in one round with a smaller share of recomputed halo
@@ -1148,31 +1187,36 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
 // of the starting residual) never accumulates into the stopping test, and
 // the CG keeps its search direction instead of restarting.
 // Iterations are enqueued in batches (the previous solve's count first); a
-// device-side flag, raised by the residual test in the V-cycle's first
-// kernel at the target or the iteration limit, turns the rest of a batch
-// into no-ops.  Each batch ends with the true FP64 residual, the one host
-// synchronisation; when the recursion met its target and the true residual
-// did not, the solve continues from the true residual with a lower target.
+// device-side flag, raised by the residual test at the target or the
+// iteration limit, turns the rest of a batch into no-ops.  The last
+// iteration of a batch makes its residual test alone and leaves its V-cycle
+// pending (a converged solve never needs it).  Each batch ends with the true
+// FP64 residual, the one host synchronisation; when the recursion met its
+// target and the true residual did not, the solve continues from the true
+// residual with a lower target.
+// center: b is centred (a singular system) by the first residual kernel.
+// On a singular system the checks also leave sum(x) in scal[S_MEAN].
 // iters: iterations made; checks: true-residual tests; rel: the last one.
-int rpcg(hipStream_t st, gp_solver &s, double tol, int maxit, int *iters, int *checks, double *relo)
+int rpcg(hipStream_t st, gp_solver &s, double tol, int maxit, bool center, int *iters, int *checks, double *relo)
 {
     const Lev L0 = mklev(s.lv[0]);
     const int n = s.n, nb = s.red_blocks;
     const double cx = s.lv[0].hx2i, cy = s.lv[0].hy2i;
     float *p = s.p, *p2 = s.q2;
-    double *red_pq = s.red, *red_rr = s.red + 1024, *red_rz = s.red + 2048;
-    k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb);
-    k_rstart<<<1, 256, 0, st>>>(s.red, s.red + nb, nb, s.scal, tol, MG_TARGET, n, maxit);
-    k_rscale<<<nb, RB, 0, st>>>(s.r64, s.r, p, n, s.scal, red_rr);
+    double *red_pq = s.red + RED_PQ, *red_rr = s.red + RED_RR, *red_rz = s.red + RED_RZ, *red_x = s.red + RED_X(s);
+    if (center) k_sum64<<<nb, RB, 0, st>>>(s.b64, n, red_x);
+    k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb, center ? red_x : nullptr, nb);
+    k_rscale<<<nb, RB, 0, st>>>(s.r64, s.r, p, n, s.scal, red_rr, s.red, s.red + nb, nb, tol, MG_TARGET, maxit);
     GP_HIP(hipGetLastError());
     int np = 0;
     int rc = vcycle(st, s, red_rz, &np, s.scal);            // z = M r, partials of r.z
     if (rc) return rc;
     int it = 0, first = 1, nchk = 0, status = GP_OK;
+    bool pending = false;   // the V-cycle of iteration it is still to be made
     double rel = 0.0;
     // each kernel finishes the reduction it consumes (r.z in the p update,
-    // p.q in the x/r update, r.r in the V-cycle's first kernel), so an
-    // iteration is one launch per kernel
+    // p.q in the x/r update, r.r in the residual test), so an iteration is
+    // one launch per kernel
     for (int batch = std::max(1, s.last_it);; batch = 1) {
         for (int b = 0; b < batch && it < maxit; b++) {
             it++;
@@ -1185,11 +1229,17 @@ int rpcg(hipStream_t st, gp_solver &s, double tol, int maxit, int *iters, int *c
             k_update_xr64<<<nb, RB, 0, st>>>(s.x64, s.r, p, s.q, n, s.scal, red_rr, red_pq, nb, cur);
             if (it % MG_REPLACE == 0) k_replace<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r, L0, cx, cy, s.scal, red_rr);
             const Gate g{red_rr, nb, it, s.scal};
-            rc = vcycle(st, s, red_rz, &np, s.scal, &g);
-            if (rc) return rc;
+            if (b + 1 < batch && it < maxit) {
+                rc = vcycle(st, s, red_rz, &np, s.scal, &g);
+                if (rc) return rc;
+            } else {
+                k_gate<<<1, 256, 0, st>>>(g);
+                pending = true;
+            }
         }
-        k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb);
-        k_final2<<<1, 256, 0, st>>>(s.red, s.red + nb, nb, s.scal, S_TRR, S_BB);
+        k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb, nullptr, 0,
+                                     s.singular ? red_x : nullptr);
+        k_final2<<<1, 256, 0, st>>>(s.red, s.red + nb, nb, s.scal, S_TRR, S_BB, s.singular ? red_x : nullptr, S_MEAN);
         GP_HIP(hipGetLastError());
         GP_HIP(hipMemcpyAsync(s.hscal, s.scal, S_NS * sizeof(double), hipMemcpyDeviceToHost, st));
         GP_HIP(hipStreamSynchronize(st));
@@ -1200,15 +1250,19 @@ int rpcg(hipStream_t st, gp_solver &s, double tol, int maxit, int *iters, int *c
         if (rel <= tol || bb == 0.0) break;
         if (it >= maxit) { status = GP_NOT_CONVERGED; break; }
         if (s.hscal[S_DONE] != 0.0) {
-            // the V-cycle of the last iteration was skipped: z = M r of the
-            // true residual (k_resid64 just wrote it) completes it
+            // the recursion met its target: z = M r of the true residual
+            // (k_resid64 just wrote it) takes the place of the pending V-cycle
             k_retarget<<<1, 1, 0, st>>>(s.scal, 0.5 * tol / rel);
             k_rscale<<<nb, RB, 0, st>>>(s.r64, s.r, nullptr, n, s.scal, red_rr);
             GP_HIP(hipGetLastError());
             const Gate g{red_rr, nb, it, s.scal};
             rc = vcycle(st, s, red_rz, &np, s.scal, &g);
             if (rc) return rc;
+        } else if (pending) {
+            rc = vcycle(st, s, red_rz, &np, s.scal);
+            if (rc) return rc;
         }
+        pending = false;
     }
     const int made = s.hscal[S_DONE] != 0.0 ? (int)s.hscal[S_IT] : it;
     *iters = made;
@@ -1442,8 +1496,7 @@ int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double 
     s.lv[0].f = s.r;     // the finest level's rhs is the CG residual
     s.lv[0].u = s.z;     // and its output the preconditioned residual
     s.red_blocks = std::min(1024, gp_cdiv(n, RB));
-    const int ntile0 = ((mx + TB - 1) / TB) * ((my + TB - 1) / TB);
-    GP_HIP(hipMalloc((void **)&s.red, (2048 + std::max(ntile0, 1024) + 64) * sizeof(double)));
+    GP_HIP(hipMalloc((void **)&s.red, (RED_X(s) + 1024 + 64) * sizeof(double)));
     GP_HIP(hipMalloc((void **)&s.scal, S_NS * sizeof(double)));
     // the coarsest level, when it is small and inside the LDS bottom solver,
     // is solved exactly: the inverse of its operator (the one the smoother
@@ -1591,12 +1644,13 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
         k_final<<<1, RB, 0, st>>>(s.red, nb, s.scal, S_MEAN, 0);
         k_sub_mean64<<<ge, 256, 0, st>>>(a, n, s.scal, S_MEAN);
     };
-    if (singular) {
+    int total_it = 0, outer = 0, nn_it = 0, fell_back = 0;
+    const bool warm = hk && hk->warm, flex = hk && hk->precond;
+    // (without the network stage, the MG-PCG's first kernel centres b)
+    if (singular && (warm || flex)) {
         sub_mean(s.b64);
         GP_HIP(hipGetLastError());
     }
-    int total_it = 0, outer = 0, nn_it = 0, fell_back = 0;
-    const bool warm = hk && hk->warm, flex = hk && hk->precond;
     const int cap = (hk && hk->cap > 0) ? hk->cap : (s.mg_ref_it > 0 ? s.mg_ref_it : 100);
     double rel = 0.0;
     int status = GP_OK;
@@ -1641,16 +1695,17 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
     }
     if (!finished) {
         int it = 0, chk = 0;
-        const int rc = rpcg(st, s, tol, std::max(1, maxiter - total_it), &it, &chk, &rel);
+        const int rc = rpcg(st, s, tol, std::max(1, maxiter - total_it), singular && !warm && !flex, &it, &chk, &rel);
         if (rc < 0) return rc;
         status = rc;
         total_it += it;
         outer += chk;
-    }
-    if (singular) {
+        // the MG-PCG's last check left sum(x) in S_MEAN
+        if (singular) k_sub_mean64<<<ge, 256, 0, st>>>(s.x64, n, s.scal, S_MEAN);
+    } else if (singular) {
         sub_mean(s.x64);
-        GP_HIP(hipGetLastError());
     }
+    GP_HIP(hipGetLastError());
     GP_HIP(hipStreamSynchronize(st));
     auto t1 = std::chrono::steady_clock::now();
     if (!warm && !flex && status == GP_OK) s.mg_ref_it = std::max(1, total_it);
