@@ -117,9 +117,14 @@ struct gp_solver {
     // dense inverse: per-axis eigenvectors V (row-major m x m) and the
     // per-mode inverse eigenvalues 1 / (cx lx_i + cy ly_j) (0 for a null mode)
     float *fd_vx = nullptr, *fd_vy = nullptr, *fd_inv = nullptr;
+    // MFMA fast diagonalisation of a coarsest level of at most 64 x 64
+    // unknowns (or null): Vx, Vy and the inverse eigenvalues, each zero-padded
+    // to 64 x 64 row-major, in one allocation
+    float *fd64 = nullptr;
     double *hscal = nullptr;   // pinned host mirror
     int red_blocks = 0;
     int last_it = 4;           // MG-PCG iterations of the previous solve (its first batch)
+    int rep_every = 4;         // MG-PCG iterations between residual replacements
     int mg_ref_it = 0;         // iterations of the last plain MG-PCG solve
 };
 

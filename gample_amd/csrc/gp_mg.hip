@@ -42,14 +42,20 @@ constexpr int NSW = MG_SWEEPS;            // smoothing sweeps per side of the V-
 #ifndef MG_INNER_FLOOR
 #define MG_INNER_FLOOR 2e-6
 #endif
-// MG-PCG: iterations between replacements of the recursive residual by the
-// true one, and the recursion's target as a fraction of tol (the true
-// residual is tested after the recursion meets it)
-#ifndef MG_REPLACE
-#define MG_REPLACE 4
+// MG-PCG: the reduction of the residual between two replacements of the
+// recursive residual by the true one (the FP32 recursion drifts by about
+// 1e-7 of the residual it starts from), and the recursion's target as a
+// fraction of tol (the true residual is tested after the recursion meets it)
+#ifndef MG_REPLACE_DROP
+#define MG_REPLACE_DROP 1e-5
 #endif
 #ifndef MG_TARGET
 #define MG_TARGET 0.8
+#endif
+// relative difference of the true and the recursive residual norms at a
+// replacement above which the search direction restarts
+#ifndef MG_DRIFT
+#define MG_DRIFT 0.1
 #endif
 #ifndef MG_BY
 #define MG_BY 32
@@ -156,9 +162,10 @@ __device__ __forceinline__ Lev fixed(Lev L)
 }
 
 // device scalars; S_SC scales the true residual to unit RMS for the FP32
-// recursion (S_ISC = 1 / S_SC), S_TRR is the true residual's r.r
+// recursion (S_ISC = 1 / S_SC), S_TRR is the true residual's r.r (S_TRR0 at
+// the start of the solve)
 enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_TMP, S_BB, S_MEAN, S_DONE, S_TGT, S_IT, S_MAXIT, S_RZ2, S_SC, S_ISC,
-       S_TRR, S_NS };
+       S_TRR, S_TRR0, S_NS };
 // every kernel of an iteration returns at once when the device-side
 // convergence flag is set (the host enqueues iterations in batches)
 __device__ __forceinline__ bool done(const double *scal) { return scal && scal[S_DONE] != 0.0; }
@@ -459,15 +466,28 @@ struct Bottom {
     const float *fvx, *fvy, *finv;
 };
 
+// Gauss-Seidel value at unknown (i, j) with the neighbour values given by
+// v(i', j'): Dirichlet walls are 0, Neumann boundaries mirror, periodic
+// axes wrap
+template <class V>
+__device__ __forceinline__ float gs_with(const float *f, const Lev &L, int i, int j, float invd, V v)
+{
+    float uL, uR, uD, uU;
+    uL = i > 0 ? v(i - 1, j) : (L.tx == GP_AX_PER ? v(L.mx - 1, j) : (L.tx == GP_AX_NEU ? v(i + 1, j) : 0.0f));
+    uR = i < L.mx - 1 ? v(i + 1, j) : (L.tx == GP_AX_PER ? v(0, j) : (L.tx == GP_AX_NEU ? v(i - 1, j) : 0.0f));
+    uD = j > 0 ? v(i, j - 1) : (L.ty == GP_AX_PER ? v(i, L.my - 1) : (L.ty == GP_AX_NEU ? v(i, j + 1) : 0.0f));
+    uU = j < L.my - 1 ? v(i, j + 1) : (L.ty == GP_AX_PER ? v(i, 0) : (L.ty == GP_AX_NEU ? v(i, j - 1) : 0.0f));
+    return (f[i + j * L.mx] * iweight(L, i, j) + L.cx * (uL + uR) + L.cy * (uD + uU)) * invd;
+}
 __device__ __forceinline__ float gs_point(const float *u, const float *f, const Lev &L, int i, int j, float invd)
 {
-    const int k = i + j * L.mx;
-    float uL, uR, uD, uU;
-    uL = i > 0 ? u[k - 1] : (L.tx == GP_AX_PER ? u[k + L.mx - 1] : (L.tx == GP_AX_NEU ? u[k + 1] : 0.0f));
-    uR = i < L.mx - 1 ? u[k + 1] : (L.tx == GP_AX_PER ? u[k - (L.mx - 1)] : (L.tx == GP_AX_NEU ? u[k - 1] : 0.0f));
-    uD = j > 0 ? u[k - L.mx] : (L.ty == GP_AX_PER ? u[k + (L.my - 1) * L.mx] : (L.ty == GP_AX_NEU ? u[k + L.mx] : 0.0f));
-    uU = j < L.my - 1 ? u[k + L.mx] : (L.ty == GP_AX_PER ? u[k - (L.my - 1) * L.mx] : (L.ty == GP_AX_NEU ? u[k - L.mx] : 0.0f));
-    return (f[k] * iweight(L, i, j) + L.cx * (uL + uR) + L.cy * (uD + uU)) * invd;
+    return gs_with(f, L, i, j, invd, [&](int a, int b) { return u[a + b * L.mx]; });
+}
+// the first red half-sweep of a smoothing from u = 0 at red unknown (i, j):
+// the neighbours are black and still zero, so gs_point gives (f w^-1 + 0 + 0) / d
+__device__ __forceinline__ float red0_point(const float *f, const Lev &L, int i, int j, float invd)
+{
+    return (f[i + j * L.mx] * iweight(L, i, j) + L.cx * (0.0f + 0.0f) + L.cy * (0.0f + 0.0f)) * invd;
 }
 // r = w d (g - u) with g the Gauss-Seidel value, w = 1/iweight
 __device__ __forceinline__ float resid_point(const float *u, const float *f, const Lev &L, int i, int j, float invd)
@@ -497,10 +517,7 @@ __device__ void bot_sweep_red0(float *u, const float *f, const Lev &L)
 {
     const float invd = 1.0f / (2.0f * (L.cx + L.cy));
     for (int j = threadIdx.y; j < L.my; j += 32)
-        for (int i = 2 * threadIdx.x + (j & 1); i < L.mx; i += 64) {
-            const int k = i + j * L.mx;
-            u[k] = (f[k] * iweight(L, i, j) + L.cx * (0.0f + 0.0f) + L.cy * (0.0f + 0.0f)) * invd;
-        }
+        for (int i = 2 * threadIdx.x + (j & 1); i < L.mx; i += 64) u[i + j * L.mx] = red0_point(f, L, i, j, invd);
     __syncthreads();
 }
 __device__ void bot_project(float *a, const Lev &L, float *scratch)
@@ -553,6 +570,42 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         float *u = lds + B.off[l], *f = u + n;
         float *fc = lds + B.off[l + 1] + nc;
         const float invd = 1.0f / (2.0f * (L.cx + L.cy));
+        if constexpr (NSW == 1) {
+            // one red-black sweep from u = 0: the red values of the first
+            // half-sweep are a function of f alone, so the black half-sweep
+            // recomputes its neighbours' and comes first; the residual is then
+            // evaluated at the red unknowns only, since the black half-sweep
+            // has just made it exactly zero at the black ones (the restriction
+            // skips them)
+            auto red0 = [&](int a, int b) { return red0_point(f, L, a, b, invd); };
+            for (int j = threadIdx.y; j < L.my; j += 32)
+                for (int i = 2 * threadIdx.x + ((j + 1) & 1); i < L.mx; i += 64)
+                    u[i + j * L.mx] = gs_with(f, L, i, j, invd, red0);
+            __syncthreads();
+            auto uv = [&](int a, int b) { return u[a + b * L.mx]; };
+            for (int j = threadIdx.y; j < L.my; j += 32)
+                for (int i = 2 * threadIdx.x + (j & 1); i < L.mx; i += 64)
+                    res[i + j * L.mx] = (gs_with(f, L, i, j, invd, uv) - red0_point(f, L, i, j, invd)) *
+                                        (2.0f * (L.cx + L.cy)) * rweight(L, i, j);
+            __syncthreads();
+            BOT_LOOP(C) {
+                const int cxn = 2 * c_node(C.tx, i), cyn = 2 * c_node(C.ty, j);
+                float acc = 0.0f;
+                for (int b = -1; b <= 1; b++) {
+                    int jj = L.ty == GP_AX_PER ? wrp(cyn + b, L.my) : f_unk(L.ty, cyn + b, L.My);
+                    if (jj < 0) continue;
+                    const float wy = b == 0 ? 0.5f : 0.25f;
+                    for (int a = -1; a <= 1; a++) {
+                        int ii = L.tx == GP_AX_PER ? wrp(cxn + a, L.mx) : f_unk(L.tx, cxn + a, L.Mx);
+                        if (ii < 0 || ((ii + jj) & 1)) continue;
+                        acc += (a == 0 ? 0.5f : 0.25f) * wy * res[ii + jj * L.mx];
+                    }
+                }
+                fc[i + j * C.mx] = acc;
+            }
+            __syncthreads();
+            continue;
+        }
         // from u = 0: the first red sweep reads no u
         bot_sweep_red0(u, f, L);
         bot_sweep(u, f, L, 1);
@@ -584,13 +637,13 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         // in; the fast diagonalisation drops the null mode)
         if (B.singular && !B.dense && !B.fvx) bot_project(f, L, scratch);
         if (B.dense) {
-            // u = M f with the precomputed inverse (n <= 256): thread t sums
-            // quarter t/256 of row t%256, the four quarters add in LDS
-            const int row = tid & 255, qd = tid >> 8;
-            const int j0 = (n * qd) / 4, j1 = (n * (qd + 1)) / 4;
+            // u = M f with the precomputed inverse (n <= DENSE_MAX): thread t
+            // sums quarter t/256 of row t%256, the four quarters add in LDS;
             // 16 independent loads in flight per step (the matrix streams
             // from L2; one load per dependent fma would serialise them)
             float acc = 0.0f;
+            const int row = tid & 255, qd = tid >> 8;
+            const int j0 = (n * qd) / 4, j1 = (n * (qd + 1)) / 4;
             if (row < n) {
                 int jj = j0;
                 for (; jj + 16 <= j1; jj += 16) {
@@ -654,6 +707,19 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
         const int n = L.mx * L.my;
         float *u = lds + B.off[l], *f = u + n;
         const float *uc = lds + B.off[l + 1];
+        if constexpr (NSW == 1) {
+            // the black half-sweep reads only red values and the red one only
+            // black values, so the coarse correction is added at the red
+            // unknowns only (to their pre-smoothed values, recomputed from f)
+            const float invd = 1.0f / (2.0f * (L.cx + L.cy));
+            for (int j = threadIdx.y; j < L.my; j += 32)
+                for (int i = 2 * threadIdx.x + (j & 1); i < L.mx; i += 64)
+                    u[i + j * L.mx] = red0_point(f, L, i, j, invd) + prolong(uc, L, C, i, j);
+            __syncthreads();
+            bot_sweep(u, f, L, 1);
+            bot_sweep(u, f, L, 0);
+            continue;
+        }
         BOT_LOOP(L) u[i + j * L.mx] += prolong(uc, L, C, i, j);
         __syncthreads();
         for (int sw = 0; sw < NSW; sw++) { bot_sweep(u, f, L, 1); bot_sweep(u, f, L, 0); }
@@ -676,6 +742,121 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const float *fin, float *uou
                 for (int w = 0; w < 16; w++) t += dred[w];
                 part[0] = t;
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// coarsest level of at most 64 x 64 unknowns solved exactly by fast
+// diagonalisation on MFMA, in one workgroup of 16 waves:
+//   u = Vy (Linv o (Vy' F Vx)) Vx'
+// with F[j][i] = f(i, j) (row j of the unknown grid), K V = V Lambda per axis
+// (V' W V = I) and Linv = 1 / (cx lambda_x + cy lambda_y) (0 for a null mode).
+// Every factor is zero-padded to 64 x 64 in LDS with row pitch FDP, so that
+// the strided operand reads of the products hit distinct banks; each wave
+// computes one 16 x 16 tile of each of the four products with
+// v_mfma_f32_16x16x4_f32 (A: lane l gives A[l % 16][k + l / 16], B: B[k +
+// l / 16][l % 16]; D: register q of lane l holds D[4 (l / 16) + q][l % 16]).
+// ---------------------------------------------------------------------------
+constexpr int FD64 = 64, FDP = FD64 + 1;
+constexpr int FD64_LDS = 6 * FD64 * FDP * (int)sizeof(float);   // dynamic LDS of k_mg_fd64
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+struct Fd64 {
+    const float *v;   // Vx, Vy, Linv: 3 x 64 x 64
+    int mx, my;
+};
+// tile (ti, tj) of A B with A[i][k] = a(i, k), B[k][j] = b(k, j), K = 64
+template <class FA, class FB>
+__device__ __forceinline__ f32x4 fd_tile(FA a, FB b, int ti, int tj, int lane)
+{
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+    for (int k0 = 0; k0 < FD64; k0 += 4)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a(ti * 16 + r, k0 + kq), b(k0 + kq, tj * 16 + r), acc, 0, 0, 0);
+    return acc;
+}
+__global__ __launch_bounds__(1024) void k_mg_fd64(const float *fin, float *uout, Fd64 A, double *part,
+                                                  const double *scal, Gate gt)
+{
+    extern __shared__ float lds[];   // F, Vx, Vy, Linv, T, U: 64 x FDP each
+    __shared__ double shr[256];
+    __shared__ double dred[16];
+    if (done(scal)) return;
+    const int tid = threadIdx.x;
+    if (gt.rr_part && gate_converged(gt, shr, tid, true)) return;
+    float *F = lds, *Vx = F + FD64 * FDP, *Vy = Vx + FD64 * FDP, *Li = Vy + FD64 * FDP;
+    float *T = Li + FD64 * FDP, *U = T + FD64 * FDP;
+    const int mx = A.mx, my = A.my;
+    {
+        // every load is issued before any is consumed (indices clamped into
+        // the level; the padding is selected afterwards)
+        float fv[4], xv[4], yv[4], lv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + 1024 * q, i = e & 63, j = e >> 6;
+            fv[q] = fin[std::min(i, mx - 1) + std::min(j, my - 1) * mx];
+            xv[q] = A.v[e];
+            yv[q] = A.v[4096 + e];
+            lv[q] = A.v[8192 + e];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = tid + 1024 * q, i = e & 63, j = e >> 6, k = j * FDP + i;
+            F[k] = (i < mx && j < my) ? fv[q] : 0.0f;
+            Vx[k] = xv[q];
+            Vy[k] = yv[q];
+            Li[k] = lv[q];
+        }
+    }
+    __syncthreads();
+    const int lane = tid & 63, wv = tid >> 6, ti = wv >> 2, tj = wv & 3;
+    const int r = lane & 15, kq = lane >> 4;
+    {   // T = F Vx
+        const f32x4 c = fd_tile([&](int i, int k) { return F[i * FDP + k]; },
+                                [&](int k, int j) { return Vx[k * FDP + j]; }, ti, tj, lane);
+#pragma unroll
+        for (int q = 0; q < 4; q++) T[(ti * 16 + 4 * kq + q) * FDP + tj * 16 + r] = c[q];
+    }
+    __syncthreads();
+    {   // U = Linv o (Vy' T)
+        const f32x4 c = fd_tile([&](int i, int k) { return Vy[k * FDP + i]; },
+                                [&](int k, int j) { return T[k * FDP + j]; }, ti, tj, lane);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = (ti * 16 + 4 * kq + q) * FDP + tj * 16 + r;
+            U[k] = c[q] * Li[k];
+        }
+    }
+    __syncthreads();
+    {   // T = Vy U
+        const f32x4 c = fd_tile([&](int i, int k) { return Vy[i * FDP + k]; },
+                                [&](int k, int j) { return U[k * FDP + j]; }, ti, tj, lane);
+#pragma unroll
+        for (int q = 0; q < 4; q++) T[(ti * 16 + 4 * kq + q) * FDP + tj * 16 + r] = c[q];
+    }
+    __syncthreads();
+    double acc = 0.0;
+    {   // u = T Vx'
+        const f32x4 c = fd_tile([&](int i, int k) { return T[i * FDP + k]; },
+                                [&](int k, int j) { return Vx[j * FDP + k]; }, ti, tj, lane);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = ti * 16 + 4 * kq + q, i = tj * 16 + r;
+            if (i < mx && j < my) {
+                uout[i + j * mx] = c[q];
+                acc += (double)F[j * FDP + i] * (double)c[q];
+            }
+        }
+    }
+    if (part) {   // this level is the finest: the partial of r.z
+        for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+        if (lane == 0) dred[wv] = acc;
+        __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+            for (int w = 0; w < 16; w++) t += dred[w];
+            part[0] = t;
         }
     }
 }
@@ -722,10 +903,15 @@ __global__ void k_gate(Gate g)
 // beta = rz_it / rz_(it-1) (0 on the first iteration, which ignores p_old);
 // block 0 stores rz_it in slot cur (the previous value stays in slot prev,
 // which the other blocks read).
+// rr_true (may be null): the previous iteration replaced its recursive
+// residual (partials rr_rec) by the true one (partials rr_true); when their
+// norms differ by more than MG_DRIFT the old direction carries the
+// recursion's drift and the search restarts from z (beta = 0).
 template <int TX, int TY>
 __global__ __launch_bounds__(RB) void k_pupdate_spmv(const float *z, const float *pold, float *pnew, float *q, Lev Lin,
                                                     double *scal, double *part, const double *rz_part, int nrz,
-                                                    int first, int cur, int prev)
+                                                    int first, int cur, int prev, const double *rr_rec = nullptr,
+                                                    const double *rr_true = nullptr, int nrr = 0)
 {
     const Lev L = fixed<TX, TY>(Lin);
     __shared__ double sh[RB / 64];
@@ -734,6 +920,11 @@ __global__ __launch_bounds__(RB) void k_pupdate_spmv(const float *z, const float
     float beta = 0.0f;
     {
         const double rz = sum_parts(rz_part, nrz, shr, threadIdx.x);
+        if (rr_true) {
+            const double a = sqrt(sum_parts(rr_true, nrr, shr, threadIdx.x));
+            const double b = sqrt(sum_parts(rr_rec, nrr, shr, threadIdx.x));
+            if (!(fabs(a - b) <= MG_DRIFT * a)) first = 1;
+        }
         if (!first) {
             const double ro = scal[prev];
             beta = ro != 0.0 ? (float)(rz / ro) : 0.0f;
@@ -934,6 +1125,7 @@ __global__ __launch_bounds__(RB) void k_rscale(const double *r, float *rf, float
         const bool conv = rr <= tol * tol * bb || bb == 0.0;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             scal[S_TRR] = rr;
+            scal[S_TRR0] = rr;
             scal[S_BB] = bb;
             scal[S_SC] = sc;
             scal[S_ISC] = 1.0 / sc;
@@ -1052,13 +1244,17 @@ Lev mklev(const gp_mg_level &l)
 inline dim3 tiles(const Lev &L, int tb = TB) { return dim3((L.mx + tb - 1) / tb, (L.my + tb - 1) / tb); }
 // regions of the solver's reduction buffer s.red (doubles): partials of p.q,
 // of r.r (both at most 1024 blocks), of r.z (one per fine-level tile, at
-// most the 32 x 32 tiles of level 0), then sums of b or x
+// most the 32 x 32 tiles of level 0), then sums of b or x, the true
+// residual's r.r at a replacement, and b.b of a true-residual test (whose
+// r.r goes to the p.q region: no iteration is under way)
 constexpr int RED_PQ = 0, RED_RR = 1024, RED_RZ = 2048;
 inline int RED_X(const gp_solver &s)
 {
     const int64_t t = (int64_t)gp_cdiv(s.lv[0].mx, TB) * gp_cdiv(s.lv[0].my, TB);
     return RED_RZ + (int)std::max<int64_t>(t, 1024);
 }
+inline int RED_TR(const gp_solver &s) { return RED_X(s) + 1024; }
+inline int RED_B(const gp_solver &s) { return RED_TR(s) + 1024; }
 // tile edge of the level kernels: 64 where 32 x 32 tiles would need more
 // than one round of workgroups (two 1024-thread groups per CU), so that a
 // fine level runs in one round with a smaller share of recomputed halo
@@ -1113,7 +1309,15 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
     }
     GP_HIP(hipGetLastError());
     *nparts = 0;
-    if (lb < nl) {
+    if (lb < nl && s.fd64) {
+        // the coarsest level (lb = nl - 1) solved exactly on MFMA
+        const bool top = lb == 0;
+        const Gate bg = take();
+        const Fd64 A{s.fd64, s.lv[lb].mx, s.lv[lb].my};
+        k_mg_fd64<<<1, 1024, FD64_LDS, st>>>(s.lv[lb].f, s.lv[lb].u, A, top ? part : nullptr, scal, bg);
+        GP_HIP(hipGetLastError());
+        if (top) *nparts = 1;
+    } else if (lb < nl) {
         Bottom B{};
         int off = 0;
         B.nl = nl - lb;
@@ -1181,11 +1385,13 @@ int vcycle(hipStream_t st, gp_solver &s, double *part, int *nparts, const double
 // MG-PCG on S x = b64 from the FP64 iterate s.x64, to ||b - S x|| <= tol ||b||.
 // The iterate is accumulated in FP64 at every iteration; the residual
 // recursion, the V-cycle and the search directions are FP32 (dot products
-// in FP64), on the residual scaled to unit RMS.  Every MG_REPLACE
+// in FP64), on the residual scaled to unit RMS.  Every s.rep_every
 // iterations the recursive residual is replaced by the true one computed
 // from the FP64 iterate, so the recursion's rounding (about eps32 cond(S)
-// of the starting residual) never accumulates into the stopping test, and
-// the CG keeps its search direction instead of restarting.
+// of the residual it starts from) never accumulates into the stopping test,
+// and the CG keeps its search direction instead of restarting.  The
+// interval is the number of iterations that reduced the residual by
+// MG_REPLACE_DROP at the previous solve's rate.
 // Iterations are enqueued in batches (the previous solve's count first); a
 // device-side flag, raised by the residual test at the target or the
 // iteration limit, turns the rest of a batch into no-ops.  The last
@@ -1204,15 +1410,17 @@ int rpcg(hipStream_t st, gp_solver &s, double tol, int maxit, bool center, int *
     const double cx = s.lv[0].hx2i, cy = s.lv[0].hy2i;
     float *p = s.p, *p2 = s.q2;
     double *red_pq = s.red + RED_PQ, *red_rr = s.red + RED_RR, *red_rz = s.red + RED_RZ, *red_x = s.red + RED_X(s);
+    double *red_tr = s.red + RED_TR(s), *red_b = s.red + RED_B(s);
     if (center) k_sum64<<<nb, RB, 0, st>>>(s.b64, n, red_x);
-    k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb, center ? red_x : nullptr, nb);
-    k_rscale<<<nb, RB, 0, st>>>(s.r64, s.r, p, n, s.scal, red_rr, s.red, s.red + nb, nb, tol, MG_TARGET, maxit);
+    k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, red_pq, red_b, center ? red_x : nullptr, nb);
+    k_rscale<<<nb, RB, 0, st>>>(s.r64, s.r, p, n, s.scal, red_rr, red_pq, red_b, nb, tol, MG_TARGET, maxit);
     GP_HIP(hipGetLastError());
     int np = 0;
     int rc = vcycle(st, s, red_rz, &np, s.scal);            // z = M r, partials of r.z
     if (rc) return rc;
     int it = 0, first = 1, nchk = 0, status = GP_OK;
     bool pending = false;   // the V-cycle of iteration it is still to be made
+    bool replaced = false;  // iteration it replaced its recursive residual
     double rel = 0.0;
     // each kernel finishes the reduction it consumes (r.z in the p update,
     // p.q in the x/r update, r.r in the residual test), so an iteration is
@@ -1221,14 +1429,16 @@ int rpcg(hipStream_t st, gp_solver &s, double tol, int maxit, bool center, int *
         for (int b = 0; b < batch && it < maxit; b++) {
             it++;
             const int cur = (it & 1) ? S_RZ : S_RZ2, prev = (it & 1) ? S_RZ2 : S_RZ;
-#define MG_PUP(X, Y) k_pupdate_spmv<X, Y><<<nb, RB, 0, st>>>(s.z, p, p2, s.q, L0, s.scal, red_pq, red_rz, np, first, cur, prev)
+#define MG_PUP(X, Y) k_pupdate_spmv<X, Y><<<nb, RB, 0, st>>>(s.z, p, p2, s.q, L0, s.scal, red_pq, red_rz, np, first, \
+                                                             cur, prev, red_rr, replaced ? red_tr : nullptr, nb)
             MG_AXES(L0.tx, L0.ty, MG_PUP);
 #undef MG_PUP
             std::swap(p, p2);
             first = 0;
             k_update_xr64<<<nb, RB, 0, st>>>(s.x64, s.r, p, s.q, n, s.scal, red_rr, red_pq, nb, cur);
-            if (it % MG_REPLACE == 0) k_replace<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r, L0, cx, cy, s.scal, red_rr);
-            const Gate g{red_rr, nb, it, s.scal};
+            replaced = it % s.rep_every == 0;
+            if (replaced) k_replace<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r, L0, cx, cy, s.scal, red_tr);
+            const Gate g{replaced ? red_tr : red_rr, nb, it, s.scal};
             if (b + 1 < batch && it < maxit) {
                 rc = vcycle(st, s, red_rz, &np, s.scal, &g);
                 if (rc) return rc;
@@ -1237,9 +1447,9 @@ int rpcg(hipStream_t st, gp_solver &s, double tol, int maxit, bool center, int *
                 pending = true;
             }
         }
-        k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb, nullptr, 0,
+        k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, red_pq, red_b, nullptr, 0,
                                      s.singular ? red_x : nullptr);
-        k_final2<<<1, 256, 0, st>>>(s.red, s.red + nb, nb, s.scal, S_TRR, S_BB, s.singular ? red_x : nullptr, S_MEAN);
+        k_final2<<<1, 256, 0, st>>>(red_pq, red_b, nb, s.scal, S_TRR, S_BB, s.singular ? red_x : nullptr, S_MEAN);
         GP_HIP(hipGetLastError());
         GP_HIP(hipMemcpyAsync(s.hscal, s.scal, S_NS * sizeof(double), hipMemcpyDeviceToHost, st));
         GP_HIP(hipStreamSynchronize(st));
@@ -1250,8 +1460,18 @@ int rpcg(hipStream_t st, gp_solver &s, double tol, int maxit, bool center, int *
         if (rel <= tol || bb == 0.0) break;
         if (it >= maxit) { status = GP_NOT_CONVERGED; break; }
         if (s.hscal[S_DONE] != 0.0) {
-            // the recursion met its target: z = M r of the true residual
-            // (k_resid64 just wrote it) takes the place of the pending V-cycle
+            // the recursion met its target at iteration S_IT and the rest of
+            // the batch did nothing: continue from that iteration (the r.z
+            // slots alternate with its parity, and p is the buffer its p
+            // update wrote); z = M r of the true residual (k_resid64 just
+            // wrote it) takes the place of its skipped V-cycle.  The
+            // recursion had drifted from the true residual, so the search
+            // direction restarts from z
+            const int made = (int)s.hscal[S_IT];
+            if ((it - made) & 1) std::swap(p, p2);
+            it = made;
+            first = 1;
+            replaced = false;
             k_retarget<<<1, 1, 0, st>>>(s.scal, 0.5 * tol / rel);
             k_rscale<<<nb, RB, 0, st>>>(s.r64, s.r, nullptr, n, s.scal, red_rr);
             GP_HIP(hipGetLastError());
@@ -1271,6 +1491,12 @@ int rpcg(hipStream_t st, gp_solver &s, double tol, int maxit, bool center, int *
     // the next solve's first batch: this count, one more when this solve
     // needed a second batch; a solve stopped by its limit leaves it as it was
     if (status == GP_OK) s.last_it = std::max(1, made + (nchk > 1 ? 1 : 0));
+    // the next solve's replacement interval from this solve's rate
+    const double rr0 = s.hscal[S_TRR0];
+    if (status == GP_OK && made > 0 && rr0 > 0.0 && s.hscal[S_TRR] > 0.0) {
+        const double rate = std::pow(std::sqrt(s.hscal[S_TRR] / rr0), 1.0 / made);
+        if (rate < 1.0) s.rep_every = std::min(16, std::max(1, (int)(std::log(MG_REPLACE_DROP) / std::log(rate))));
+    }
     return status;
 }
 
@@ -1443,6 +1669,7 @@ int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double 
     for (;;) {
         const gp_mg_level &f = s.lv.back();
         if ((int64_t)f.mx * f.my <= DENSE_MAX) break;
+        if (f.mx <= FD64 && f.my <= FD64) break;   // solved exactly on MFMA
         const int Mx = M(tx, f.mx), My = M(ty, f.my);
         if ((Mx & 1) || (My & 1) || Mx < 4 || My < 4) break;
         if ((int)s.lv.size() >= 24) break;
@@ -1455,13 +1682,20 @@ int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double 
         c.hx2i = f.hx2i / 4.0; c.hy2i = f.hy2i / 4.0;
         s.lv.push_back(c);
     }
-    // the first level small enough for the LDS bottom solver
+    // the first level small enough for the LDS bottom solver; a coarsest
+    // level of at most 64 x 64 (and more than DENSE_MAX) unknowns is the
+    // bottom alone, solved by k_mg_fd64
     const int nl = (int)s.lv.size();
+    const gp_mg_level &cl = s.lv[nl - 1];
+    const bool fd64 = cl.mx <= FD64 && cl.my <= FD64 && (int64_t)cl.mx * cl.my > DENSE_MAX;
     s.bottom = nl;
     for (int k = 0; k < nl; k++)
         if ((int64_t)s.lv[k].mx * s.lv[k].my <= BOT_MAX) { s.bottom = k; break; }
+    if (fd64) s.bottom = nl - 1;
     if (s.bottom < nl && nl - s.bottom > BOT_LEVELS) return gp_fail(GP_ERR_UNSUPPORTED, "too many multigrid levels");
-    if (s.bottom < nl) {
+    if (fd64) {
+        GP_HIP(hipFuncSetAttribute((const void *)k_mg_fd64, hipFuncAttributeMaxDynamicSharedMemorySize, FD64_LDS));
+    } else if (s.bottom < nl) {
         // the bottom kernel's dynamic LDS (u and f of every level it holds)
         size_t bytes = 0;
         for (int k = s.bottom; k < nl; k++) bytes += 2 * sizeof(float) * (size_t)s.lv[k].mx * s.lv[k].my;
@@ -1496,7 +1730,7 @@ int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double 
     s.lv[0].f = s.r;     // the finest level's rhs is the CG residual
     s.lv[0].u = s.z;     // and its output the preconditioned residual
     s.red_blocks = std::min(1024, gp_cdiv(n, RB));
-    GP_HIP(hipMalloc((void **)&s.red, (RED_X(s) + 1024 + 64) * sizeof(double)));
+    GP_HIP(hipMalloc((void **)&s.red, (RED_B(s) + 1024 + 64) * sizeof(double)));
     GP_HIP(hipMalloc((void **)&s.scal, S_NS * sizeof(double)));
     // the coarsest level, when it is small and inside the LDS bottom solver,
     // is solved exactly: the inverse of its operator (the one the smoother
@@ -1579,7 +1813,7 @@ int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double 
     // diagonalisation instead of Gauss-Seidel sweeps
     {
         const gp_mg_level &c = s.lv[nl - 1];
-        if (s.bottom < nl && !s.dinv && c.mx <= FD_MAX && c.my <= FD_MAX) {
+        if (s.bottom < nl && !fd64 && !s.dinv && c.mx <= FD_MAX && c.my <= FD_MAX) {
             std::vector<double> vx, lx, vy, ly;
             axis_eig(tx, c.mx, vx, lx);
             axis_eig(ty, c.my, vy, ly);
@@ -1600,6 +1834,26 @@ int gp_mg_build(gp_solver &s, int tx, int ty, int mx, int my, double cx, double 
         }
     }
 
+    if (fd64) {
+        // the MFMA fast diagonalisation: Vx, Vy and 1 / (cx lx + cy ly), each
+        // zero-padded to 64 x 64 (a null mode has 0)
+        std::vector<double> vx, lx, vy, ly;
+        axis_eig(tx, cl.mx, vx, lx);
+        axis_eig(ty, cl.my, vy, ly);
+        std::vector<float> h(3 * FD64 * FD64, 0.0f);
+        for (int i = 0; i < cl.mx; i++)
+            for (int k = 0; k < cl.mx; k++) h[i * FD64 + k] = (float)vx[(size_t)i * cl.mx + k];
+        for (int j = 0; j < cl.my; j++)
+            for (int k = 0; k < cl.my; k++) h[FD64 * FD64 + j * FD64 + k] = (float)vy[(size_t)j * cl.my + k];
+        for (int j = 0; j < cl.my; j++)
+            for (int i = 0; i < cl.mx; i++) {
+                const double ev = cl.hx2i * lx[i] + cl.hy2i * ly[j];
+                h[2 * FD64 * FD64 + j * FD64 + i] = std::fabs(ev) > 1e-9 * (cl.hx2i + cl.hy2i) ? (float)(1.0 / ev) : 0.0f;
+            }
+        GP_HIP(hipMalloc((void **)&s.fd64, h.size() * sizeof(float)));
+        GP_HIP(hipMemcpy(s.fd64, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+
     GP_HIP(hipHostMalloc((void **)&s.hscal, S_NS * sizeof(double), hipHostMallocDefault));
     return GP_OK;
 }
@@ -1612,7 +1866,7 @@ void gp_mg_free(gp_solver &s)
     }
     s.lv.clear();
     void *ps[] = {s.b64, s.x64, s.r64, s.rhs_bc, s.r, s.z, s.p, s.q, s.q2, s.e, s.red, s.scal, s.dinv,
-                  s.fd_vx, s.fd_vy, s.fd_inv};
+                  s.fd_vx, s.fd_vy, s.fd_inv, s.fd64};
     for (void *p : ps) if (p) (void)hipFree(p);
     if (s.hscal) (void)hipHostFree(s.hscal);
     s = gp_solver();
