@@ -87,6 +87,9 @@ __device__ __forceinline__ int wwrap(int g, int m)
     const int w = wrp(g, m);
     return (w >= 0 && w < m) ? w : ((g % m) + m) % m;
 }
+// wrap of a window index on a level at least a window wide (the index is at
+// most one period outside [0, m))
+__device__ __forceinline__ int wnear(int g, int m) { return g < 0 ? g + m : (g >= m ? g - m : g); }
 __device__ __forceinline__ float wgt(int t, int i, int m) { return (t == GP_AX_NEU && (i == 0 || i == m - 1)) ? 0.5f : 1.0f; }
 __device__ __forceinline__ bool in_axis(int t, int g, int m) { return t == GP_AX_PER || (g >= 0 && g < m); }
 // 1/w of the row scaling D (w = 1/2 on Neumann boundary nodes): 1, 2 or 4
@@ -238,14 +241,20 @@ __global__ __launch_bounds__(NT) void k_mg_down(const float *f, float *u, float 
     const int gi0 = ti0 - HD, gj0 = tj0 - HD;
     const float invd = 1.0f / (2.0f * (L.cx + L.cy)), d = 2.0f * (L.cx + L.cy);
     float fv[NE];
+    // (a level at least a window wide needs no general wrap: the integer
+    // remainders of wwrap would dominate this phase)
+    auto load = [&](auto wr) {
 #pragma unroll
-    for (int k = 0; k < NE; k++) {
-        const int e = tid + NT * k;
-        const int li = e % WD, lj = e / WD;
-        const int gi = gi0 + li, gj = gj0 + lj;
-        const bool v = e < WD * WD && in_axis(L.tx, gi, L.mx) && in_axis(L.ty, gj, L.my);
-        fv[k] = v ? f[wwrap(gi, L.mx) + wwrap(gj, L.my) * L.mx] : 0.0f;
-    }
+        for (int k = 0; k < NE; k++) {
+            const int e = tid + NT * k;
+            const int li = e % WD, lj = e / WD;
+            const int gi = gi0 + li, gj = gj0 + lj;
+            const bool v = e < WD * WD && in_axis(L.tx, gi, L.mx) && in_axis(L.ty, gj, L.my);
+            fv[k] = v ? f[wr(gi, L.mx) + wr(gj, L.my) * L.mx] : 0.0f;
+        }
+    };
+    if (L.mx >= WD && L.my >= WD) load(wnear);
+    else load(wwrap);
 #pragma unroll
     for (int k = 0; k < NE; k++) {
         const int e = tid + NT * k;
@@ -258,59 +267,52 @@ __global__ __launch_bounds__(NT) void k_mg_down(const float *f, float *u, float 
         }
     }
     __syncthreads();
-    // half-sweeps 1 .. 2NS-1 (black, red, ...) on bands 1 .. 2NS-1
+    // half-sweeps 1 .. 2NS-1 (black, red, ...) on bands 1 .. 2NS-1; the
+    // threads take the band's points of the colour in row order (every
+    // band row holds hw of them: WD is even)
 #pragma unroll
     for (int h = 1; h < 2 * NS; h++) {
         const int col = h & 1;   // 1 black, 0 red
-        for (int lj = h + ty; lj < WD - h; lj += BY) {
+        const int hw = (WD - 2 * h) / 2;
+        for (int p = tid; p < hw * (WD - 2 * h); p += NT) {
+            const int r = p / hw, lj = h + r;
             const int gj = gj0 + lj;
-            if (!in_axis(L.ty, gj, L.my)) continue;
-            for (int li = h + tx; li < WD - h; li += 32) {
-                const int gi = gi0 + li, k = li + lj * WD;
-                if (in_axis(L.tx, gi, L.mx) && colour(gi, gj) == col)
-                    su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WD, L, k, gi, gj)) * invd;
-            }
+            const int li = h + ((col - gj - gi0 - h) & 1) + 2 * (p - r * hw);
+            const int gi = gi0 + li, k = li + lj * WD;
+            if (in_axis(L.ty, gj, L.my) && in_axis(L.tx, gi, L.mx))
+                su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WD, L, k, gi, gj)) * invd;
         }
         __syncthreads();
     }
     // residual on band 2NS (the tile and its rim)
-    constexpr int RB0 = 2 * NS;
-    for (int lj = RB0 + ty; lj < WD - RB0; lj += BY) {
-        const int gj = gj0 + lj;
-        const bool vj = in_axis(L.ty, gj, L.my);
-        for (int li = RB0 + tx; li < WD - RB0; li += 32) {
-            const int gi = gi0 + li, k = li + lj * WD;
-            float r = 0.0f;
-            if (vj && in_axis(L.tx, gi, L.mx))
-                r = sf[k] - (d * su[k] - nsum(su, WD, L, k, gi, gj)) * rweight(L, gi, gj);
-            sr[(li - RB0) + (lj - RB0) * (TB + 2)] = r;
-        }
+    constexpr int RB0 = 2 * NS, WR = TB + 2;
+    for (int p = tid; p < WR * WR; p += NT) {
+        const int lj = RB0 + p / WR, li = RB0 + p % WR;
+        const int gj = gj0 + lj, gi = gi0 + li, k = li + lj * WD;
+        float r = 0.0f;
+        if (in_axis(L.ty, gj, L.my) && in_axis(L.tx, gi, L.mx))
+            r = sf[k] - (d * su[k] - nsum(su, WD, L, k, gi, gj)) * rweight(L, gi, gj);
+        sr[(li - RB0) + (lj - RB0) * WR] = r;
     }
     __syncthreads();
     // pre-smoothed u of the tile
-    for (int lj = ty; lj < TB; lj += BY) {
-        const int gj = tj0 + lj;
-        if (gj >= L.my) break;
-        for (int li = tx; li < TB; li += 32) {
-            const int gi = ti0 + li;
-            if (gi < L.mx) u[gi + gj * L.mx] = su[(HD + li) + (HD + lj) * WD];
-        }
+    for (int p = tid; p < TB * TB; p += NT) {
+        const int li = p % TB, lj = p / TB;
+        const int gi = ti0 + li, gj = tj0 + lj;
+        if (gi < L.mx && gj < L.my) u[gi + gj * L.mx] = su[(HD + li) + (HD + lj) * WD];
     }
     // restriction for the coarse unknowns centred in this tile
     const int ci0 = ti0 > 0 ? ti0 / 2 - 1 : 0, cj0 = tj0 > 0 ? tj0 / 2 - 1 : 0;
     constexpr int CW = TB / 2 + 2;
-    for (int cjj = ty; cjj < CW; cjj += BY) {
-        const int J = cj0 + cjj;
-        if (J >= C.my) break;
+    for (int p = tid; p < CW * CW; p += NT) {
+        const int J = cj0 + p / CW, I = ci0 + p % CW;
+        if (J >= C.my || I >= C.mx) continue;
         const int nyc = 2 * c_node(C.ty, J);
         const int fv2 = f_unk(L.ty, nyc, L.My);
-        if (fv2 < tj0 || fv2 >= tj0 + TB) continue;
-        for (int cii = tx; cii < CW; cii += 32) {
-            const int I = ci0 + cii;
-            if (I >= C.mx) continue;
-            const int nxc = 2 * c_node(C.tx, I);
-            const int fu = f_unk(L.tx, nxc, L.Mx);
-            if (fu < ti0 || fu >= ti0 + TB) continue;
+        const int nxc = 2 * c_node(C.tx, I);
+        const int fu = f_unk(L.tx, nxc, L.Mx);
+        if (fv2 < tj0 || fv2 >= tj0 + TB || fu < ti0 || fu >= ti0 + TB) continue;
+        {
             float acc = 0.0f;
 #pragma unroll
             for (int b = -1; b <= 1; b++) {
@@ -347,36 +349,41 @@ __global__ __launch_bounds__(NT) void k_mg_up(const float *upre, const float *f,
     const int gi0 = ti0 - HU, gj0 = tj0 - HU;
     const float invd = 1.0f / (2.0f * (L.cx + L.cy));
     float uv[NE], fvv[NE];
+    auto load = [&](auto wr) {
 #pragma unroll
-    for (int k = 0; k < NE; k++) {
-        const int e = tid + NT * k;
-        const int li = e % WU, lj = e / WU;
-        const int gi = gi0 + li, gj = gj0 + lj;
-        uv[k] = 0.0f;
-        fvv[k] = 0.0f;
-        if (e < WU * WU && in_axis(L.tx, gi, L.mx) && in_axis(L.ty, gj, L.my)) {
-            const int wi = wwrap(gi, L.mx), wj = wwrap(gj, L.my), kk = wi + wj * L.mx;
-            uv[k] = upre[kk] + prolong(uc, L, C, wi, wj);
-            fvv[k] = f[kk];
+        for (int k = 0; k < NE; k++) {
+            const int e = tid + NT * k;
+            const int li = e % WU, lj = e / WU;
+            const int gi = gi0 + li, gj = gj0 + lj;
+            uv[k] = 0.0f;
+            fvv[k] = 0.0f;
+            if (e < WU * WU && in_axis(L.tx, gi, L.mx) && in_axis(L.ty, gj, L.my)) {
+                const int wi = wr(gi, L.mx), wj = wr(gj, L.my), kk = wi + wj * L.mx;
+                uv[k] = upre[kk] + prolong(uc, L, C, wi, wj);
+                fvv[k] = f[kk];
+            }
         }
-    }
+    };
+    if (L.mx >= WU && L.my >= WU) load(wnear);
+    else load(wwrap);
 #pragma unroll
     for (int k = 0; k < NE; k++) {
         const int e = tid + NT * k;
         if (e < WU * WU) { su[e] = uv[k]; sf[e] = fvv[k]; }
     }
     __syncthreads();
+    // the band's points of the colour in row order (WU is even)
 #pragma unroll
     for (int h = 1; h <= 2 * NS; h++) {
         const int col = h & 1;   // 1 black, 0 red
-        for (int lj = h + ty; lj < WU - h; lj += BY) {
+        const int hw = (WU - 2 * h) / 2;
+        for (int p = tid; p < hw * (WU - 2 * h); p += NT) {
+            const int r = p / hw, lj = h + r;
             const int gj = gj0 + lj;
-            if (!in_axis(L.ty, gj, L.my)) continue;
-            for (int li = h + tx; li < WU - h; li += 32) {
-                const int gi = gi0 + li, k = li + lj * WU;
-                if (in_axis(L.tx, gi, L.mx) && colour(gi, gj) == col)
-                    su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WU, L, k, gi, gj)) * invd;
-            }
+            const int li = h + ((col - gj - gi0 - h) & 1) + 2 * (p - r * hw);
+            const int gi = gi0 + li, k = li + lj * WU;
+            if (in_axis(L.ty, gj, L.my) && in_axis(L.tx, gi, L.mx))
+                su[k] = (sf[k] * iweight(L, gi, gj) + nsum(su, WU, L, k, gi, gj)) * invd;
         }
         __syncthreads();
     }
