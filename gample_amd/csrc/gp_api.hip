@@ -101,6 +101,8 @@ int gp_ctx_create(const gp_config *cfg, gp_ctx **out)
         int rc = gp_fields_init(c);
         if (rc) { gp_ctx_destroy(c); return rc; }
         e = hipHostMalloc((void **)&c->h_drift, 3 * sizeof(unsigned long long) * GP_MAX_DRIFT_SPECIES);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_counters, 8 * sizeof(unsigned long long) * GP_MAX_SPECIES);
+        if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_ke, sizeof(double) * GP_MAX_SPECIES);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->drift_ev, hipEventDisableTiming);
         if (e != hipSuccess) { gp_ctx_destroy(c); return gp_fail(GP_ERR_HIP, std::string("drift counters: ") + hipGetErrorString(e)); }
         *out = c;
@@ -131,6 +133,8 @@ int gp_ctx_destroy(gp_ctx *c)
     if (c->stage) (void)hipFree(c->stage);
     if (c->stage_on) (void)hipFree(c->stage_on);
     if (c->h_drift) (void)hipHostFree(c->h_drift);
+    if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->h_ke) (void)hipHostFree(c->h_ke);
     if (c->drift_ev) (void)hipEventDestroy(c->drift_ev);
     if (c->boff) (void)hipFree(c->boff);
     if (c->rng_s) (void)hipFree(c->rng_s);
@@ -526,17 +530,25 @@ int gp_step(gp_ctx *c, double dt, double tol, int maxiter, int solver, int sort_
         GP_HIP(hipEventRecord(c->drift_ev, c->stream));
         c->drift_pending = 1;
     }
-    for (int s = 0; s < (int)c->sp.size(); s++) {
-        int64_t cnt[4] = {0, 0, 0, 0};
-        double ke = 0.0;
-        if (has_delete || diag) {
-            int rc = gp_launch_reduce(c, s, cnt, diag ? &ke : nullptr);
+    if (has_delete || diag) {
+        // every species' counters in one synchronisation; the compactions
+        // then run without any, and only the diagnostics wait for the stream
+        for (int s = 0; s < nsp; s++)
+            GP_HIP(hipMemcpyAsync(c->h_counters + 8 * s, c->sp[s].counters, 8 * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, c->stream));
+        GP_HIP(hipStreamSynchronize(c->stream));
+        int64_t cnt[GP_MAX_SPECIES][4] = {};
+        for (int s = 0; s < nsp; s++) {
+            int rc = gp_launch_reduce_from(c, s, c->h_counters + 8 * s, cnt[s], diag ? c->h_ke + s : nullptr);
             if (rc) return rc;
         }
-        if (diag && s < 8) {
-            diag->kinetic[s] = ke;
-            diag->count[s] = c->sp[s].n;
-            for (int w = 0; w < 4; w++) diag->absorbed[s][w] = cnt[w];
+        if (diag) {
+            GP_HIP(hipStreamSynchronize(c->stream));
+            for (int s = 0; s < nsp && s < 8; s++) {
+                diag->kinetic[s] = c->h_ke[s];
+                diag->count[s] = c->sp[s].n;
+                for (int w = 0; w < 4; w++) diag->absorbed[s][w] = cnt[s][w];
+            }
         }
     }
     if (sort_interval > 0 && cycle % sort_interval == 0)

@@ -169,6 +169,8 @@ struct gp_ctx {
     // outside their tile windows (0: off)
     double sort_drift = 0.0;
     unsigned long long *h_drift = nullptr;   // pinned, three per species: escaped, drifted, moved
+    unsigned long long *h_counters = nullptr;   // pinned, the eight counters of each species (reduce)
+    double *h_ke = nullptr;                     // pinned, each species' kinetic energy (reduce)
     hipEvent_t drift_ev = nullptr;
     int drift_pending = 0;
     // generation streams (Collision.mm:189-336, 464-678): thread id of every
@@ -194,6 +196,8 @@ struct gp_ctx {
 int gp_launch_push(gp_ctx *c, int sid, double dt, int deposit);
 int gp_launch_deposit(gp_ctx *c, int sid, int64_t first, int64_t count);
 int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke, gp_arena *ar = nullptr);
+int gp_launch_reduce_from(gp_ctx *c, int sid, const unsigned long long *h, int64_t counts[4], double *ke,
+                          gp_arena *ar = nullptr);
 // the device's static reduce scratch (set up once per device by gp_ctx_create)
 int gp_noalloc_setup(int device);
 int gp_noalloc_arena(int device, gp_arena *ar);

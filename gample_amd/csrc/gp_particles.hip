@@ -872,6 +872,34 @@ __global__ __launch_bounds__(GP_BLOCK) void k_interleave(const uint32_t *__restr
 // live particles of the tail [n_new, n) taken from the end, which is the
 // permutation of the reference's loop (Particle.mm:906-968).
 // ---------------------------------------------------------------------------
+// ascending sort of n <= SMALL_SORT distinct keys in one workgroup: a
+// bitonic network in LDS, padded to a power of two with the largest key (the
+// few hundred to few thousand particles a bounded step absorbs, where the
+// radix sort's nine launches cost more than the sort)
+constexpr int SMALL_SORT = 8192;
+__global__ __launch_bounds__(1024) void k_sort_small(const uint32_t *in, uint32_t *out, int n)
+{
+    __shared__ uint32_t s[SMALL_SORT];
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int i = threadIdx.x; i < m; i += 1024) s[i] = i < n ? in[i] : 0xffffffffu;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += 1024) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t x = s[i], y = s[l];
+                    if ((x > y) == ((i & k) == 0)) {
+                        s[i] = y;
+                        s[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int i = threadIdx.x; i < n; i += 1024) out[i] = s[i];
+}
 __global__ void k_lower_bound(const uint32_t *a, int64_t n, uint32_t v, uint32_t *out)
 {
     int64_t lo = 0, hi = n;
@@ -1223,7 +1251,6 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
     a.pfirst = a.use_tiles ? a.n_binned : first;
     a.tile_off = S.tile_off;
     a.ntx = c->ntx; a.ntiles = c->ntiles;
-    a.chunk = 16 * push_block(c->order);
     a.Ex = c->Ex; a.Ey = c->Ey; a.Bz = c->Bz; a.bz0 = c->bz0;
     a.constE = (float)(0.5 * (double)S.qf * dt / (double)S.mf);             // Particle.mm:618
     a.constB = (float)(0.5 * (double)S.qf * dt / (double)S.mf / 2.99792458e10);  // Particle.mm:619
@@ -1252,6 +1279,11 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
     a.ke_part = S.ke_part;
     a.ke_coef = 0.5 * (double)S.mf * (double)S.wf;
     const int64_t rest = end - a.pfirst;
+    // particles outside the tile ranges go in chunks of up to 16 blocks; a
+    // launch of few of them (an injection's deposit) spreads them over up to
+    // 512 workgroups instead of leaving each of a few to loop over its chunk
+    const int pb = push_block(c->order);
+    a.chunk = pb * (int)std::min<int64_t>(16, std::max<int64_t>(1, gp_cdiv(rest, (int64_t)pb * 512)));
     const int extra = rest > 0 ? gp_cdiv(rest, a.chunk) : 0;
     const int grid = (a.use_tiles ? c->ntiles : 0) + extra;
     if (grid == 0) return GP_OK;
@@ -1367,19 +1399,21 @@ static int dead_bits(int64_t n)
     return bits;
 }
 
-int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke, gp_arena *ar)
+// compaction of one species from its counters h (read from the device by
+// the caller): no host synchronisation; ke (may be null) is filled by an
+// asynchronous copy, valid once the stream has been synchronised
+int gp_launch_reduce_from(gp_ctx *c, int sid, const unsigned long long *h, int64_t counts[4], double *ke, gp_arena *ar)
 {
     gp_species &S = c->sp[sid];
     hipStream_t s = c->stream;
-    unsigned long long h[8];
-    GP_HIP(hipMemcpyAsync(h, S.counters, sizeof(h), hipMemcpyDeviceToHost, s));
-    GP_HIP(hipStreamSynchronize(s));
     if (h[5]) return gp_fail(GP_ERR_STATE, "particles outside the domain with no boundary condition: " + std::to_string(h[5]));
     const int64_t nd = (int64_t)h[0];
     if (ar && nd > 0) {
         // the arena must hold the larger of the two phases' scratch (the
-        // sort's is released before the scan of the tail flags)
-        const size_t need = std::max(radix_scratch_bytes(nd, dead_bits(S.n)), scan_scratch_bytes(nd));
+        // sort's, none for a sort in LDS, is released before the scan of the
+        // tail flags)
+        const size_t need = std::max(nd <= SMALL_SORT ? 0 : radix_scratch_bytes(nd, dead_bits(S.n)),
+                                     scan_scratch_bytes(nd));
         if (need > ar->cap - ar->used)
             return gp_fail(GP_ERR_CAPACITY, "gp_reduce_noalloc: " + std::to_string(nd) + " absorbed particles need " +
                                                 std::to_string(need) + " B of scratch, more than the " +
@@ -1394,8 +1428,15 @@ int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke, gp_arena
         const int64_t n_new = S.n - nd;
         // sort the absorbed indices ascending
         uint32_t *ks = nullptr, *vs = nullptr;
-        int rc = gp_radix_sort_u32(s, S.dead, S.val, S.dead_tmp, S.val2, nd, dead_bits(S.n), 1, &ks, &vs, ar);
-        if (rc) return rc;
+        int rc = GP_OK;
+        if (nd <= SMALL_SORT) {
+            k_sort_small<<<1, 1024, 0, s>>>(S.dead, S.dead_tmp, (int)nd);
+            GP_HIP(hipGetLastError());
+            ks = S.dead_tmp;
+        } else {
+            rc = gp_radix_sort_u32(s, S.dead, S.val, S.dead_tmp, S.val2, nd, dead_bits(S.n), 1, &ks, &vs, ar);
+            if (rc) return rc;
+        }
         uint32_t *nh = S.key2;         // device scalar: number of holes
         uint32_t *flags = S.key;       // tail flags -> positions (nd)
         uint32_t *donor = S.key2 + 1;  // donors (<= nd)
@@ -1414,6 +1455,18 @@ int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke, gp_arena
         if (S.n_binned > S.n) S.n_binned = S.n;
     }
     GP_HIP(hipMemsetAsync(S.counters, 0, sizeof(unsigned long long) * 8, s));
+    return GP_OK;
+}
+
+int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke, gp_arena *ar)
+{
+    gp_species &S = c->sp[sid];
+    hipStream_t s = c->stream;
+    unsigned long long *h = c->h_counters + 8 * sid;
+    GP_HIP(hipMemcpyAsync(h, S.counters, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    const int rc = gp_launch_reduce_from(c, sid, h, counts, ke, ar);
+    if (rc) return rc;
     GP_HIP(hipStreamSynchronize(s));
     return GP_OK;
 }
