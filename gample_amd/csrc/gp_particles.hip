@@ -874,9 +874,9 @@ __global__ __launch_bounds__(GP_BLOCK) void k_interleave(const uint32_t *__restr
 // ---------------------------------------------------------------------------
 // ascending sort of n <= SMALL_SORT distinct keys in one workgroup: a
 // bitonic network in LDS, padded to a power of two with the largest key (the
-// few hundred to few thousand particles a bounded step absorbs, where the
+// hundreds to about ten thousand particles a bounded step absorbs, where the
 // radix sort's nine launches cost more than the sort)
-constexpr int SMALL_SORT = 8192;
+constexpr int SMALL_SORT = 16384;   // 64 KB of LDS
 __global__ __launch_bounds__(1024) void k_sort_small(const uint32_t *in, uint32_t *out, int n)
 {
     __shared__ uint32_t s[SMALL_SORT];
