@@ -125,6 +125,33 @@ __device__ __forceinline__ float gather_lds(const float *b, const float *sx, con
     return acc;
 }
 
+// gather_lds for 4x4 stencils from a window held in 4 copies, copy c shifted
+// left by c elements (copy c, element k = window element k + c; each copy W2
+// long, pitch a multiple of 4): a stencil row of 4 taps starting at window
+// element s is the aligned 16-byte read at element s - c of copy c = s & 3,
+// one ds_read_b128 in place of two ds_read2_b32.  The arithmetic is
+// gather_lds's, term for term.
+template <int PITCH, int W2>
+__device__ __forceinline__ float gather_lds4(const float *b4, int x0, int y0, const float *sx, const float *sy)
+{
+    static_assert(PITCH % 4 == 0 && W2 % 4 == 0, "copies and rows must stay 16-byte aligned");
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const int c = x0 & 3;
+    const f4v *p = (const f4v *)(b4 + c * W2 + (x0 - c) + rowW<PITCH>(y0));
+    f4v rw[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) rw[j] = p[j * (PITCH / 4)];
+    float acc = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        float r = sy[0] * rw[0][i];
+#pragma unroll
+        for (int j = 1; j < 4; j++) r = __builtin_fmaf(sy[j], rw[j][i], r);
+        acc = (i == 0) ? sx[0] * r : __builtin_fmaf(sx[i], r, acc);
+    }
+    return acc;
+}
+
 // exact round-to-nearest-even of c * 2^40 as int64 (|c| < 2^22)
 __device__ __forceinline__ long long fx40(float c)
 {
@@ -198,12 +225,18 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
     // node window edge: relative to X0 = cx0 - NGB the stencils reach from 0
     // (rho: i2 - NGB with i2 = cx0) to WIN + NW - 1 (Ex, jx: i1 + 1 - NGB +
     // NW - 1 with i1 = i2 = cx0 + WIN - 1; likewise Ey, jy in y)
-    constexpr int W = WIN + NW;
-    constexpr int NF = PUSH ? W * W : 1;
+    constexpr int W = WIN + NW, W2 = W * W;
+    // the 3rd-order charge-deposit push with uniform B reads its E windows
+    // from 4 shifted copies (gather_lds4): the random-cell stencil rows of a
+    // wave conflict in fewer LDS banks as 16-byte reads, which outweighs the
+    // occupancy the larger windows cost (2 workgroups a CU instead of 3)
+    constexpr bool G4 = ORDER == 3 && UNIB && DEP == 1 && PUSH;
+    constexpr int NF = PUSH ? (G4 ? 4 * W2 : W2) : 1;
     constexpr int NB = (PUSH && !UNIB) ? W * W : 1;
     constexpr int NR = DEP >= 1 ? W * W : 1;
     constexpr int NJ = DEP >= 2 ? W * W : 1;
-    __shared__ float sEx[NF], sEy[NF], sBz[NB];
+    __shared__ __attribute__((aligned(16))) float sEx[NF], sEy[NF];
+    __shared__ float sBz[NB];
     __shared__ unsigned long long sRho[NR], sJx[NJ], sJy[NJ];
     __shared__ double sKE[PB / 64];
 
@@ -234,7 +267,13 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
     if (tiled && p0 < p1) {
         for (int e = tid; e < W * W; e += PB) {
             const int gx = X0 + e % W, gy = Y0 + e / W;
-            if constexpr (PUSH) {
+            if constexpr (G4) {
+                const float ex = (gx >= 0 && gx < a.nx + 2 && gy >= 0 && gy < a.ny + 1) ? a.Ex[gx + gy * pex] : 0.0f;
+                const float ey = (gx >= 0 && gx < a.nx + 1 && gy >= 0 && gy < a.ny + 2) ? a.Ey[gx + gy * pey] : 0.0f;
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                    if (e >= c) { sEx[c * W2 + e - c] = ex; sEy[c * W2 + e - c] = ey; }
+            } else if constexpr (PUSH) {
                 sEx[e] = (gx >= 0 && gx < a.nx + 2 && gy >= 0 && gy < a.ny + 1) ? a.Ex[gx + gy * pex] : 0.0f;
                 sEy[e] = (gx >= 0 && gx < a.nx + 1 && gy >= 0 && gy < a.ny + 2) ? a.Ey[gx + gy * pey] : 0.0f;
                 if constexpr (!UNIB)
@@ -315,8 +354,13 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
             oi = i2; oj = j1;
             if (inw) {
                 // (window rows are non-negative here: 24-bit multiplies)
-                Epx = gather_lds<NW, W>(sEx + (i1 + 1 - NGB - X0) + rowW<W>(j1 - NGB - Y0), sx0, sy0);
-                Epy = gather_lds<NW, W>(sEy + (i2 - NGB - X0) + rowW<W>(j2 + 1 - NGB - Y0), sx1, sy1);
+                if constexpr (G4) {
+                    Epx = gather_lds4<W, W2>(sEx, i1 + 1 - NGB - X0, j1 - NGB - Y0, sx0, sy0);
+                    Epy = gather_lds4<W, W2>(sEy, i2 - NGB - X0, j2 + 1 - NGB - Y0, sx1, sy1);
+                } else {
+                    Epx = gather_lds<NW, W>(sEx + (i1 + 1 - NGB - X0) + rowW<W>(j1 - NGB - Y0), sx0, sy0);
+                    Epy = gather_lds<NW, W>(sEy + (i2 - NGB - X0) + rowW<W>(j2 + 1 - NGB - Y0), sx1, sy1);
+                }
                 if constexpr (UNIB) Bpz = a.bz0;
                 else Bpz = gather_lds<NW, W>(sBz + (i1 + 1 - NGB - X0) + rowW<W>(j2 + 1 - NGB - Y0), sx0, sy1);
             } else {
