@@ -16,19 +16,21 @@ int gp_fail(int code, const std::string &msg)
     return code;
 }
 
-int gp_time_mark(gp_ctx *c, int which)
+int gp_time_mark(gp_ctx *c, int which, int which2)
 {
     hipEvent_t e;
     GP_HIP(hipEventCreate(&e));
+    c->tev_all.push_back(e);
     GP_HIP(hipEventRecord(e, c->stream));
     c->tev[which].push_back(e);
+    if (which2 >= 0 && which2 < GP_TIME_SLOTS) c->tev[which2].push_back(e);
     return GP_OK;
 }
 
 int gp_time_collect(gp_ctx *c)
 {
     GP_HIP(hipStreamSynchronize(c->stream));
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < GP_TIME_SLOTS; k++) {
         auto &v = c->tev[k];
         for (size_t i = 0; i + 1 < v.size(); i += 2) {
             float ms = 0.f;
@@ -36,9 +38,11 @@ int gp_time_collect(gp_ctx *c)
             c->t_ms[k] += ms;
             c->t_n[k] += 1;
         }
-        for (hipEvent_t e : v) (void)hipEventDestroy(e);
+        // an unpaired begin (a launch that failed after it) is dropped
         v.clear();
     }
+    for (hipEvent_t e : c->tev_all) (void)hipEventDestroy(e);
+    c->tev_all.clear();
     return GP_OK;
 }
 
@@ -126,8 +130,7 @@ int gp_ctx_destroy(gp_ctx *c)
     if (!c) return GP_OK;
     (void)hipDeviceSynchronize();
     for (auto &S : c->sp) species_free(S);
-    for (auto &v : c->tev)
-        for (hipEvent_t e : v) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->tev_all) (void)hipEventDestroy(e);
     gp_nn_destroy(c);
     gp_fields_free(c);
     if (c->stage) (void)hipFree(c->stage);
@@ -402,9 +405,9 @@ int gp_solve_fields(gp_ctx *c, double dt, double tol, int maxiter, int solver, g
     GP_CHECK_CTX(c);
     if (!(tol > 0) || maxiter < 1) return gp_fail(GP_ERR_ARG, "tol must be > 0 and maxiter >= 1");
     GP_TRY({
-        if (c->timing) { int rc = gp_time_mark(c, 2); if (rc) return rc; }
+        if (c->timing) { int rc = gp_time_mark(c, GP_TIME_SOLVE); if (rc) return rc; }
         int rc = gp_solve_fields_impl(c, dt, tol, maxiter, solver, st, diag);
-        if (c->timing && rc >= 0) { int r2 = gp_time_mark(c, 2); if (r2) return r2; }
+        if (c->timing && rc >= 0) { int r2 = gp_time_mark(c, GP_TIME_SOLVE); if (r2) return r2; }
         return rc;
     })
 }
@@ -646,14 +649,14 @@ int gp_timing(gp_ctx *c, int enable)
     int rc = gp_time_collect(c);
     if (rc) return rc;
     c->timing = enable ? 1 : 0;
-    for (int k = 0; k < 4; k++) { c->t_ms[k] = 0; c->t_n[k] = 0; }
+    for (int k = 0; k < GP_TIME_SLOTS; k++) { c->t_ms[k] = 0; c->t_n[k] = 0; }
     return GP_OK;
 }
 
 int gp_kernel_time(gp_ctx *c, int which, double *total_ms, int64_t *launches)
 {
     GP_CHECK_CTX(c);
-    if (which < 0 || which > 3 || !total_ms || !launches) return gp_fail(GP_ERR_ARG, "bad timing query");
+    if (which < 0 || which >= GP_TIME_SLOTS || !total_ms || !launches) return gp_fail(GP_ERR_ARG, "bad timing query");
     int rc = gp_time_collect(c);
     if (rc) return rc;
     *total_ms = c->t_ms[which];

@@ -27,6 +27,14 @@
 #define GP_MAX_DRIFT_SPECIES 64   // species followed by drift-triggered sorting
 #define GP_MOVED_SAMPLE 8         // one tile in 8 counts cell changes (grids of 256+ tiles)
 #define GP_MAX_SPECIES 8
+// timing slots: all pushes, all sorts, field solves, all reduces, then per
+// species s the push, reduce and sort at GP_TIME_SPECIES + 3 s + 0, 1, 2
+#define GP_TIME_PUSH 0
+#define GP_TIME_SORT 1
+#define GP_TIME_SOLVE 2
+#define GP_TIME_REDUCE 3
+#define GP_TIME_SPECIES 4
+#define GP_TIME_SLOTS (GP_TIME_SPECIES + 3 * GP_MAX_SPECIES)
 #define GP_RHO_FRAC 46       // fractional bits of the rho accumulators
 #define GP_CUR_FRAC 40       // fractional bits of current/moment accumulators
 
@@ -152,12 +160,15 @@ struct gp_ctx {
     double *hred = nullptr;         // pinned
     int have_solution = 0;
     // timing: event pairs recorded on the context stream around each launch
-    // of a category (0 push, 1 sort, 2 field solve), read back on demand
+    // of a slot (GP_TIME_* below), read back on demand; an event may close a
+    // pair in two slots (a species' push and all pushes), so the events are
+    // owned by tev_all
     int timing = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::vector<hipEvent_t> tev[4];
-    double t_ms[4] = {0, 0, 0, 0};
-    int64_t t_n[4] = {0, 0, 0, 0};
+    std::vector<hipEvent_t> tev[GP_TIME_SLOTS];
+    std::vector<hipEvent_t> tev_all;
+    double t_ms[GP_TIME_SLOTS] = {};
+    int64_t t_n[GP_TIME_SLOTS] = {};
     gp_nn *nn = nullptr;
     // staged tile windows of the deposit (gp_particles.hip)
     void *stage = nullptr;
@@ -260,7 +271,7 @@ int gp_nn_fcg_cap(const gp_ctx *c);
 int gp_nn_save_impl(gp_ctx *c, const char *path);
 int gp_nn_load_impl(gp_ctx *c, const char *path);
 
-int gp_time_mark(gp_ctx *c, int which);     // record one event of a begin/end pair
+int gp_time_mark(gp_ctx *c, int which, int which2 = -1);   // one event of a begin/end pair in one or two slots
 int gp_time_collect(gp_ctx *c);             // synchronise and fold pairs into t_ms
 
 static inline int gp_cdiv(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }

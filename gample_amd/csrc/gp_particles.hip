@@ -40,7 +40,13 @@ constexpr int WIN = GP_TILE + 2 * GP_DRIFT;   // cell window of a tile
 #define PUSH_WAVES 6
 #endif
 constexpr int push_block(int order) { return order <= 3 ? PUSH_BLOCK : 256; }
-#define PUSH_OCC __attribute__((amdgpu_waves_per_eu(ORDER <= 3 && DEP <= 1 ? PUSH_WAVES : 1)))
+// the 3rd-order charge-deposit push with uniform B reads its E windows from
+// 4 shifted copies (gather_lds4): the random-cell stencil rows of a wave
+// conflict in fewer LDS banks as 16-byte reads, which outweighs the
+// occupancy the larger windows cost (77.5 KB: 2 workgroups, 4 waves per
+// SIMD, instead of 3)
+constexpr bool push_g4(int order, bool unib, int dep, bool push) { return order == 3 && unib && dep == 1 && push; }
+#define PUSH_OCC __attribute__((amdgpu_waves_per_eu(push_g4(ORDER, UNIB, DEP, PUSH) ? 4 : (ORDER <= 3 && DEP <= 1 ? PUSH_WAVES : 1))))
 
 // ---------------------------------------------------------------------------
 // B-spline weights.  Horner forms of the reference polynomials
@@ -226,11 +232,7 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
     // (rho: i2 - NGB with i2 = cx0) to WIN + NW - 1 (Ex, jx: i1 + 1 - NGB +
     // NW - 1 with i1 = i2 = cx0 + WIN - 1; likewise Ey, jy in y)
     constexpr int W = WIN + NW, W2 = W * W;
-    // the 3rd-order charge-deposit push with uniform B reads its E windows
-    // from 4 shifted copies (gather_lds4): the random-cell stencil rows of a
-    // wave conflict in fewer LDS banks as 16-byte reads, which outweighs the
-    // occupancy the larger windows cost (2 workgroups a CU instead of 3)
-    constexpr bool G4 = ORDER == 3 && UNIB && DEP == 1 && PUSH;
+    constexpr bool G4 = push_g4(ORDER, UNIB, DEP, PUSH);
     constexpr int NF = PUSH ? (G4 ? 4 * W2 : W2) : 1;
     constexpr int NB = (PUSH && !UNIB) ? W * W : 1;
     constexpr int NR = DEP >= 1 ? W * W : 1;
@@ -1422,10 +1424,11 @@ int gp_launch_push(gp_ctx *c, int sid, double dt, int deposit)
 {
     gp_species &S = c->sp[sid];
     const int dep = deposit ? (c->cfg.cur_dens ? 2 : 1) : 0;
-    if (c->timing && S.n > 0) { int rc = gp_time_mark(c, 0); if (rc) return rc; }
+    const int slot = GP_TIME_SPECIES + 3 * sid;
+    if (c->timing && S.n > 0) { int rc = gp_time_mark(c, GP_TIME_PUSH, slot); if (rc) return rc; }
     int rc = run_particle_kernel(c, sid, dt, dep, true, 0, S.n);
     if (rc) return rc;
-    if (c->timing && S.n > 0) { rc = gp_time_mark(c, 0); if (rc) return rc; }
+    if (c->timing && S.n > 0) { rc = gp_time_mark(c, GP_TIME_PUSH, slot); if (rc) return rc; }
     return GP_OK;
 }
 
@@ -1464,6 +1467,8 @@ int gp_launch_reduce_from(gp_ctx *c, int sid, const unsigned long long *h, int64
                                                 std::to_string(ar->cap - ar->used) + " B it has");
     }
     for (int w = 0; w < 4; w++) counts[w] = (int64_t)h[1 + w];
+    const int slot = GP_TIME_SPECIES + 3 * sid + 1;
+    if (c->timing) { int rc = gp_time_mark(c, GP_TIME_REDUCE, slot); if (rc) return rc; }
     if (ke) {
         GP_HIP(hipMemcpyAsync(ke, S.ke_out, sizeof(double), hipMemcpyDeviceToHost, s));
     }
@@ -1499,6 +1504,7 @@ int gp_launch_reduce_from(gp_ctx *c, int sid, const unsigned long long *h, int64
         if (S.n_binned > S.n) S.n_binned = S.n;
     }
     GP_HIP(hipMemsetAsync(S.counters, 0, sizeof(unsigned long long) * 8, s));
+    if (c->timing) { int rc = gp_time_mark(c, GP_TIME_REDUCE, slot); if (rc) return rc; }
     return GP_OK;
 }
 
@@ -1528,7 +1534,8 @@ int gp_launch_sort(gp_ctx *c, int sid)
         S.n_binned = 0;
         return GP_OK;
     }
-    if (c->timing) { int rc = gp_time_mark(c, 1); if (rc) return rc; }
+    const int slot = GP_TIME_SPECIES + 3 * sid + 2;
+    if (c->timing) { int rc = gp_time_mark(c, GP_TIME_SORT, slot); if (rc) return rc; }
     S.nsorts++;
     const int g = gp_cdiv(n, GP_BLOCK);
     const int W = WIN + c->order + 1;
@@ -1559,7 +1566,7 @@ int gp_launch_sort(gp_ctx *c, int sid)
     }
     std::swap(S.a, S.b);
     S.n_binned = n;
-    if (c->timing) { int rc = gp_time_mark(c, 1); if (rc) return rc; }
+    if (c->timing) { int rc = gp_time_mark(c, GP_TIME_SORT, slot); if (rc) return rc; }
     return GP_OK;
 }
 

@@ -11,9 +11,13 @@
 // reference's sections (solvePoisson, solveVectorPotential, outputField,
 // flowout_<species>, outputMoment_<species>, artificialIonization,
 // hollowCathode, elapsedTime, memoryUsage) and _timeseries.csv a probe row.
-// Under the fused step elapsedTime holds gp_step and the sources; the
-// reference-order pieces time integCDens_, solvePoisson, update_, reduce_.
+// Under the fused step elapsedTime holds gp_step, the sources and the
+// step's phases as the device timed them with stream events: solvePoisson,
+// update_<s> (the push with the next cycle's deposit fused in, so
+// integCDens_<s> is 0), reduce_<s> and sort_<s>; the reference-order pieces
+// time integCDens_, solvePoisson, update_, reduce_ on the host.
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -170,6 +174,14 @@ int main(int argc, char **argv)
         const int sort_every = init.paramForComputing.sortInterval;
         auto t_start = std::chrono::steady_clock::now();
         const int LO = T.LogOutput;
+        // device phase times of a logged fused step (HIP events on the
+        // context stream, recorded for that step only)
+        auto device_us = [&](int which) {
+            double ms = 0.0;
+            int64_t n = 0;
+            check(gp_kernel_time(fld.ctx, which, &ms, &n), "gp_kernel_time");
+            return std::to_string((long long)std::llround(ms * 1000.0));
+        };
         double lastEE = 0.0;
         for (int cycle = T.Start + 1; cycle <= T.End; cycle++) {
             const bool logc = LO > 0 && cycle % LO == 0;
@@ -227,10 +239,22 @@ int main(int argc, char **argv)
                                       outputMoments(cycle, fld, mom.integrateMoments(p), p.pName, init));
             };
             if (fused) {
+                if (logc) check(gp_timing(fld.ctx, 1), "gp_timing");
                 timed("gp_step", [&] {
                     check(gp_step(fld.ctx, dt, fld.tol, fld.maxiter, fld.solver, sort_every, cycle, &st,
                                   want ? &dg : nullptr), "gp_step");
                 });
+                if (logc) {
+                    elapsed["solvePoisson"] = device_us(2);
+                    for (size_t s = 0; s < ptcl.size(); s++) {
+                        const int b = 4 + 3 * (int)s;
+                        elapsed["integCDens_" + ptcl[s].pName] = "0";
+                        elapsed["update_" + ptcl[s].pName] = device_us(b);
+                        elapsed["reduce_" + ptcl[s].pName] = device_us(b + 1);
+                        elapsed["sort_" + ptcl[s].pName] = device_us(b + 2);
+                    }
+                    check(gp_timing(fld.ctx, 0), "gp_timing");
+                }
                 // the fields after the push are those of the solve (the push
                 // changes no field), so the field outputs follow the step
                 field_sections();

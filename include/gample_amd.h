@@ -285,9 +285,12 @@ int  gp_field_set(gp_ctx *ctx, int which, const void *host, int64_t bytes);
 int  gp_field_energy(gp_ctx *ctx, double dt, double *we_ref, double *we_cgs, double *joule);
 
 /* ---- timing ----------------------------------------------------------- */
-/* HIP-event timing of the push kernel on the context stream: enable = 1
- * starts recording, and gp_kernel_time returns total ms and launch count
- * since enabling.  which: 0 push(+deposit), 1 sort, 2 field solve. */
+/* HIP-event timing on the context stream: enable = 1 starts recording, and
+ * gp_kernel_time returns the total ms and launch count since enabling of
+ * which = 0 every push (+deposit), 1 every sort, 2 field solves, 3 every
+ * reduce, and for species s: 4 + 3 s its push, 5 + 3 s its reduce, 6 + 3 s
+ * its sort (the phases main.mm:176-228 times per cycle as update_<s>,
+ * reduce_<s> and solvePoisson). */
 int  gp_timing(gp_ctx *ctx, int enable);
 int  gp_kernel_time(gp_ctx *ctx, int which, double *total_ms, int64_t *launches);
 
