@@ -622,8 +622,8 @@ int gp_field_set(gp_ctx *c, int which, const void *host, int64_t bytes)
     void *dst = nullptr;
     int64_t need = 0;
     switch (which) {
-    case GP_F_EX: dst = c->Ex; need = (nx + 2) * (ny + 1) * 4; break;
-    case GP_F_EY: dst = c->Ey; need = (nx + 1) * (ny + 2) * 4; break;
+    case GP_F_EX: dst = c->Ex; need = (nx + 2) * (ny + 1) * 4; c->e_periodic = 0; break;
+    case GP_F_EY: dst = c->Ey; need = (nx + 1) * (ny + 2) * 4; c->e_periodic = 0; break;
     case GP_F_EZ: dst = c->Ez; need = (nx + 1) * (ny + 1) * 4; break;
     case GP_F_BX: dst = c->Bx; need = (nx + 1) * (ny + 2) * 4; break;
     case GP_F_BY: dst = c->By; need = (nx + 2) * (ny + 1) * 4; break;

@@ -538,6 +538,7 @@ int gp_solve_fields_impl(gp_ctx *c, double dt, double tol, int maxiter, int solv
     k_gradient<<<g2(c->nx + 2, c->ny + 2), b2, 0, sm>>>(c->phi, c->Ex, c->Ey, c->nx, c->ny, (double)c->dxf, (double)c->dyf);
     GP_HIP(hipGetLastError());
     c->have_solution = 1;
+    c->e_periodic = 1;   // phi repeats its periodic nodes exactly (k_reconstruct)
     if (diag) {
         rc = gp_energy_impl(c, dt, &diag->we_ref, &diag->we_cgs, &diag->joule);
         if (rc) return rc;

@@ -28,13 +28,15 @@
 #define GP_MOVED_SAMPLE 8         // one tile in 8 counts cell changes (grids of 256+ tiles)
 #define GP_MAX_SPECIES 8
 // timing slots: all pushes, all sorts, field solves, all reduces, then per
-// species s the push, reduce and sort at GP_TIME_SPECIES + 3 s + 0, 1, 2
+// species s the push, reduce and sort at GP_TIME_SPECIES + 3 s + 0, 1, 2,
+// then the push kernel alone
 #define GP_TIME_PUSH 0
 #define GP_TIME_SORT 1
 #define GP_TIME_SOLVE 2
 #define GP_TIME_REDUCE 3
 #define GP_TIME_SPECIES 4
-#define GP_TIME_SLOTS (GP_TIME_SPECIES + 3 * GP_MAX_SPECIES)
+#define GP_TIME_PUSHK (GP_TIME_SPECIES + 3 * GP_MAX_SPECIES)   // the push kernel launch alone
+#define GP_TIME_SLOTS (GP_TIME_PUSHK + 1)
 #define GP_RHO_FRAC 46       // fractional bits of the rho accumulators
 #define GP_CUR_FRAC 40       // fractional bits of current/moment accumulators
 
@@ -151,6 +153,9 @@ struct gp_ctx {
     double *rho = nullptr, *jx = nullptr, *jy = nullptr, *phi = nullptr;
     int uniform_b = 1;
     float bz0 = 0;
+    // Ex, Ey exactly periodic on the periodic axes (uniform at set-up, or the
+    // gradient of the periodic potential); cleared when a caller sets them
+    int e_periodic = 1;
     gp_solver ps;                   // Poisson
     gp_solver ds;                   // Darwin (Neumann/periodic, singular)
     double *dsol_chi = nullptr, *dsol_ax = nullptr, *dsol_ay = nullptr, *last_ax = nullptr, *last_ay = nullptr;

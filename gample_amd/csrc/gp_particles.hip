@@ -40,13 +40,11 @@ constexpr int WIN = GP_TILE + 2 * GP_DRIFT;   // cell window of a tile
 #define PUSH_WAVES 6
 #endif
 constexpr int push_block(int order) { return order <= 3 ? PUSH_BLOCK : 256; }
-// the 3rd-order charge-deposit push with uniform B reads its E windows from
-// 4 shifted copies (gather_lds4): the random-cell stencil rows of a wave
-// conflict in fewer LDS banks as 16-byte reads, which outweighs the
-// occupancy the larger windows cost (77.5 KB: 2 workgroups, 4 waves per
-// SIMD, instead of 3)
-constexpr bool push_g4(int order, bool unib, int dep, bool push) { return order == 3 && unib && dep == 1 && push; }
-#define PUSH_OCC __attribute__((amdgpu_waves_per_eu(push_g4(ORDER, UNIB, DEP, PUSH) ? 4 : (ORDER <= 3 && DEP <= 1 ? PUSH_WAVES : 1))))
+// the 3rd-order charge-deposit push with uniform B (the hot kernel) stages
+// its E windows as column coefficients (coef3): 16-byte rows in LDS, and no
+// per-particle y weights (72 KB: 2 workgroups, 4 waves per SIMD)
+constexpr bool push_cw(int order, bool unib, int dep, bool push) { return order == 3 && unib && dep == 1 && push; }
+#define PUSH_OCC __attribute__((amdgpu_waves_per_eu(push_cw(ORDER, UNIB, DEP, PUSH) ? 4 : (ORDER <= 3 && DEP <= 1 ? PUSH_WAVES : 1))))
 
 // ---------------------------------------------------------------------------
 // B-spline weights.  Horner forms of the reference polynomials
@@ -131,28 +129,47 @@ __device__ __forceinline__ float gather_lds(const float *b, const float *sx, con
     return acc;
 }
 
-// gather_lds for 4x4 stencils from a window held in 4 copies, copy c shifted
-// left by c elements (copy c, element k = window element k + c; each copy W2
-// long, pitch a multiple of 4): a stencil row of 4 taps starting at window
-// element s is the aligned 16-byte read at element s - c of copy c = s & 3,
-// one ds_read_b128 in place of two ds_read2_b32.  The arithmetic is
-// gather_lds's, term for term.
-template <int PITCH, int W2>
-__device__ __forceinline__ float gather_lds4(const float *b4, int x0, int y0, const float *sx, const float *sy)
+// Cubic gather along y as a polynomial in the fraction h.  The four taps
+// f0..f3 of a stencil column weighted by the order-3 weights w0..w3(h) are
+// d0 + h (d1 + h (d2 + h d3)) with
+//   d0 = (f0 + 4 f1 + f2) / 6,  d1 = (f2 - f0) / 2,
+//   d2 = (f0 + f2) / 2 - f1,    d3 = (f3 - f0 + 3 (f1 - f2)) / 6
+// (the coefficients of h^k in w0..w3, Particle.mm:97-131).  The gather is
+// then sum_i sx[i] r_i over the columns: two weight sets per gathered field
+// instead of four, and three fmas per column instead of four.  coef3 is the
+// one definition used by every path (the staged coefficient windows, the tap
+// windows and global memory), so all give the same bits.
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f4v coef3(float f0, float f1, float f2, float f3)
 {
-    static_assert(PITCH % 4 == 0 && W2 % 4 == 0, "copies and rows must stay 16-byte aligned");
-    typedef float f4v __attribute__((ext_vector_type(4)));
-    const int c = x0 & 3;
-    const f4v *p = (const f4v *)(b4 + c * W2 + (x0 - c) + rowW<PITCH>(y0));
-    f4v rw[4];
+    const float a02 = f0 + f2;
+    f4v d;
+    d[0] = __builtin_fmaf(f1, 2.0f / 3.0f, a02 * (1.0f / 6.0f));
+    d[1] = (f2 - f0) * 0.5f;
+    d[2] = __builtin_fmaf(a02, 0.5f, -f1);
+    d[3] = __builtin_fmaf(f1 - f2, 3.0f, f3 - f0) * (1.0f / 6.0f);
+    return d;
+}
+__device__ __forceinline__ float horner3(const f4v d, float h)
+{
+    return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(d[3], h, d[2]), h, d[1]), h, d[0]);
+}
+// from a coefficient window: q -> the 4 consecutive columns of the stencil
+__device__ __forceinline__ float gather_cw(const f4v *q, float h, const float *sx)
+{
+    float acc = sx[0] * horner3(q[0], h);
 #pragma unroll
-    for (int j = 0; j < 4; j++) rw[j] = p[j * (PITCH / 4)];
+    for (int i = 1; i < 4; i++) acc = __builtin_fmaf(sx[i], horner3(q[i], h), acc);
+    return acc;
+}
+// from taps at b[i + j pitch] (LDS window or global field)
+template <typename P>
+__device__ __forceinline__ float gather_c3(P b, int pitch, float h, const float *sx)
+{
     float acc = 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        float r = sy[0] * rw[0][i];
-#pragma unroll
-        for (int j = 1; j < 4; j++) r = __builtin_fmaf(sy[j], rw[j][i], r);
+        const float r = horner3(coef3(b[i], b[i + pitch], b[i + 2 * pitch], b[i + 3 * pitch]), h);
         acc = (i == 0) ? sx[0] * r : __builtin_fmaf(sx[i], r, acc);
     }
     return acc;
@@ -210,6 +227,9 @@ struct PushArgs {
     // image, and the windows are folded onto the interior nodes as the
     // reference's ghost fold would (EMField.mm:749-850); rho deposit only
     int imgx, imgy;
+    // E gathered at periodic images on the edge tiles: only while E is exactly
+    // periodic (gp_ctx::e_periodic) and B uniform
+    int gimgx, gimgy;
     int moved_mask;   // tiles b with (b & moved_mask) == 0 count cell changes
     unsigned long long *rho, *jx, *jy;
     unsigned long long *stage;   // per tile: W*W rho window, then jx and jy (DEP 2)
@@ -231,13 +251,18 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
     // node window edge: relative to X0 = cx0 - NGB the stencils reach from 0
     // (rho: i2 - NGB with i2 = cx0) to WIN + NW - 1 (Ex, jx: i1 + 1 - NGB +
     // NW - 1 with i1 = i2 = cx0 + WIN - 1; likewise Ey, jy in y)
-    constexpr int W = WIN + NW, W2 = W * W;
-    constexpr bool G4 = push_g4(ORDER, UNIB, DEP, PUSH);
-    constexpr int NF = PUSH ? (G4 ? 4 * W2 : W2) : 1;
+    constexpr int W = WIN + NW;
+    // CW: E staged as column coefficients (push_cw); CROWS cell rows of them
+    constexpr bool CW = push_cw(ORDER, UNIB, DEP, PUSH);
+    constexpr int CROWS = WIN + 1;
+    constexpr int NF = (PUSH && !CW) ? W * W : 1;
+    constexpr int NC = CW ? CROWS * W : 1;
     constexpr int NB = (PUSH && !UNIB) ? W * W : 1;
     constexpr int NR = DEP >= 1 ? W * W : 1;
     constexpr int NJ = DEP >= 2 ? W * W : 1;
-    __shared__ __attribute__((aligned(16))) float sEx[NF], sEy[NF];
+    static_assert(!CW || NR * 2 >= 2 * W * W, "CW stages the E taps in the charge window");
+    __shared__ f4v sCx[NC], sCy[NC];
+    __shared__ float sEx[NF], sEy[NF];
     __shared__ float sBz[NB];
     __shared__ unsigned long long sRho[NR], sJx[NJ], sJy[NJ];
     __shared__ double sKE[PB / 64];
@@ -262,25 +287,45 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
     // images are needed only by the tiles on the periodic edges
     const bool img_x = a.imgx && tiled && (b % a.ntx == 0 || b % a.ntx == a.ntx - 1);
     const bool img_y = a.imgy && tiled && (b < a.ntx || b >= a.ntiles - a.ntx);
+    // gather images: the E window of an edge tile holds, beyond the array, the
+    // values one period away (a.gimgx/y only while the fields are periodic)
+    const bool gimg_x = a.gimgx && tiled && (b % a.ntx == 0 || b % a.ntx == a.ntx - 1);
+    const bool gimg_y = a.gimgy && tiled && (b < a.ntx || b >= a.ntiles - a.ntx);
     const bool count_moved = (b & a.moved_mask) == 0;
     const int X0 = cx0 - NGB, Y0 = cy0 - NGB;                // first window node
     const int pex = a.nx + 2, pey = a.nx + 1;
+    // field node (gx, gy) of an array with ex x ey nodes and row pitch ex for
+    // a window: wrapped by one period on gather-imaged axes, 0 outside
+    auto fnode = [&](const float *F, int gx, int gy, int ex, int ey) -> float {
+        if (gimg_x) gx = gx < 0 ? gx + a.ngx : (gx >= ex ? gx - a.ngx : gx);
+        if (gimg_y) gy = gy < 0 ? gy + a.ngy : (gy >= ey ? gy - a.ngy : gy);
+        return (gx >= 0 && gx < ex && gy >= 0 && gy < ey) ? F[gx + gy * ex] : 0.0f;
+    };
 
     if (tiled && p0 < p1) {
+        if constexpr (CW) {
+            // E taps into the (not yet used) charge window, then each column's
+            // cubic coefficients over cell rows r .. r+3
+            float *tEx = (float *)sRho, *tEy = tEx + W * W;
+            for (int e = tid; e < W * W; e += PB) {
+                const int gx = X0 + e % W, gy = Y0 + e / W;
+                tEx[e] = fnode(a.Ex, gx, gy, a.nx + 2, a.ny + 1);
+                tEy[e] = fnode(a.Ey, gx, gy, a.nx + 1, a.ny + 2);
+            }
+            __syncthreads();
+            for (int e = tid; e < CROWS * W; e += PB) {
+                sCx[e] = coef3(tEx[e], tEx[e + W], tEx[e + 2 * W], tEx[e + 3 * W]);
+                sCy[e] = coef3(tEy[e], tEy[e + W], tEy[e + 2 * W], tEy[e + 3 * W]);
+            }
+            __syncthreads();
+        }
         for (int e = tid; e < W * W; e += PB) {
             const int gx = X0 + e % W, gy = Y0 + e / W;
-            if constexpr (G4) {
-                const float ex = (gx >= 0 && gx < a.nx + 2 && gy >= 0 && gy < a.ny + 1) ? a.Ex[gx + gy * pex] : 0.0f;
-                const float ey = (gx >= 0 && gx < a.nx + 1 && gy >= 0 && gy < a.ny + 2) ? a.Ey[gx + gy * pey] : 0.0f;
-#pragma unroll
-                for (int c = 0; c < 4; c++)
-                    if (e >= c) { sEx[c * W2 + e - c] = ex; sEy[c * W2 + e - c] = ey; }
-            } else if constexpr (PUSH) {
-                sEx[e] = (gx >= 0 && gx < a.nx + 2 && gy >= 0 && gy < a.ny + 1) ? a.Ex[gx + gy * pex] : 0.0f;
-                sEy[e] = (gx >= 0 && gx < a.nx + 1 && gy >= 0 && gy < a.ny + 2) ? a.Ey[gx + gy * pey] : 0.0f;
-                if constexpr (!UNIB)
-                    sBz[e] = (gx >= 0 && gx < a.nx + 2 && gy >= 0 && gy < a.ny + 2) ? a.Bz[gx + gy * pex] : 0.0f;
+            if constexpr (PUSH && !CW) {
+                sEx[e] = fnode(a.Ex, gx, gy, a.nx + 2, a.ny + 1);
+                sEy[e] = fnode(a.Ey, gx, gy, a.nx + 1, a.ny + 2);
             }
+            if constexpr (PUSH && !UNIB) sBz[e] = fnode(a.Bz, gx, gy, a.nx + 2, a.ny + 2);
             if constexpr (DEP >= 1) sRho[e] = 0ull;
             if constexpr (DEP >= 2) { sJx[e] = 0ull; sJy[e] = 0ull; }
         }
@@ -344,32 +389,59 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
         if constexpr (PUSH) {
             const float xh = x - 0.5f, yh = y - 0.5f;
             const float fi1 = floorf(xh), fj1 = floorf(y), fi2 = floorf(x), fj2 = floorf(yh);
-            const int i1 = (int)fi1, j1 = (int)fj1, i2 = (int)fi2, j2 = (int)fj2;
-            float sx0[NW], sy0[NW], sx1[NW], sy1[NW];
-            wts<ORDER>(xh - fi1, sx0);
-            wts<ORDER>(y - fj1, sy0);
-            wts<ORDER>(x - fi2, sx1);
-            wts<ORDER>(yh - fj2, sy1);
-            float Epx, Epy, Bpz;
-            const bool inw = tiled && (unsigned)(i2 - cx0) < (unsigned)WIN && (unsigned)(j1 - cy0) < (unsigned)WIN;
-            if constexpr (DEP == 0) drift += inw ? 0 : 1;
+            int i1 = (int)fi1, j1 = (int)fj1, i2 = (int)fi2, j2 = (int)fj2;
             oi = i2; oj = j1;
-            if (inw) {
-                // (window rows are non-negative here: 24-bit multiplies)
-                if constexpr (G4) {
-                    Epx = gather_lds4<W, W2>(sEx, i1 + 1 - NGB - X0, j1 - NGB - Y0, sx0, sy0);
-                    Epy = gather_lds4<W, W2>(sEy, i2 - NGB - X0, j2 + 1 - NGB - Y0, sx1, sy1);
+            // window cell; on an edge tile whose E window is staged with the
+            // periodic images, a cell one period away is gathered at its image
+            // (E is periodic there, so the taps are the same values)
+            bool inw = tiled && (unsigned)(i2 - cx0) < (unsigned)WIN && (unsigned)(j1 - cy0) < (unsigned)WIN;
+            if (!inw && (gimg_x || gimg_y)) {
+                const int gi = i2 - cx0, gj = j1 - cy0;
+                const int shx = gimg_x ? (gi < 0 ? a.ngx : (gi >= WIN ? -a.ngx : 0)) : 0;
+                const int shy = gimg_y ? (gj < 0 ? a.ngy : (gj >= WIN ? -a.ngy : 0)) : 0;
+                inw = (unsigned)(gi + shx) < (unsigned)WIN && (unsigned)(gj + shy) < (unsigned)WIN;
+                if (inw) { i1 += shx; i2 += shx; j1 += shy; j2 += shy; }
+            }
+            if constexpr (DEP == 0) drift += inw ? 0 : 1;
+            float sx0[NW], sx1[NW];
+            wts<ORDER>(xh - fi1, sx0);
+            wts<ORDER>(x - fi2, sx1);
+            float Epx, Epy, Bpz;
+            if constexpr (ORDER == 3) {
+                // cubic gather as column polynomials in the y fraction
+                const float hy0 = y - fj1, hy1 = yh - fj2;
+                if (inw) {
+                    // (window rows are non-negative here: 24-bit multiplies)
+                    if constexpr (CW) {
+                        Epx = gather_cw(sCx + (i1 + 1 - cx0) + rowW<W>(j1 - cy0), hy0, sx0);
+                        Epy = gather_cw(sCy + (i2 - cx0) + rowW<W>(j2 + 1 - cy0), hy1, sx1);
+                    } else {
+                        Epx = gather_c3(sEx + (i1 + 1 - NGB - X0) + rowW<W>(j1 - NGB - Y0), W, hy0, sx0);
+                        Epy = gather_c3(sEy + (i2 - NGB - X0) + rowW<W>(j2 + 1 - NGB - Y0), W, hy1, sx1);
+                    }
+                    if constexpr (UNIB) Bpz = a.bz0;
+                    else Bpz = gather_c3(sBz + (i1 + 1 - NGB - X0) + rowW<W>(j2 + 1 - NGB - Y0), W, hy1, sx0);
                 } else {
+                    Epx = gather_c3(a.Ex + (i1 + 1 - NGB) + (int64_t)(j1 - NGB) * pex, pex, hy0, sx0);
+                    Epy = gather_c3(a.Ey + (i2 - NGB) + (int64_t)(j2 + 1 - NGB) * pey, pey, hy1, sx1);
+                    if constexpr (UNIB) Bpz = a.bz0;
+                    else Bpz = gather_c3(a.Bz + (i1 + 1 - NGB) + (int64_t)(j2 + 1 - NGB) * pex, pex, hy1, sx0);
+                }
+            } else {
+                float sy0[NW], sy1[NW];
+                wts<ORDER>(y - fj1, sy0);
+                wts<ORDER>(yh - fj2, sy1);
+                if (inw) {
                     Epx = gather_lds<NW, W>(sEx + (i1 + 1 - NGB - X0) + rowW<W>(j1 - NGB - Y0), sx0, sy0);
                     Epy = gather_lds<NW, W>(sEy + (i2 - NGB - X0) + rowW<W>(j2 + 1 - NGB - Y0), sx1, sy1);
+                    if constexpr (UNIB) Bpz = a.bz0;
+                    else Bpz = gather_lds<NW, W>(sBz + (i1 + 1 - NGB - X0) + rowW<W>(j2 + 1 - NGB - Y0), sx0, sy1);
+                } else {
+                    Epx = gather<NW>(a.Ex + (i1 + 1 - NGB) + (int64_t)(j1 - NGB) * pex, pex, sx0, sy0);
+                    Epy = gather<NW>(a.Ey + (i2 - NGB) + (int64_t)(j2 + 1 - NGB) * pey, pey, sx1, sy1);
+                    if constexpr (UNIB) Bpz = a.bz0;
+                    else Bpz = gather<NW>(a.Bz + (i1 + 1 - NGB) + (int64_t)(j2 + 1 - NGB) * pex, pex, sx0, sy1);
                 }
-                if constexpr (UNIB) Bpz = a.bz0;
-                else Bpz = gather_lds<NW, W>(sBz + (i1 + 1 - NGB - X0) + rowW<W>(j2 + 1 - NGB - Y0), sx0, sy1);
-            } else {
-                Epx = gather<NW>(a.Ex + (i1 + 1 - NGB) + (int64_t)(j1 - NGB) * pex, pex, sx0, sy0);
-                Epy = gather<NW>(a.Ey + (i2 - NGB) + (int64_t)(j2 + 1 - NGB) * pey, pey, sx1, sy1);
-                if constexpr (UNIB) Bpz = a.bz0;
-                else Bpz = gather<NW>(a.Bz + (i1 + 1 - NGB) + (int64_t)(j2 + 1 - NGB) * pex, pex, sx0, sy1);
             }
             // Boris (Particle.mm:156-209) for the gathered components
             // Ex, Ey, Bz; Ez = Bx = By = 0 exactly as in the reference.
@@ -445,14 +517,6 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
             }
             const float fi2 = floorf(x), fj1 = floorf(y);
             const int i2 = (int)fi2, j1 = (int)fj1;
-            if constexpr (PUSH)
-                if (count_moved) moved += (i2 != oi || j1 != oj) ? 1 : 0;
-            float sx[NW], sy[NW];
-            wts<ORDER>(x - fi2, sx);
-            wts<ORDER>(y - fj1, sy);
-            unsigned qx[NW], qy[NW];
-#pragma unroll
-            for (int k = 0; k < NW; k++) { qx[k] = q23(sx[k]); qy[k] = q23(sy[k]); }
             // window cell (the stencil's first node is window node (wi, wj));
             // on an edge tile, a cell outside the window may have its image
             // one period away inside it (the rare branch)
@@ -463,7 +527,16 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
                 if (img_y) wj += wj < 0 ? a.ngy : (wj >= WIN ? -a.ngy : 0);
                 inw = (unsigned)wi < (unsigned)WIN && (unsigned)wj < (unsigned)WIN;
             }
-            if constexpr (PUSH) drift += inw ? 0 : 1;
+            if constexpr (PUSH) {
+                if (count_moved) moved += (i2 != oi || j1 != oj) ? 1 : 0;
+                drift += inw ? 0 : 1;
+            }
+            float sx[NW], sy[NW];
+            wts<ORDER>(x - fi2, sx);
+            wts<ORDER>(y - fj1, sy);
+            unsigned qx[NW], qy[NW];
+#pragma unroll
+            for (int k = 0; k < NW; k++) { qx[k] = q23(sx[k]); qy[k] = q23(sy[k]); }
             if (inw) {
                 unsigned long long *r = sRho + wi + rowW<W>(wj);
 #pragma unroll
@@ -1310,6 +1383,10 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
     // one period of its interior node)
     a.imgx = dep <= 1 && c->cfg.pbc[0] == GP_PBC_PERIODIC && c->cfg.fbc[0] == GP_FBC_PERIODIC && c->ngx >= 2 * GP_TILE;
     a.imgy = dep <= 1 && c->cfg.pbc[2] == GP_PBC_PERIODIC && c->cfg.fbc[2] == GP_FBC_PERIODIC && c->ngy >= 2 * GP_TILE;
+    a.gimgx = push && c->e_periodic && c->uniform_b && c->cfg.pbc[0] == GP_PBC_PERIODIC && c->cfg.fbc[0] == GP_FBC_PERIODIC &&
+              c->ngx >= 2 * GP_TILE;
+    a.gimgy = push && c->e_periodic && c->uniform_b && c->cfg.pbc[2] == GP_PBC_PERIODIC && c->cfg.fbc[2] == GP_FBC_PERIODIC &&
+              c->ngy >= 2 * GP_TILE;
     a.rho = (unsigned long long *)S.rho_acc;
     a.jx = (unsigned long long *)S.jx_acc;
     a.jy = (unsigned long long *)S.jy_acc;
@@ -1334,6 +1411,8 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
     const int grid = (a.use_tiles ? c->ntiles : 0) + extra;
     if (grid == 0) return GP_OK;
     if (grid > S.ke_nblk) return gp_fail(GP_ERR_STATE, "kinetic-energy partial buffer too small");
+    const bool tk = push && c->timing;
+    if (tk) { int rc = gp_time_mark(c, GP_TIME_PUSHK, -1); if (rc) return rc; }
     int rc;
     switch (c->order) {
     case 1: rc = launch_push_o<1>(grid, c->stream, a, c->uniform_b, dep, push); break;
@@ -1341,6 +1420,7 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
     default: rc = launch_push_o<5>(grid, c->stream, a, c->uniform_b, dep, push); break;
     }
     if (rc) return rc;
+    if (tk) { rc = gp_time_mark(c, GP_TIME_PUSHK, -1); if (rc) return rc; }
     if (a.use_tiles && dep >= 1) {
         const int nc = dep >= 2 ? 3 : 1;
         const dim3 g(gp_cdiv(c->nx + 2, GP_BLOCK), c->ny + 2, nc);

@@ -290,7 +290,8 @@ int  gp_field_energy(gp_ctx *ctx, double dt, double *we_ref, double *we_cgs, dou
  * which = 0 every push (+deposit), 1 every sort, 2 field solves, 3 every
  * reduce, and for species s: 4 + 3 s its push, 5 + 3 s its reduce, 6 + 3 s
  * its sort (the phases main.mm:176-228 times per cycle as update_<s>,
- * reduce_<s> and solvePoisson). */
+ * reduce_<s> and solvePoisson); 28 the push+deposit kernel alone (every
+ * species, without the window reduction and energy sum that follow it). */
 int  gp_timing(gp_ctx *ctx, int enable);
 int  gp_kernel_time(gp_ctx *ctx, int which, double *total_ms, int64_t *launches);
 

@@ -221,10 +221,48 @@ int gpo_push_ref(const gpo_grid *g, const float *Ex, const float *Ey, const floa
     return 0;
 }
 
+/* factorised tap gather F = sum_i sx[i] * (sum_j sy[j] * b[i + j*pitch]),
+ * each sum an fmaf chain started by a plain product */
+static inline float gather_taps(const float *b, int pitch, int nw, const float *sx, const float *sy)
+{
+    float acc = 0.0f;
+    for (int i = 0; i < nw; i++) {
+        float r = sy[0] * b[i];
+        for (int j = 1; j < nw; j++) r = fmaf(sy[j], b[i + j * pitch], r);
+        acc = (i == 0) ? sx[0] * r : fmaf(sx[i], r, acc);
+    }
+    return acc;
+}
+
+/* Order 3: a stencil column's taps f0..f3 weighted by w0..w3(h) is the cubic
+ * d0 + h (d1 + h (d2 + h d3)), with d_k the coefficients of h^k in the
+ * weights (src/Particle.mm:97-131):
+ *   d0 = (f0 + 4 f1 + f2)/6, d1 = (f2 - f0)/2, d2 = (f0 + f2)/2 - f1,
+ *   d3 = (f3 - f0 + 3 (f1 - f2))/6,
+ * each in the kernel's FP32 operation order (coef3, horner3, gather_c3 in
+ * gample_amd/csrc/gp_particles.hip). */
+static inline float gather_c3(const float *b, int pitch, float h, const float *sx)
+{
+    float acc = 0.0f;
+    for (int i = 0; i < 4; i++) {
+        const float f0 = b[i], f1 = b[i + pitch], f2 = b[i + 2 * pitch], f3 = b[i + 3 * pitch];
+        const float a02 = f0 + f2;
+        const float d0 = fmaf(f1, 2.0f / 3.0f, a02 * (1.0f / 6.0f));
+        const float d1 = (f2 - f0) * 0.5f;
+        const float d2 = fmaf(a02, 0.5f, -f1);
+        const float d3 = fmaf(f1 - f2, 3.0f, f3 - f0) * (1.0f / 6.0f);
+        const float r = fmaf(fmaf(fmaf(d3, h, d2), h, d1), h, d0);
+        acc = (i == 0) ? sx[0] * r : fmaf(sx[i], r, acc);
+    }
+    return acc;
+}
+
 /* Kernel op order (the HIP push restates this line by line):
  *   - Horner weights (wfast),
- *   - factorised gather F_p = sum_i sx[i] * (sum_j sy[j] * F[i][j]), each sum
- *     an fmaf chain started by a plain product,
+ *   - gather at orders 1 and 5 factorised, F_p = sum_i sx[i] *
+ *     (sum_j sy[j] * F[i][j]), each sum an fmaf chain started by a plain
+ *     product (gather_taps); at order 3 the column polynomials in the y
+ *     fraction (gather_c3),
  *   - Boris specialised to the only components the reference gathers
  *     (Ex, Ey, Bz; Ez = Bx = By = 0 exactly, src/Particle.mm:134-154), with
  *     the cross products written as fmaf,
@@ -252,39 +290,18 @@ int gpo_push_fast(const gpo_grid *g, const float *Ex, const float *Ey, const flo
         wfast(order, py - fj1, sy0);
         wfast(order, px - fi2, sx1);
         wfast(order, yh - fj2, sy1);
-        /* Ex at (i1+1-ngb+i, j1-ngb+j) */
+        /* Ex at (i1+1-ngb+i, j1-ngb+j), Ey at (i2-ngb+i, j2+1-ngb+j),
+         * Bz at (i1+1-ngb+i, j2+1-ngb+j); order 3 in the column-polynomial
+         * form (gather_c3), orders 1 and 5 as weighted tap sums */
         float Epx, Epy, Bpz;
-        {
-            const float *b = Ex + (i1 + 1 - ngb) + (int64_t)(j1 - ngb) * pex;
-            float acc = 0.0f;
-            for (int i = 0; i < nw; i++) {
-                float r = sy0[0] * b[i];
-                for (int j = 1; j < nw; j++) r = fmaf(sy0[j], b[i + j * pex], r);
-                acc = (i == 0) ? sx0[0] * r : fmaf(sx0[i], r, acc);
-            }
-            Epx = acc;
-        }
-        {
-            const float *b = Ey + (i2 - ngb) + (int64_t)(j2 + 1 - ngb) * pey;
-            float acc = 0.0f;
-            for (int i = 0; i < nw; i++) {
-                float r = sy1[0] * b[i];
-                for (int j = 1; j < nw; j++) r = fmaf(sy1[j], b[i + j * pey], r);
-                acc = (i == 0) ? sx1[0] * r : fmaf(sx1[i], r, acc);
-            }
-            Epy = acc;
-        }
-        if (uniform_bz) {
-            Bpz = bz0;
+        if (order == 3) {
+            Epx = gather_c3(Ex + (i1 + 1 - ngb) + (int64_t)(j1 - ngb) * pex, pex, py - fj1, sx0);
+            Epy = gather_c3(Ey + (i2 - ngb) + (int64_t)(j2 + 1 - ngb) * pey, pey, yh - fj2, sx1);
+            Bpz = uniform_bz ? bz0 : gather_c3(Bz + (i1 + 1 - ngb) + (int64_t)(j2 + 1 - ngb) * pex, pex, yh - fj2, sx0);
         } else {
-            const float *b = Bz + (i1 + 1 - ngb) + (int64_t)(j2 + 1 - ngb) * pex;
-            float acc = 0.0f;
-            for (int i = 0; i < nw; i++) {
-                float r = sy1[0] * b[i];
-                for (int j = 1; j < nw; j++) r = fmaf(sy1[j], b[i + j * pex], r);
-                acc = (i == 0) ? sx0[0] * r : fmaf(sx0[i], r, acc);
-            }
-            Bpz = acc;
+            Epx = gather_taps(Ex + (i1 + 1 - ngb) + (int64_t)(j1 - ngb) * pex, pex, nw, sx0, sy0);
+            Epy = gather_taps(Ey + (i2 - ngb) + (int64_t)(j2 + 1 - ngb) * pey, pey, nw, sx1, sy1);
+            Bpz = uniform_bz ? bz0 : gather_taps(Bz + (i1 + 1 - ngb) + (int64_t)(j2 + 1 - ngb) * pex, pex, nw, sx0, sy1);
         }
         const float ex0 = constE * Epx, ey0 = constE * Epy;
         float etx = ex0, ety = ey0;
