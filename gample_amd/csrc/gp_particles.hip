@@ -198,17 +198,14 @@ __device__ __forceinline__ unsigned q23(float s)
 }
 
 // periodic wrap x = fmod(x + (ng-ngb), ng) + ngb (Particle.mm:216-221); the
-// two conditional subtractions are the exact remainder for 0 <= a < 3 ng.
+// conditional subtraction is the exact remainder for 0 <= a < 2 ng.
 __device__ __forceinline__ float wrap(float x, int ng, int ngb)
 {
     const float L = (float)ng;
     float a = x + (float)(ng - ngb);
-    if (a >= 0.0f && a < 3.0f * L) {
-        if (a >= L) a -= L;
-        if (a >= L) a -= L;
-    } else {
-        a = fmodf(a, L);
-    }
+    // (for 0 <= a < 2L the one subtraction is exact, so it is fmod's result)
+    if (a >= 0.0f && a < 2.0f * L) a = a >= L ? a - L : a;
+    else a = fmodf(a, L);
     return a + (float)ngb;
 }
 

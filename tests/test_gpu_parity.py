@@ -738,6 +738,55 @@ def test_drifted_particles_reach_the_window_edges(gp, orc, order):
     ctx.close()
 
 
+@pytest.mark.parametrize("order", [1, 3, 5])
+def test_periodic_image_gather(gp, orc, order):
+    """Particles that crossed a periodic boundary after the last sort stay in
+    the segment of their edge tile, one period away from its window.  With E
+    from the field solve (exactly periodic) the push gathers them at their
+    image inside the window; with E set by the caller it reads global memory.
+    Both are bit-exact against the oracle's push at the particle's own
+    position, and so is the folded charge."""
+    say(f"[image gather] order={order}")
+    ngx, ngy, dx, dt = 64, 64, 0.01, 2e-12
+    ngb = order // 2
+    ctx = mkctx(gp, ngx, ngy, order, dx=dx)
+    sid = ctx.add_species(-EC, ME, 1.0, 300000, vth_hint=1e8)
+    rng = np.random.default_rng(70 + order)
+    p = rand_parts(200000, ngx, ngy, ngb, seed=80 + order)
+    cX = float(np.float32(dt / float(np.float32(dx))))
+    p[2] = (rng.uniform(-3.0, 3.0, p[2].size) / cX).astype(np.float32)
+    p[3] = (rng.uniform(-3.0, 3.0, p[3].size) / cX).astype(np.float32)
+    ctx.upload(sid, *p)
+    ctx.sort(sid)
+    s_sorted = ctx.download(sid)
+    ctx.reset_grids(); ctx.deposit(sid)
+    st, _ = ctx.solve_fields(dt, tol=1e-8)
+    assert st.relres <= 1e-8
+    for _ in range(2):                                      # drift up to 6 cells, no re-sort
+        ctx.push(sid, dt, deposit=0)
+    s0 = ctx.download(sid)
+    crossed = (np.abs(s0[0] - s_sorted[0]) > ngx / 2) | (np.abs(s0[1] - s_sorted[1]) > ngy / 2)
+    assert crossed.sum() > 2000, crossed.sum()
+    Ex, Ey = ctx.field(gp.F_EX), ctx.field(gp.F_EY)
+    Bz = np.zeros(ctx.field_shape(gp.F_BZ)[0], np.float32)
+    g = orc.grid(ngx, ngy, order)
+    cst = consts(-EC, ME, dt, dx)
+    state = s0
+    for user_set in (False, True):
+        if user_set:                                        # the same values, set by the caller
+            ctx.set_field(gp.F_EX, Ex); ctx.set_field(gp.F_EY, Ey)
+        ctx.reset_grids()
+        ctx.push(sid, dt, deposit=1)
+        exp = orc.push_fast(g, Ex, Ey, Bz, cst, state, uniform_bz=True)
+        got = ctx.download(sid)
+        for a, b in zip(got, exp):
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+        rho_exp, _, _ = orc.deposit_fx(g, exp)
+        np.testing.assert_array_equal(orc.fold_acc(ctx.acc(sid), ngb, ngx, ngy), orc.fold_acc(rho_exp, ngb, ngx, ngy))
+        state = exp
+    ctx.close()
+
+
 def test_fused_step_equals_reference_order(gp):
     """gp_step (solve -> fused push+deposit) against the reference's call
     order per cycle (reset -> deposit each species -> solve -> push each
