@@ -18,7 +18,10 @@ The roofline object is for the fused push+deposit kernel: algorithmic bytes
 vz does not change when Ez = Bx = By = 0, so the kernel does not store it)
 over the kernel's mean duration from HIP events recorded on the context
 stream around each of its launches during the timed steps (species_push_ms:
-per species, the push with the window reduction and energy sum after it).  sort_cycle times the steps from one drift-
+per species, the push with the window reduction and energy sum after it).
+stream_ceiling is the same 36 B per particle moved by a kernel that does
+nothing else (gp_stream_probe): the HBM rate this access pattern reaches on
+the box, the practical ceiling of the push's fraction.  sort_cycle times the steps from one drift-
 triggered sort of the electrons to the next after the timed steps, so its
 rate carries the sort's amortised cost whatever the timed window held.
 cpu_baseline times the C/OpenMP oracle's push+deposit on a bounded sample of
@@ -102,6 +105,7 @@ def gpu_run(args):
     sort_ms, sort_n = ctx.kernel_time(1)
     solve_ms, solve_n = ctx.kernel_time(2)
     ctx.timing(False)
+    probe_ms = ctx.stream_probe(se, 10)
     cycle_rep = sort_cycle(ctx, se, dt, tol, sort_every, cyc, args.cycle_max_steps)
     cyc += cycle_rep.pop("steps_run")
     _, dg, _ = ctx.solve_fields(dt, tol=tol, diag=True)
@@ -110,6 +114,7 @@ def gpu_run(args):
     ctx.close()
     return dict(n_total=nn, seconds=t1 - t0, push_ms=push_ms, push_n=push_n, pushk_ms=pushk_ms, pushk_n=pushk_n,
                 species_push_ms={"electrons": e_ms / max(1, e_n), "ions": i_ms / max(1, i_n)},
+                probe_ms=probe_ms,
                 sort_ms=sort_ms, sort_n=sort_n,
                 solve_ms=solve_ms, solve_n=solve_n, iters=iters, P=P, n_species=n, nn=nnrep, sorts=sorts,
                 cycle=cycle_rep)
@@ -353,7 +358,11 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_ms": push_avg_ms,
             "launches": r["pushk_n"],
             "timing": "HIP events around each k_push launch on the context stream",
-            "species_push_ms": r["species_push_ms"]}
+            "species_push_ms": r["species_push_ms"],
+            "stream_ceiling": {"kernel": "k_stream_probe: read x,y,vx,vy,vz, write 4 arrays (36 B per particle)",
+                               "ms": r["probe_ms"], "achieved": bytes_per_launch / (r["probe_ms"] * 1e-3) / 1e9,
+                               "frac": bytes_per_launch / (r["probe_ms"] * 1e-3) / HBM_PEAK,
+                               "push_over_stream": push_avg_ms / r["probe_ms"]}}
     line = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",

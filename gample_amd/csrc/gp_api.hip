@@ -476,6 +476,14 @@ int gp_sort(gp_ctx *c, int sid)
     return GP_OK;
 }
 
+int gp_stream_probe(gp_ctx *c, int sid, int reps, double *ms_per_launch)
+{
+    GP_CHECK_CTX(c);
+    GP_CHECK_SID(c, sid);
+    if (reps < 1 || !ms_per_launch) return gp_fail(GP_ERR_ARG, "reps must be >= 1 and the output non-null");
+    return gp_launch_stream_probe(c, sid, reps, ms_per_launch);
+}
+
 int gp_moments(gp_ctx *c, int sid, double *host10)
 {
     GP_CHECK_CTX(c);

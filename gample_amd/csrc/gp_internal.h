@@ -211,6 +211,7 @@ struct gp_ctx {
 // ---- particle side (gp_particles.hip) ----
 int gp_launch_push(gp_ctx *c, int sid, double dt, int deposit);
 int gp_launch_deposit(gp_ctx *c, int sid, int64_t first, int64_t count);
+int gp_launch_stream_probe(gp_ctx *c, int sid, int reps, double *ms);
 int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke, gp_arena *ar = nullptr);
 int gp_launch_reduce_from(gp_ctx *c, int sid, const unsigned long long *h, int64_t counts[4], double *ke,
                           gp_arena *ar = nullptr);

@@ -613,6 +613,22 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
     }
 }
 
+// HBM streaming rate of the push's access pattern, for its roofline: read
+// x, y, vx, vy, vz and write four of them (36 B per particle), one 512-thread
+// workgroup per contiguous segment like a tile, into the species' scratch
+// arrays (the live particles are not modified).
+__global__ __launch_bounds__(512) void k_stream_probe(gp_soa in, gp_soa out, int64_t n, int64_t per)
+{
+    const int64_t p0 = (int64_t)blockIdx.x * per, p1 = p0 + per < n ? p0 + per : n;
+    for (int64_t i = p0 + threadIdx.x; i < p1; i += 512) {
+        const float x = in.x[i], y = in.y[i], vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
+        out.x[i] = x + vz;
+        out.y[i] = y + vx;
+        out.vx[i] = vx + vy;
+        out.vy[i] = vy + x;
+    }
+}
+
 // floor(a / 16) for any sign
 __device__ __forceinline__ int fdiv16(int a) { return a >= 0 ? a / GP_TILE : -((-a + GP_TILE - 1) / GP_TILE); }
 
@@ -1513,6 +1529,36 @@ int gp_launch_deposit(gp_ctx *c, int sid, int64_t first, int64_t count)
 {
     const int dep = c->cfg.cur_dens ? 2 : 1;
     return run_particle_kernel(c, sid, 0.0, dep, false, first, count);
+}
+
+int gp_launch_stream_probe(gp_ctx *c, int sid, int reps, double *ms)
+{
+    gp_species &S = c->sp[sid];
+    const int64_t n = S.n;
+    *ms = 0.0;
+    if (n == 0) return GP_OK;
+    const int64_t per = 32768;
+    const int64_t nb = gp_cdiv(n, per);
+    if (nb > INT32_MAX) return gp_fail(GP_ERR_ARG, "too many particles for the probe grid");
+    hipEvent_t e0, e1;
+    GP_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return gp_fail(GP_ERR_HIP, "hipEventCreate"); }
+    hipError_t err = hipSuccess;
+    k_stream_probe<<<(unsigned)nb, 512, 0, c->stream>>>(S.a, S.b, n, per);   // warm-up
+    err = hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps && err == hipSuccess; r++) {
+        k_stream_probe<<<(unsigned)nb, 512, 0, c->stream>>>(S.a, S.b, n, per);
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipEventRecord(e1, c->stream);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err != hipSuccess) return gp_fail(GP_ERR_HIP, std::string("stream probe: ") + hipGetErrorString(err));
+    *ms = (double)t / reps;
+    return GP_OK;
 }
 
 // key bits of the absorbed-index sort of a species of n particles

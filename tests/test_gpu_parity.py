@@ -787,6 +787,21 @@ def test_periodic_image_gather(gp, orc, order):
     ctx.close()
 
 
+def test_stream_probe_leaves_particles(gp):
+    """gp_stream_probe streams the species into its scratch store: the live
+    particles are unchanged and the rate is a positive time per launch."""
+    say("[stream probe]")
+    ctx = mkctx(gp, 64, 64, 3)
+    sid = ctx.add_species(-EC, ME, 1.0, 100000)
+    p = rand_parts(80000, 64, 64, 1, seed=5)
+    ctx.upload(sid, *p)
+    ms = ctx.stream_probe(sid, 3)
+    assert ms > 0.0
+    for a, b in zip(ctx.download(sid), p):
+        np.testing.assert_array_equal(a, b)
+    ctx.close()
+
+
 def test_fused_step_equals_reference_order(gp):
     """gp_step (solve -> fused push+deposit) against the reference's call
     order per cycle (reset -> deposit each species -> solve -> push each
