@@ -705,6 +705,14 @@ int gp_nn_add_pair(gp_ctx *c)
     if (!c->have_solution) return gp_fail(GP_ERR_STATE, "no Poisson solution yet");
     GP_TRY({ return gp_nn_add_pair_dev(c, c->ps.b64, c->ps.x64); })
 }
+int gp_nn_add_increment(gp_ctx *c, int *added)
+{
+    GP_CHECK_CTX(c);
+    if (!c->nn) return gp_fail(GP_ERR_STATE, "gp_nn_init first");
+    if (!c->have_solution) return gp_fail(GP_ERR_STATE, "no Poisson solution yet");
+    int tmp = 0;
+    GP_TRY({ return gp_nn_add_increment_dev(c, c->ps.b64, c->ps.x64, (int64_t)c->ps.n, added ? added : &tmp); })
+}
 int gp_nn_config(gp_ctx *c, int loss_kind, int smooth_sweeps, int fcg_cap)
 {
     GP_CHECK_CTX(c);

@@ -266,6 +266,7 @@ int gp_nn_train_impl(gp_ctx *c, const double *rho, const double *phi, int nsamp,
 int gp_nn_apply(gp_ctx *c, gp_solver *s, const float *in_dev, float *out_dev);
 int gp_nn_infer_impl(gp_ctx *c, const double *rhs, double *out);
 int gp_nn_add_pair_dev(gp_ctx *c, const double *b, const double *x);
+int gp_nn_add_increment_dev(gp_ctx *c, const double *b, const double *x, int64_t m, int *added);
 int gp_nn_grad_impl(gp_ctx *c, const float *in, const float *tgt, int B, int kind, float *loss, float *grad);
 int gp_nn_forward_impl(gp_ctx *c, const float *in, int B, float *out);
 int gp_nn_params_impl(gp_ctx *c, float *get, const float *set, int64_t *count);

@@ -856,6 +856,10 @@ struct gp_nn {
     float *bw = nullptr;           // per-image loss weights of the current batch
     std::vector<float> hbw;
     float *tmpf = nullptr;   // mx*my FP32 staging
+    // gp_nn_add_increment: the previous solve's b and x, and the differences
+    double *prev_b = nullptr, *prev_x = nullptr, *inc_b = nullptr, *inc_x = nullptr;
+    int64_t prev_n = 0;
+    int have_prev = 0;
 };
 
 namespace {
@@ -1256,7 +1260,8 @@ void gp_nn_destroy(gp_ctx *c)
     gp_nn *n = c->nn;
     if (!n) return;
     nn_free_batch(n);
-    void *ps[] = {n->P, n->G, n->M, n->V, n->WB, n->WT, n->stats, n->lpart, n->tmpf, n->bw};
+    void *ps[] = {n->P, n->G, n->M, n->V, n->WB, n->WT, n->stats, n->lpart, n->tmpf, n->bw,
+                  n->prev_b, n->prev_x, n->inc_b, n->inc_x};
     for (void *p : ps) if (p) (void)hipFree(p);
     for (float *p : n->ds_in) (void)hipFree(p);
     for (float *p : n->ds_tgt) (void)hipFree(p);
@@ -1290,6 +1295,50 @@ int gp_nn_add_pair_dev(gp_ctx *c, const double *b, const double *x)
     n->ds_in.push_back(in);
     n->ds_tgt.push_back(tg);
     n->ds_E.push_back(E > 0.0 ? E : 1.0);
+    return GP_OK;
+}
+
+namespace {
+__global__ void k_sub_keep(const double *a, double *prev, double *d, int64_t n, int diff)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = a[i];
+        if (diff) d[i] = v - prev[i];
+        prev[i] = v;
+    }
+}
+}  // namespace
+
+// the time-loop increment (b_n - b_{n-1}, x_n - x_{n-1}) of the last two
+// solves on the device (b, x: the current ones, m unknowns); the first call
+// only records b_n, x_n.  *added = 1 when a pair was appended.
+int gp_nn_add_increment_dev(gp_ctx *c, const double *b, const double *x, int64_t m, int *added)
+{
+    gp_nn *n = c->nn;
+    hipStream_t st = c->stream;
+    *added = 0;
+    if (n->prev_n != m) {
+        double **bs[] = {&n->prev_b, &n->prev_x, &n->inc_b, &n->inc_x};
+        for (double **p : bs) {
+            if (*p) GP_HIP(hipFree(*p));
+            *p = nullptr;
+        }
+        n->prev_n = 0;
+        n->have_prev = 0;
+        for (double **p : bs) GP_HIP(hipMalloc((void **)p, m * sizeof(double)));
+        n->prev_n = m;
+    }
+    const int g = (int)std::min<int64_t>(gp_cdiv(m, 256), 4096);
+    k_sub_keep<<<g, 256, 0, st>>>(b, n->prev_b, n->inc_b, m, n->have_prev);
+    k_sub_keep<<<g, 256, 0, st>>>(x, n->prev_x, n->inc_x, m, n->have_prev);
+    GP_HIP(hipGetLastError());
+    if (!n->have_prev) {
+        n->have_prev = 1;
+        return GP_OK;
+    }
+    int rc = gp_nn_add_pair_dev(c, n->inc_b, n->inc_x);
+    if (rc) return rc;
+    *added = 1;
     return GP_OK;
 }
 

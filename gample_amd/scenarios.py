@@ -17,6 +17,10 @@ solves' iterations and residuals are recorded per cycle.
 Both return per-cycle histories (counts, absorbed and injected particles,
 energies, solver iterations, wall time) so that a caller can check the
 particle balance and report the run.
+
+learned: the learned Poisson preconditioner trained at the workload's scale
+on thousands of the run's own time-loop increments, kept on the device, and
+compared with MG-CG on held-out steps of the same run.
 """
 from __future__ import annotations
 
@@ -25,7 +29,7 @@ import time
 
 import numpy as np
 
-from . import Context, Gen, make_config
+from . import F_RHS, F_SOL, Context, Gen, make_config
 
 ME = 9.10938188e-28      # g
 EC = 4.803204197e-10     # statC
@@ -58,6 +62,40 @@ def _load(ctx, se, si, P, ngx, ngy, n, seed=0):
     ctx.sort(si)
 
 
+class WallSource:
+    """Half-Maxwellian flux through the x = 0 wall, n0 vbar / 4 per unit wall
+    length and time (vbar = sqrt(8 kT / (pi m))), in whole particles per cycle
+    with the fraction carried over; particle k of a call draws from PCG stream
+    (stream + k), and each call advances the stream by its count, so every
+    injected particle's stream is distinct."""
+
+    def __init__(self, ctx, se, si, P, ngy, seed):
+        self.ctx, self.dt, self.seed = ctx, P["dt"], seed
+        ly = ngy * float(np.float32(P["dx"]))
+        self.flux = {}
+        for sid, (m, kt) in ((se, (ME, P["te"])), (si, (MXE, P["ti"]))):
+            vbar = math.sqrt(8 * KB * kt / (math.pi * m))
+            self.flux[sid] = P["n0"] * vbar / 4 * ly * P["dt"] / P["w"]
+        self.vth1 = {se: math.sqrt(KB * P["te"] / ME), si: math.sqrt(KB * P["ti"] / MXE)}   # per-component width
+        self.carry = {se: 0.0, si: 0.0}
+        self.next_stream = {se: 0, si: 0}
+
+    def inject(self):
+        """one cycle's injection of every species (charge deposited at once);
+        the particles injected per species"""
+        inj = {}
+        for sid in self.flux:
+            want = self.flux[sid] + self.carry[sid]
+            k = int(want)
+            self.carry[sid] = want - k
+            if k > 0:
+                self.ctx.inject_wall(sid, k, float(np.float32(self.vth1[sid])), (0.0, 0.0, 0.0), self.dt,
+                                     self.seed + 1 + sid, self.next_stream[sid], deposit=1)
+                self.next_stream[sid] += k
+            inj[sid] = k
+        return inj
+
+
 def bounded_plasma(ngx=512, ngy=512, ppc=256, steps=500, order=3, tol=1e-8, sort_drift=3e-3, seed=0,
                    capacity_factor=1.25, log=None):
     P = plasma(ppc=ppc)
@@ -73,17 +111,9 @@ def bounded_plasma(ngx=512, ngy=512, ppc=256, steps=500, order=3, tol=1e-8, sort
     ctx.reset_grids()
     ctx.deposit(se)
     ctx.deposit(si)
-    dt, ly = P["dt"], ngy * float(np.float32(P["dx"]))
-    # wall flux per cycle in particles: n0 vbar / 4 * Ly * dt / w
-    flux = {}
-    for sid, (m, kt) in ((se, (ME, P["te"])), (si, (MXE, P["ti"]))):
-        vbar = math.sqrt(8 * KB * kt / (math.pi * m))
-        flux[sid] = P["n0"] * vbar / 4 * ly * dt / P["w"]
-    vth1 = {se: math.sqrt(KB * P["te"] / ME), si: math.sqrt(KB * P["ti"] / MXE)}   # per-component width
-    carry = {se: 0.0, si: 0.0}
-    # particle k of a call draws from PCG stream (stream + k): advancing the
-    # stream by each call's count keeps every injected particle's stream distinct
-    next_stream = {se: 0, si: 0}
+    dt = P["dt"]
+    src = WallSource(ctx, se, si, P, ngy, seed)
+    flux = src.flux
     hist = {k: [] for k in ("cycle", "count_e", "count_i", "absorbed_e", "absorbed_i", "injected_e", "injected_i",
                             "kinetic", "field", "iterations", "relres", "ms")}
     ctx.sync()
@@ -91,16 +121,7 @@ def bounded_plasma(ngx=512, ngy=512, ppc=256, steps=500, order=3, tol=1e-8, sort
     for cyc in range(1, steps + 1):
         t0 = time.perf_counter()
         st, dg = ctx.step(dt, tol=tol, maxiter=400, sort_interval=0, cycle=cyc, diag=True)
-        inj = {}
-        for sid in (se, si):
-            want = flux[sid] + carry[sid]
-            k = int(want)
-            carry[sid] = want - k
-            if k > 0:
-                ctx.inject_wall(sid, k, float(np.float32(vth1[sid])), (0.0, 0.0, 0.0), dt, seed + 1 + sid,
-                                next_stream[sid], deposit=1)
-                next_stream[sid] += k
-            inj[sid] = k
+        inj = src.inject()
         ctx.sync()
         hist["cycle"].append(cyc)
         hist["count_e"].append(ctx.count(se))
@@ -166,6 +187,122 @@ def darwin(ngx=512, ngy=512, ppc=128, steps=500, order=3, tol=1e-8, sort_drift=3
     return out
 
 
+def learned(kind="periodic", ngx=1024, ppc=None, pairs=10000, eval_fields=8, warmup=20, train_seconds=120.0,
+            max_epochs=100, batch=4, lr=1e-3, tol=1e-8, order=3, seed=0, save=None, log=None):
+    """The learned Poisson preconditioner at the workload's scale: the network
+    is trained on `pairs` time-loop increments (b_n - b_{n-1}, x_n - x_{n-1})
+    that the run itself produces, kept on the device (gp_nn_add_increment),
+    then compared with MG-CG on held-out steps of the same run.
+
+    kind "periodic": the metric workload (ngx^2 periodic, uniform Bz, 128 ppc);
+    "bounded": the bounded plasma (walls in x with the wall source, 256 ppc).
+    Held-out step n is solved from x_{n-1} by MG-CG (mg_warm), by MG-CG after
+    the network's S-optimally scaled correction (nn_warm_mg), by the network
+    flexible CG (nn_warm_fcg), and from zero by MG-CG (mg_cold).  On the
+    held-out increments: the network's relative energy-norm error and the
+    residual it leaves, against the warm start's."""
+    periodic = kind == "periodic"
+    ppc = ppc or (128 if periodic else 256)
+    P = plasma(ppc=ppc)
+    n = ppc * ngx * ngx
+    if periodic:
+        cfg = make_config(ngx, ngx, P["dx"], P["dx"], order=order, b0=(0.0, 0.0, P["bz"]))
+    else:
+        cfg = make_config(ngx, ngx, P["dx"], P["dx"], order=order, pbc=("Delete", "Delete", "periodic", "periodic"),
+                          fbc=("Dirichlet", "Dirichlet", "periodic", "periodic"), b0=(0.0, 0.0, P["bz"]))
+    ctx = Context(cfg)
+    cap = n if periodic else int(n * 1.25)
+    se = ctx.add_species(-EC, ME, P["w"], cap, vth_hint=P["vth_e"] / math.sqrt(2))
+    si = ctx.add_species(EC, MXE, P["w"], cap, vth_hint=P["vth_i"] / math.sqrt(2))
+    _load(ctx, se, si, P, ngx, ngx, n, seed)
+    ctx.set_sort_drift(3e-3)
+    ctx.reset_grids(); ctx.deposit(se); ctx.deposit(si)
+    src = None if periodic else WallSource(ctx, se, si, P, ngx, seed)
+    dt = P["dt"]
+    cyc = 0
+
+    def step():
+        nonlocal cyc
+        cyc += 1
+        ctx.step(dt, tol=tol, maxiter=400, sort_interval=0, cycle=cyc)
+        if src is not None:
+            src.inject()
+
+    for _ in range(warmup):
+        step()
+    ctx.nn_init(32, seed=1)
+    ctx.nn_config(1, 2, 0)
+    t0 = time.perf_counter()
+    got = 0
+    while got < pairs:
+        step()
+        got += 1 if ctx.nn_add_increment() else 0
+        if log and got and got % 2000 == 0:
+            log(f"[learned] {got} increments ({time.perf_counter() - t0:.0f} s)")
+    my, mx = ctx.field_shape(F_SOL)[0]
+    bs, xs = [], []
+    for _ in range(eval_fields + 1):                 # held out: not in the training set
+        step()
+        bs.append(ctx.field(F_RHS).ravel().copy())
+        xs.append(ctx.field(F_SOL).ravel().copy())
+    t1 = time.perf_counter()
+    loss0 = ctx.nn_train(epochs=1, batch=batch, lr=0.0)
+    losses, epochs = [loss0], 0
+    while epochs < max_epochs and time.perf_counter() - t1 < train_seconds:
+        losses.append(ctx.nn_train(epochs=1, batch=batch, lr=lr))
+        epochs += 1
+        if log:
+            log(f"[learned] epoch {epochs}: loss {losses[-1]:.4f}")
+    t2 = time.perf_counter()
+    h = float(np.float32(P["dx"]))
+
+    def S(v):      # the solver's operator on the unknown grid (periodic y; periodic or Dirichlet x)
+        u = v.reshape(my, mx)
+        if periodic:
+            ex = np.roll(u, 1, 1) + np.roll(u, -1, 1)
+        else:
+            ex = np.zeros_like(u)
+            ex[:, 1:] += u[:, :-1]
+            ex[:, :-1] += u[:, 1:]
+        return ((4 * u - np.roll(u, 1, 0) - np.roll(u, -1, 0) - ex) / h ** 2).ravel()
+
+    rows = {k: [] for k in ("mg_cold", "mg_warm", "nn_warm_mg", "nn_warm_fcg")}
+    err, res_warm, res_nn = [], [], []
+    for k in range(1, eval_fields + 1):
+        b, xp = bs[k], xs[k - 1]
+        db, dxk = bs[k] - bs[k - 1], xs[k] - xs[k - 1]
+        for key, solver, x0 in (("mg_cold", 0, None), ("mg_warm", 0, xp), ("nn_warm_mg", 2, xp),
+                                ("nn_warm_fcg", 1, xp)):
+            _, st, _ = ctx.solve_rhs(b, x0=x0, tol=tol, maxiter=400, solver=solver)
+            rows[key].append((st.iterations, 1e3 * st.seconds, st.relres, st.fell_back))
+        z = ctx.nn_infer(db)
+        sz = S(z)
+        a = (z @ db) / (z @ sz)                      # S dx = db: the S-optimal multiple of z
+        e = dxk - a * z
+        err.append(float(np.sqrt((e @ S(e)) / (dxk @ S(dxk)))))
+        ar = (db @ sz) / (sz @ sz)
+        nb = np.linalg.norm(b)
+        res_warm.append(float(np.linalg.norm(db) / nb))
+        res_nn.append(float(np.linalg.norm(db - ar * sz) / nb))
+    out = dict(config=dict(kind=kind, ngx=ngx, ppc=ppc, particles=2 * n, order=order, tol=tol, warmup=warmup),
+               train_pairs=pairs, pairs="time-loop increments (b_n - b_{n-1}, x_n - x_{n-1}), device resident",
+               harvest_s=t1 - t0, train_s=t2 - t1, epochs=epochs, batch=batch, lr=lr,
+               loss_untrained=loss0, loss_trained=losses[-1], loss_curve=losses,
+               heldout_fields=eval_fields, heldout_energy_error=float(np.mean(err)),
+               heldout_relres_warm_start=float(np.mean(res_warm)),
+               heldout_relres_after_network=float(np.mean(res_nn)))
+    for key, v in rows.items():
+        a = np.array(v, dtype=float)
+        out[key] = {"iterations": float(a[:, 0].mean()), "ms": float(a[:, 1].mean()),
+                    "max_relres": float(a[:, 2].max()), "fell_back": int(a[:, 3].sum())}
+    out["nn_warm_mg_over_mg_warm_time"] = out["nn_warm_mg"]["ms"] / out["mg_warm"]["ms"]
+    out["nn_warm_fcg_over_mg_cold_time"] = out["nn_warm_fcg"]["ms"] / out["mg_cold"]["ms"]
+    if save:
+        ctx.nn_save(save)
+    ctx.close()
+    return out
+
+
 def summary(out):
     """the run's figures for a report: sizes, wall time, per-cycle time,
     solver iterations, and for the bounded plasma the particle balance"""
@@ -204,8 +341,14 @@ def summary(out):
 def main(argv=None):
     import argparse
     import json
-    ap = argparse.ArgumentParser(description="bounded-plasma or Darwin run on one GPU")
-    ap.add_argument("which", choices=["bounded", "darwin"])
+    ap = argparse.ArgumentParser(description="bounded-plasma, Darwin or learned-preconditioner run on one GPU")
+    ap.add_argument("which", choices=["bounded", "darwin", "learned"])
+    ap.add_argument("--kind", choices=["periodic", "bounded"], default="periodic",
+                    help="learned: the workload the pairs come from")
+    ap.add_argument("--pairs", type=int, default=10000, help="learned: training increments")
+    ap.add_argument("--eval-fields", type=int, default=8)
+    ap.add_argument("--train-seconds", type=float, default=120.0)
+    ap.add_argument("--save", default=None, help="learned: write the trained network here (gp_nn_save)")
     ap.add_argument("--ngx", type=int, default=512)
     ap.add_argument("--ppc", type=int, default=None)
     ap.add_argument("--steps", type=int, default=500)
@@ -213,6 +356,11 @@ def main(argv=None):
     ap.add_argument("--history", default=None, help="write the per-cycle history here as JSON")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
+    if a.which == "learned":
+        out = learned(kind=a.kind, ngx=a.ngx, ppc=a.ppc, pairs=a.pairs, eval_fields=a.eval_fields,
+                      train_seconds=a.train_seconds, order=a.order, save=a.save, log=log)
+        print(json.dumps(out), flush=True)
+        return
     if a.which == "bounded":
         out = bounded_plasma(ngx=a.ngx, ngy=a.ngx, ppc=a.ppc or 256, steps=a.steps, order=a.order, log=log)
     else:

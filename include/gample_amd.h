@@ -321,6 +321,11 @@ int  gp_nn_config(gp_ctx *ctx, int loss_kind, int smooth_sweeps, int fcg_cap);
 /* Append the Poisson solver's current (b, x) pair (the last solve's
  * right-hand side and solution, device resident) to the training set. */
 int  gp_nn_add_pair(gp_ctx *ctx);
+/* Append the time-loop increment of the Poisson solver's last two solves,
+ * (b_n - b_{n-1}, x_n - x_{n-1}), formed on the device: the residual a solve
+ * warm-started from x_{n-1} begins with and the correction it needs.  The
+ * first call after gp_nn_init records b_n and x_n only (*added = 0). */
+int  gp_nn_add_increment(gp_ctx *ctx, int *added);
 /* Flat FP32 parameter vector (get and/or set; count returns its length).
  * Order per layer (in, W1..W6, out): weights [co][3x3 tap][ci], then bias. */
 int  gp_nn_params(gp_ctx *ctx, float *get, const float *set, int64_t *count);

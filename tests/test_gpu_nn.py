@@ -193,3 +193,35 @@ def test_training_and_solves(gp, orc):
     st, rc = ctx.solve_fields(dt, tol=1e-8, maxiter=400, solver=1)
     assert st.relres <= 1e-8 and st.used_nn == 1
     ctx.close()
+
+
+def test_device_increments_equal_host_increments(gp):
+    """gp_nn_add_increment forms (b_n - b_{n-1}, x_n - x_{n-1}) of the last two
+    solves on the device.  A network trained on those pairs sees exactly the
+    pairs the caller would form from host copies: the training losses of two
+    equally seeded networks, one fed each way, agree to the bit."""
+    say("[nn increments]")
+    ctx = harvest(gp, 64, 0, seed=3)
+    ctx.nn_init(32, seed=1)
+    my, mx = ctx.field_shape(gp.F_SOL)[0]
+    bs, xs, added = [], [], []
+    for cyc in range(1, 8):
+        ctx.step(5e-12, tol=1e-10, maxiter=400, cycle=cyc)
+        bs.append(ctx.field(gp.F_RHS).ravel().copy())
+        xs.append(ctx.field(gp.F_SOL).ravel().copy())
+        added.append(ctx.nn_add_increment())
+    assert added == [False] + [True] * 6
+    db = np.array([bs[k] - bs[k - 1] for k in range(1, 7)])
+    dx = np.array([xs[k] - xs[k - 1] for k in range(1, 7)])
+    l_dev = [ctx.nn_train(epochs=1, batch=2, lr=0.0) for _ in range(2)]
+    other = harvest(gp, 64, 0, seed=4)
+    other.nn_init(32, seed=1)
+    other.nn_train(epochs=0, rho=db, phi=dx)
+    l_host = [other.nn_train(epochs=1, batch=2, lr=0.0) for _ in range(2)]
+    assert l_dev == l_host and np.isfinite(l_dev[0]) and l_dev[0] > 0, (l_dev, l_host)
+    # and training moves both the same way
+    a = ctx.nn_train(epochs=2, batch=2, lr=1e-3)
+    b = other.nn_train(epochs=2, batch=2, lr=1e-3)
+    assert a == b, (a, b)
+    ctx.close()
+    other.close()
