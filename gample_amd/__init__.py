@@ -90,6 +90,7 @@ EXPORTS = [
     "gp_species_grid_get", "gp_species_read", "gp_load_pair", "gp_generate", "gp_set_cathode", "gp_inject_wall", "gp_reset_grids", "gp_deposit",
     "gp_solve_fields", "gp_solve_darwin", "gp_darwin_stats", "gp_push", "gp_reduce", "gp_reduce_noalloc", "gp_sort", "gp_set_sort_drift", "gp_sort_stats", "gp_moments",
     "gp_step", "gp_field_get", "gp_field_set", "gp_field_energy", "gp_timing", "gp_kernel_time", "gp_stream_probe", "gp_nn_add_increment",
+    "gp_nn_set_system", "gp_nn_add_darwin_increments",
     "gp_nn_init", "gp_nn_train", "gp_nn_infer", "gp_nn_save", "gp_nn_load", "gp_nn_add_pair", "gp_nn_params",
     "gp_nn_dims", "gp_nn_forward", "gp_nn_grad", "gp_solve_rhs", "gp_nn_config",
 ]
@@ -141,6 +142,8 @@ def lib() -> C.CDLL:
         L.gp_kernel_time.argtypes = [P, I, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.gp_stream_probe.argtypes = [P, I, I, C.POINTER(C.c_double)]
         L.gp_nn_add_increment.argtypes = [P, C.POINTER(I)]
+        L.gp_nn_set_system.argtypes = [P, I]
+        L.gp_nn_add_darwin_increments.argtypes = [P, C.POINTER(I)]
         L.gp_nn_init.argtypes = [P, I, C.c_uint64]
         L.gp_nn_train.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), I, I, I, C.c_float,
                                   C.POINTER(C.c_float)]
@@ -402,6 +405,17 @@ class Context:
         a = C.c_int()
         _check(lib().gp_nn_add_increment(self.h, C.byref(a)))
         return bool(a.value)
+
+    def nn_set_system(self, sys):
+        """training pairs for 0 the Poisson solve, 1 the Darwin solves (a
+        change drops the pairs held for the other, keeps the weights)"""
+        _check(lib().gp_nn_set_system(self.h, sys))
+
+    def nn_add_darwin_increments(self):
+        """append the last Darwin solve's chi, Ax, Ay increments; the pairs added"""
+        a = C.c_int()
+        _check(lib().gp_nn_add_darwin_increments(self.h, C.byref(a)))
+        return a.value
 
     def nn_train(self, epochs, batch=4, lr=1e-3, rho=None, phi=None):
         loss = C.c_float()

@@ -703,6 +703,7 @@ int gp_nn_add_pair(gp_ctx *c)
     GP_CHECK_CTX(c);
     if (!c->nn) return gp_fail(GP_ERR_STATE, "gp_nn_init first");
     if (!c->have_solution) return gp_fail(GP_ERR_STATE, "no Poisson solution yet");
+    if (gp_nn_train_system(c) != 0) return gp_fail(GP_ERR_STATE, "the training set is for the Darwin hierarchy");
     GP_TRY({ return gp_nn_add_pair_dev(c, c->ps.b64, c->ps.x64); })
 }
 int gp_nn_add_increment(gp_ctx *c, int *added)
@@ -710,8 +711,35 @@ int gp_nn_add_increment(gp_ctx *c, int *added)
     GP_CHECK_CTX(c);
     if (!c->nn) return gp_fail(GP_ERR_STATE, "gp_nn_init first");
     if (!c->have_solution) return gp_fail(GP_ERR_STATE, "no Poisson solution yet");
+    if (gp_nn_train_system(c) != 0) return gp_fail(GP_ERR_STATE, "the training set is for the Darwin hierarchy");
     int tmp = 0;
-    GP_TRY({ return gp_nn_add_increment_dev(c, c->ps.b64, c->ps.x64, (int64_t)c->ps.n, added ? added : &tmp); })
+    GP_TRY({ return gp_nn_add_increment_dev(c, 0, c->ps.b64, c->ps.x64, (int64_t)c->ps.n, added ? added : &tmp); })
+}
+int gp_nn_add_darwin_increments(gp_ctx *c, int *added)
+{
+    GP_CHECK_CTX(c);
+    if (!c->nn) return gp_fail(GP_ERR_STATE, "gp_nn_init first");
+    if (c->cfg.em_model != 2 || !c->dsol_chi) return gp_fail(GP_ERR_STATE, "no Darwin solve (em_model = 2)");
+    if (gp_nn_train_system(c) != 1) return gp_fail(GP_ERR_STATE, "gp_nn_set_system(ctx, 1) first");
+    GP_TRY({
+        const int64_t m = (int64_t)(c->ds.nkx + 1) * (c->ds.nky + 1);
+        const double *bs[3] = {c->dar_bchi, c->dar_bx, c->dar_by};
+        const double *xs[3] = {c->dsol_chi, c->dsol_ax, c->dsol_ay};
+        int tot = 0;
+        for (int k = 0; k < 3; k++) {
+            int a = 0;
+            int rc = gp_nn_add_increment_dev(c, k + 1, bs[k], xs[k], m, &a);
+            if (rc) return rc;
+            tot += a;
+        }
+        if (added) *added = tot;
+        return GP_OK;
+    })
+}
+int gp_nn_set_system(gp_ctx *c, int sys)
+{
+    GP_CHECK_CTX(c);
+    GP_TRY({ return gp_nn_set_system_impl(c, sys); })
 }
 int gp_nn_config(gp_ctx *c, int loss_kind, int smooth_sweeps, int fcg_cap)
 {

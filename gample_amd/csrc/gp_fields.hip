@@ -403,6 +403,7 @@ int gp_fields_init(gp_ctx *c)
         GP_HIP(hipMalloc((void **)&c->last_ay, m * 8));
         GP_HIP(hipMalloc((void **)&c->dar_bx, m * 8));
         GP_HIP(hipMalloc((void **)&c->dar_by, m * 8));
+        GP_HIP(hipMalloc((void **)&c->dar_bchi, m * 8));
         GP_HIP(hipMemset(c->dsol_chi, 0, m * 8));
         GP_HIP(hipMemset(c->dsol_ax, 0, m * 8));
         GP_HIP(hipMemset(c->dsol_ay, 0, m * 8));
@@ -418,7 +419,7 @@ void gp_fields_free(gp_ctx *c)
     float *fs[] = {c->Ex, c->Ey, c->Ez, c->Bx, c->By, c->Bz, c->dEx, c->dEy, c->dBz};
     for (float *p : fs) if (p) (void)hipFree(p);
     double *ds[] = {c->rho, c->jx, c->jy, c->phi, c->dsol_chi, c->dsol_ax, c->dsol_ay, c->last_ax, c->last_ay,
-                    c->dar_bx, c->dar_by, c->red};
+                    c->dar_bx, c->dar_by, c->dar_bchi, c->red};
     for (double *p : ds) if (p) (void)hipFree(p);
     if (c->hred) (void)hipHostFree(c->hred);
     gp_mg_free(c->ps);
@@ -604,6 +605,7 @@ int gp_solve_darwin_impl(gp_ctx *c, double dt, double tol, int maxiter, int solv
     if (rc) return rc;
     k_darwin_weight<<<gp_cdiv(m, 256), 256, 0, sm>>>(d.b64, a);
     k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(d.x64, c->dsol_chi, m);
+    k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(c->dar_bchi, d.b64, m);
     int s0 = gp_solve_spd(sm, d, tol, maxiter, 1, &st[0], hp);
     if (s0 < 0) return s0;
     k_copy64<<<gp_cdiv(m, 256), 256, 0, sm>>>(c->dsol_chi, d.x64, m);

@@ -160,6 +160,7 @@ struct gp_ctx {
     gp_solver ds;                   // Darwin (Neumann/periodic, singular)
     double *dsol_chi = nullptr, *dsol_ax = nullptr, *dsol_ay = nullptr, *last_ax = nullptr, *last_ay = nullptr;
     double *dar_bx = nullptr, *dar_by = nullptr;   // Ax, Ay right-hand sides
+    double *dar_bchi = nullptr;                    // chi right-hand side of the last Darwin solve
     gp_solve_stats last_darwin[3] = {};   // chi, Ax, Ay of the last Darwin solve
     double *red = nullptr;          // generic reduction scratch
     double *hred = nullptr;         // pinned
@@ -266,7 +267,11 @@ int gp_nn_train_impl(gp_ctx *c, const double *rho, const double *phi, int nsamp,
 int gp_nn_apply(gp_ctx *c, gp_solver *s, const float *in_dev, float *out_dev);
 int gp_nn_infer_impl(gp_ctx *c, const double *rhs, double *out);
 int gp_nn_add_pair_dev(gp_ctx *c, const double *b, const double *x);
-int gp_nn_add_increment_dev(gp_ctx *c, const double *b, const double *x, int64_t m, int *added);
+// stream k of time-loop increments: 0 the Poisson solve, 1..3 the Darwin chi,
+// Ax, Ay solves; pairs go to the training set of the current training system
+int gp_nn_add_increment_dev(gp_ctx *c, int k, const double *b, const double *x, int64_t m, int *added);
+int gp_nn_set_system_impl(gp_ctx *c, int sys);
+int gp_nn_train_system(const gp_ctx *c);   // 0 Poisson, 1 Darwin (0 without a network)
 int gp_nn_grad_impl(gp_ctx *c, const float *in, const float *tgt, int B, int kind, float *loss, float *grad);
 int gp_nn_forward_impl(gp_ctx *c, const float *in, int B, float *out);
 int gp_nn_params_impl(gp_ctx *c, float *get, const float *set, int64_t *count);

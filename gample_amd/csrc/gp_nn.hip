@@ -836,8 +836,11 @@ struct gp_nn {
     float *P = nullptr, *G = nullptr, *M = nullptr, *V = nullptr;
     bf16s *WB = nullptr, *WT = nullptr;
     int64_t adam_t = 0;
-    // activations for a batch of up to bcap images
+    // activations for a batch of up to bcap images of up to hw_cap pixels
+    // (sized for the largest geometry met, so that switching between the
+    // Poisson and Darwin hierarchies reallocates nothing)
     int bcap = 0;
+    int64_t hw_cap = 0, m_cap = 0;
     float *x_in = nullptr, *y_out = nullptr, *tgt = nullptr, *dy = nullptr, *dtmp = nullptr;
     bf16s *a0 = nullptr, *p1 = nullptr, *a1 = nullptr, *p2 = nullptr, *a2 = nullptr, *p3 = nullptr, *a3 = nullptr;
     bf16s *g2 = nullptr, *u2 = nullptr, *g1 = nullptr, *u1 = nullptr, *g0 = nullptr, *u0 = nullptr;
@@ -856,10 +859,12 @@ struct gp_nn {
     float *bw = nullptr;           // per-image loss weights of the current batch
     std::vector<float> hbw;
     float *tmpf = nullptr;   // mx*my FP32 staging
-    // gp_nn_add_increment: the previous solve's b and x, and the differences
-    double *prev_b = nullptr, *prev_x = nullptr, *inc_b = nullptr, *inc_x = nullptr;
-    int64_t prev_n = 0;
-    int have_prev = 0;
+    // time-loop increments, per stream (0 Poisson, 1..3 Darwin chi, Ax, Ay):
+    // the previous solve's b and x; inc_b, inc_x hold the differences
+    double *prev_b[4] = {}, *prev_x[4] = {}, *inc_b = nullptr, *inc_x = nullptr;
+    int64_t prev_n[4] = {}, inc_n = 0;
+    int have_prev[4] = {};
+    int train_sys = 0;   // the hierarchy of the training set: 0 Poisson, 1 Darwin
 };
 
 namespace {
@@ -876,11 +881,12 @@ void nn_free_batch(gp_nn *n)
     n->d0a = n->d0b = n->d1a = n->d1b = n->d1c = n->d2a = n->d2b = n->d2c = n->d3a = n->d3b = nullptr;
     n->x_in = n->y_out = n->tgt = n->dy = n->part = nullptr;
     n->bcap = 0;
+    n->hw_cap = 0;
 }
 
 int nn_alloc_batch(gp_nn *n, int B)
 {
-    if (B <= n->bcap) return GP_OK;
+    if (B <= n->bcap && (int64_t)n->H * n->W <= n->hw_cap) return GP_OK;
     nn_free_batch(n);
     const int64_t P0 = (int64_t)B * n->H * n->W, P1 = P0 / 4, P2 = P0 / 16, P3 = P0 / 64;
     const int c0 = n->c0, c1 = 2 * c0, c2 = 4 * c0, c3 = 4 * c0;
@@ -908,6 +914,7 @@ int nn_alloc_batch(gp_nn *n, int B)
     if (n->lpart) { (void)hipFree(n->lpart); n->lpart = nullptr; }
     GP_HIP(hipMalloc((void **)&n->lpart, ((P0 + NB - 1) / NB + 16) * 4));
     n->bcap = B;
+    n->hw_cap = (int64_t)n->H * n->W;
     return GP_OK;
 }
 
@@ -1197,10 +1204,15 @@ int nn_set_grid(gp_nn *n, const gp_solver &s)
     n->singular = s.singular;
     n->cyr = (float)(s.lv[0].hy2i / s.lv[0].hx2i);
     if (mx == n->mx && my == n->my && n->tmpf) return GP_OK;
-    nn_free_batch(n);
-    if (n->tmpf) { (void)hipFree(n->tmpf); n->tmpf = nullptr; }
     n->mx = mx; n->my = my; n->W = W; n->H = H;
-    GP_HIP(hipMalloc((void **)&n->tmpf, (int64_t)mx * my * 4));
+    if ((int64_t)H * W > n->hw_cap) nn_free_batch(n);   // the next nn_alloc_batch sizes them anew
+    if ((int64_t)mx * my > n->m_cap) {
+        if (n->tmpf) (void)hipFree(n->tmpf);
+        n->tmpf = nullptr;
+        n->m_cap = 0;
+        GP_HIP(hipMalloc((void **)&n->tmpf, (int64_t)mx * my * 4));
+        n->m_cap = (int64_t)mx * my;
+    }
     return GP_OK;
 }
 
@@ -1260,8 +1272,9 @@ void gp_nn_destroy(gp_ctx *c)
     gp_nn *n = c->nn;
     if (!n) return;
     nn_free_batch(n);
-    void *ps[] = {n->P, n->G, n->M, n->V, n->WB, n->WT, n->stats, n->lpart, n->tmpf, n->bw,
-                  n->prev_b, n->prev_x, n->inc_b, n->inc_x};
+    void *ps[] = {n->P, n->G, n->M, n->V, n->WB, n->WT, n->stats, n->lpart, n->tmpf, n->bw, n->inc_b, n->inc_x,
+                  n->prev_b[0], n->prev_b[1], n->prev_b[2], n->prev_b[3],
+                  n->prev_x[0], n->prev_x[1], n->prev_x[2], n->prev_x[3]};
     for (void *p : ps) if (p) (void)hipFree(p);
     for (float *p : n->ds_in) (void)hipFree(p);
     for (float *p : n->ds_tgt) (void)hipFree(p);
@@ -1275,7 +1288,7 @@ int gp_nn_add_pair_dev(gp_ctx *c, const double *b, const double *x)
 {
     gp_nn *n = c->nn;
     hipStream_t st = c->stream;
-    int rc0 = nn_set_grid(n, c->ps);
+    int rc0 = nn_set_grid(n, n->train_sys ? c->ds : c->ps);
     if (rc0) return rc0;
     const int64_t m = (int64_t)n->mx * n->my, HW = (int64_t)n->H * n->W;
     float *in = nullptr, *tg = nullptr;
@@ -1309,36 +1322,64 @@ __global__ void k_sub_keep(const double *a, double *prev, double *d, int64_t n, 
 }
 }  // namespace
 
-// the time-loop increment (b_n - b_{n-1}, x_n - x_{n-1}) of the last two
-// solves on the device (b, x: the current ones, m unknowns); the first call
-// only records b_n, x_n.  *added = 1 when a pair was appended.
-int gp_nn_add_increment_dev(gp_ctx *c, const double *b, const double *x, int64_t m, int *added)
+// the time-loop increment (b_n - b_{n-1}, x_n - x_{n-1}) of stream k's last
+// two solves on the device (b, x: the current ones, m unknowns); the first
+// call of a stream only records b_n, x_n.  *added = 1 when a pair was appended.
+int gp_nn_add_increment_dev(gp_ctx *c, int k, const double *b, const double *x, int64_t m, int *added)
 {
     gp_nn *n = c->nn;
     hipStream_t st = c->stream;
     *added = 0;
-    if (n->prev_n != m) {
-        double **bs[] = {&n->prev_b, &n->prev_x, &n->inc_b, &n->inc_x};
-        for (double **p : bs) {
-            if (*p) GP_HIP(hipFree(*p));
-            *p = nullptr;
-        }
-        n->prev_n = 0;
-        n->have_prev = 0;
-        for (double **p : bs) GP_HIP(hipMalloc((void **)p, m * sizeof(double)));
-        n->prev_n = m;
+    if (k < 0 || k > 3) return gp_fail(GP_ERR_ARG, "increment stream must be 0..3");
+    if (n->inc_n < m) {
+        if (n->inc_b) GP_HIP(hipFree(n->inc_b));
+        if (n->inc_x) GP_HIP(hipFree(n->inc_x));
+        n->inc_b = n->inc_x = nullptr;
+        n->inc_n = 0;
+        GP_HIP(hipMalloc((void **)&n->inc_b, m * sizeof(double)));
+        GP_HIP(hipMalloc((void **)&n->inc_x, m * sizeof(double)));
+        n->inc_n = m;
+    }
+    if (n->prev_n[k] != m) {
+        if (n->prev_b[k]) GP_HIP(hipFree(n->prev_b[k]));
+        if (n->prev_x[k]) GP_HIP(hipFree(n->prev_x[k]));
+        n->prev_b[k] = n->prev_x[k] = nullptr;
+        n->prev_n[k] = 0;
+        n->have_prev[k] = 0;
+        GP_HIP(hipMalloc((void **)&n->prev_b[k], m * sizeof(double)));
+        GP_HIP(hipMalloc((void **)&n->prev_x[k], m * sizeof(double)));
+        n->prev_n[k] = m;
     }
     const int g = (int)std::min<int64_t>(gp_cdiv(m, 256), 4096);
-    k_sub_keep<<<g, 256, 0, st>>>(b, n->prev_b, n->inc_b, m, n->have_prev);
-    k_sub_keep<<<g, 256, 0, st>>>(x, n->prev_x, n->inc_x, m, n->have_prev);
+    k_sub_keep<<<g, 256, 0, st>>>(b, n->prev_b[k], n->inc_b, m, n->have_prev[k]);
+    k_sub_keep<<<g, 256, 0, st>>>(x, n->prev_x[k], n->inc_x, m, n->have_prev[k]);
     GP_HIP(hipGetLastError());
-    if (!n->have_prev) {
-        n->have_prev = 1;
+    if (!n->have_prev[k]) {
+        n->have_prev[k] = 1;
         return GP_OK;
     }
     int rc = gp_nn_add_pair_dev(c, n->inc_b, n->inc_x);
     if (rc) return rc;
     *added = 1;
+    return GP_OK;
+}
+
+// the hierarchy the training pairs belong to (0 Poisson, 1 Darwin); a change
+// drops the pairs held for the other one (the weights are kept)
+int gp_nn_set_system_impl(gp_ctx *c, int sys)
+{
+    gp_nn *n = c->nn;
+    if (!n) return gp_fail(GP_ERR_STATE, "gp_nn_init first");
+    if (sys < 0 || sys > 1) return gp_fail(GP_ERR_ARG, "training system must be 0 (Poisson) or 1 (Darwin)");
+    if (sys == 1 && c->cfg.em_model != 2) return gp_fail(GP_ERR_STATE, "Darwin training needs em_model = 2");
+    if (sys == n->train_sys) return GP_OK;
+    GP_HIP(hipStreamSynchronize(c->stream));
+    for (float *p : n->ds_in) GP_HIP(hipFree(p));
+    for (float *p : n->ds_tgt) GP_HIP(hipFree(p));
+    n->ds_in.clear();
+    n->ds_tgt.clear();
+    n->ds_E.clear();
+    n->train_sys = sys;
     return GP_OK;
 }
 
@@ -1348,7 +1389,7 @@ int gp_nn_train_impl(gp_ctx *c, const double *rho, const double *phi, int nsamp,
     gp_nn *n = c->nn;
     if (!n) return gp_fail(GP_ERR_STATE, "gp_nn_init first");
     hipStream_t st = c->stream;
-    int rc0 = nn_set_grid(n, c->ps);
+    int rc0 = nn_set_grid(n, n->train_sys ? c->ds : c->ps);
     if (rc0) return rc0;
     const int64_t m = (int64_t)n->mx * n->my, HW = (int64_t)n->H * n->W;
     if (nsamp > 0) {
@@ -1577,3 +1618,4 @@ int gp_nn_config_impl(gp_ctx *c, int loss_kind, int smooth, int fcg_cap)
     return GP_OK;
 }
 int gp_nn_fcg_cap(const gp_ctx *c) { return c->nn ? c->nn->fcg_cap : 0; }
+int gp_nn_train_system(const gp_ctx *c) { return c->nn ? c->nn->train_sys : 0; }

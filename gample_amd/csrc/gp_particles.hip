@@ -1266,7 +1266,10 @@ __global__ void k_inject(InjArgs a)
     float x = a.lo + vx * a.dtdx * r4;
     const float hi = a.lo + a.ngx;
     if (x >= hi) x = __int_as_float(__float_as_int(hi) - 1);   // a particle faster than L/dt stays inside
-    const float y = a.lo + r5 * a.ngy;
+    // r5 = 1 - 2^-24 can round onto the upper face (as a loaded position can)
+    float y = a.lo + r5 * a.ngy;
+    const float hy = a.lo + a.ngy;
+    if (y >= hy) y = __int_as_float(__float_as_int(hy) - 1);
     const int64_t j = a.base + k;
     a.s.x[j] = x; a.s.y[j] = y; a.s.vx[j] = vx; a.s.vy[j] = vy; a.s.vz[j] = vz;
 }

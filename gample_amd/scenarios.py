@@ -20,7 +20,9 @@ particle balance and report the run.
 
 learned: the learned Poisson preconditioner trained at the workload's scale
 on thousands of the run's own time-loop increments, kept on the device, and
-compared with MG-CG on held-out steps of the same run.
+compared with MG-CG on held-out steps of the same run.  darwin_learned: the
+network trained on a Darwin run's Poisson increments, fine-tuned on its
+Darwin ones and used in all four solves, against the same run with MG-CG.
 """
 from __future__ import annotations
 
@@ -303,6 +305,111 @@ def learned(kind="periodic", ngx=1024, ppc=None, pairs=10000, eval_fields=8, war
     return out
 
 
+def darwin_learned(ngx=512, ppc=128, warmup=20, poisson_pairs=1000, darwin_steps=1000, eval_steps=100,
+                   train_seconds=60.0, solver=2, tol=1e-8, order=3, seed=0, log=None):
+    """The Darwin model (the darwin run's set-up) with the learned
+    preconditioner: the network is trained on the run's own Poisson increments,
+    then fine-tuned on its Darwin increments (the chi, Ax and Ay systems,
+    EMField.mm:1147-1275, each b_n - b_{n-1} -> x_n - x_{n-1}), and serves as
+    the warm start (solver 2) or flexible-CG preconditioner (solver 1) of all
+    four solves for eval_steps cycles.  The same seeded run with plain MG-CG
+    gives the reference figures for the same cycles."""
+
+    def make():
+        P = plasma(ppc=ppc)
+        n = ppc * ngx * ngx
+        cfg = make_config(ngx, ngx, P["dx"], P["dx"], order=order, pbc=("Reflect", "Reflect", "periodic", "periodic"),
+                          fbc=("Dirichlet", "Dirichlet", "periodic", "periodic"), b0=(0.0, 0.0, P["bz"]),
+                          cur_dens=1, em_model=2)
+        ctx = Context(cfg)
+        se = ctx.add_species(-EC, ME, P["w"], n, vth_hint=P["vth_e"] / math.sqrt(2))
+        si = ctx.add_species(EC, MXE, P["w"], n, vth_hint=P["vth_i"] / math.sqrt(2))
+        _load(ctx, se, si, P, ngx, ngx, n, seed)
+        ctx.set_sort_drift(3e-3)
+        ctx.reset_grids(); ctx.deposit(se); ctx.deposit(si)
+        return ctx, P["dt"]
+
+    total = warmup + (poisson_pairs + 1) + (darwin_steps + 1) + eval_steps
+
+    def record(ctx, st, rows, t):
+        d3 = ctx.darwin_stats()
+        rows.append([st.iterations, 1e3 * st.seconds] + [d.iterations for d in d3] +
+                    [1e3 * d.seconds for d in d3] + [max(st.relres, *[d.relres for d in d3]),
+                                                     st.fell_back + sum(d.fell_back for d in d3), t])
+
+    # plain MG-CG
+    ctx, dt = make()
+    ref = []
+    for cyc in range(1, total + 1):
+        t0 = time.perf_counter()
+        st, _ = ctx.step(dt, tol=tol, maxiter=400, cycle=cyc)
+        ctx.sync()
+        if cyc > total - eval_steps:
+            record(ctx, st, ref, 1e3 * (time.perf_counter() - t0))
+    ctx.close()
+    if log:
+        log("[darwin-learned] MG-CG run done")
+    # learned
+    ctx, dt = make()
+    cyc = 0
+
+    def step(solver_=0):
+        nonlocal cyc
+        cyc += 1
+        t0 = time.perf_counter()
+        st, _ = ctx.step(dt, tol=tol, maxiter=400, solver=solver_, cycle=cyc)
+        ctx.sync()
+        return st, 1e3 * (time.perf_counter() - t0)
+
+    for _ in range(warmup):
+        step()
+    ctx.nn_init(32, seed=1)
+    ctx.nn_config(1, 2, 0)
+    for _ in range(poisson_pairs + 1):
+        step()
+        ctx.nn_add_increment()
+
+    def train(budget):
+        t = time.perf_counter()
+        losses = [ctx.nn_train(epochs=1, batch=4, lr=0.0)]
+        while time.perf_counter() - t < budget and len(losses) < 200:
+            losses.append(ctx.nn_train(epochs=1, batch=4, lr=1e-3))
+        return losses
+
+    lp = train(train_seconds / 2)
+    if log:
+        log(f"[darwin-learned] Poisson training: loss {lp[0]:.3f} -> {lp[-1]:.4f} in {len(lp) - 1} epochs")
+    ctx.nn_set_system(1)
+    added = 0
+    for _ in range(darwin_steps + 1):
+        step()
+        added += ctx.nn_add_darwin_increments()
+    ld = train(train_seconds / 2)
+    if log:
+        log(f"[darwin-learned] Darwin fine-tuning on {added} pairs: loss {ld[0]:.3f} -> {ld[-1]:.4f} "
+            f"in {len(ld) - 1} epochs")
+    got = []
+    for _ in range(eval_steps):
+        st, t = step(solver)
+        record(ctx, st, got, t)
+    ctx.close()
+    names = ["poisson_iterations", "poisson_ms", "chi_iterations", "ax_iterations", "ay_iterations", "chi_ms",
+             "ax_ms", "ay_ms"]
+    A, B = np.array(ref, dtype=float), np.array(got, dtype=float)
+    res = {"mg_cg": {k: float(A[:, i].mean()) for i, k in enumerate(names)},
+           "learned": {k: float(B[:, i].mean()) for i, k in enumerate(names)}}
+    for key, M in (("mg_cg", A), ("learned", B)):
+        res[key]["max_relres"] = float(M[:, 8].max())
+        res[key]["fell_back"] = int(M[:, 9].sum())
+        res[key]["step_ms"] = float(M[:, 10].mean())
+        res[key]["darwin_ms"] = float(M[:, 5:8].sum(axis=1).mean())
+    res["darwin_time_ratio"] = res["learned"]["darwin_ms"] / res["mg_cg"]["darwin_ms"]
+    res["poisson_time_ratio"] = res["learned"]["poisson_ms"] / res["mg_cg"]["poisson_ms"]
+    return dict(config=dict(ngx=ngx, ppc=ppc, order=order, tol=tol, warmup=warmup, poisson_pairs=poisson_pairs,
+                            darwin_steps=darwin_steps, darwin_pairs=added, eval_steps=eval_steps, solver=solver),
+                poisson_loss=[lp[0], lp[-1], len(lp) - 1], darwin_loss=[ld[0], ld[-1], len(ld) - 1], **res)
+
+
 def summary(out):
     """the run's figures for a report: sizes, wall time, per-cycle time,
     solver iterations, and for the bounded plasma the particle balance"""
@@ -342,13 +449,14 @@ def main(argv=None):
     import argparse
     import json
     ap = argparse.ArgumentParser(description="bounded-plasma, Darwin or learned-preconditioner run on one GPU")
-    ap.add_argument("which", choices=["bounded", "darwin", "learned"])
+    ap.add_argument("which", choices=["bounded", "darwin", "learned", "darwin_learned"])
     ap.add_argument("--kind", choices=["periodic", "bounded"], default="periodic",
                     help="learned: the workload the pairs come from")
     ap.add_argument("--pairs", type=int, default=10000, help="learned: training increments")
     ap.add_argument("--eval-fields", type=int, default=8)
     ap.add_argument("--train-seconds", type=float, default=120.0)
     ap.add_argument("--save", default=None, help="learned: write the trained network here (gp_nn_save)")
+    ap.add_argument("--solver", type=int, default=2, help="darwin_learned: 2 network warm start, 1 flexible CG")
     ap.add_argument("--ngx", type=int, default=512)
     ap.add_argument("--ppc", type=int, default=None)
     ap.add_argument("--steps", type=int, default=500)
@@ -356,6 +464,11 @@ def main(argv=None):
     ap.add_argument("--history", default=None, help="write the per-cycle history here as JSON")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
+    if a.which == "darwin_learned":
+        out = darwin_learned(ngx=a.ngx, ppc=a.ppc or 128, train_seconds=a.train_seconds, order=a.order,
+                             poisson_pairs=a.pairs, darwin_steps=a.pairs, eval_steps=a.steps, solver=a.solver, log=log)
+        print(json.dumps(out), flush=True)
+        return
     if a.which == "learned":
         out = learned(kind=a.kind, ngx=a.ngx, ppc=a.ppc, pairs=a.pairs, eval_fields=a.eval_fields,
                       train_seconds=a.train_seconds, order=a.order, save=a.save, log=log)

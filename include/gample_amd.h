@@ -326,6 +326,17 @@ int  gp_nn_add_pair(gp_ctx *ctx);
  * warm-started from x_{n-1} begins with and the correction it needs.  The
  * first call after gp_nn_init records b_n and x_n only (*added = 0). */
 int  gp_nn_add_increment(gp_ctx *ctx, int *added);
+/* The hierarchy the training pairs belong to: 0 the Poisson solve (the
+ * default), 1 the Darwin solves (em_model = 2).  Changing it drops the pairs
+ * held for the other one and keeps the weights, so a network trained on
+ * Poisson pairs can be fine-tuned on Darwin ones.  The network serves either
+ * hierarchy through gp_solve_fields / gp_solve_darwin / gp_step (solver 1..3). */
+int  gp_nn_set_system(gp_ctx *ctx, int sys);
+/* Append the time-loop increments of the last Darwin solve's three systems
+ * (chi, Ax, Ay: b_n - b_{n-1} and x_n - x_{n-1} of each, on the device) to the
+ * Darwin training set; a system's first call only records it.  *added: the
+ * pairs appended (0 or 3).  Reference: the systems of EMField.mm:1147-1275. */
+int  gp_nn_add_darwin_increments(gp_ctx *ctx, int *added);
 /* Flat FP32 parameter vector (get and/or set; count returns its length).
  * Order per layer (in, W1..W6, out): weights [co][3x3 tap][ci], then bias. */
 int  gp_nn_params(gp_ctx *ctx, float *get, const float *set, int64_t *count);

@@ -591,7 +591,10 @@ void gpo_inject_wall(uint64_t seed, uint64_t stream, int64_t n, float vth, const
         const float hi = lo + ngx;
         if (xx >= hi) xx = nextafterf(hi, 0.0f);
         x[k] = xx;
-        y[k] = lo + r5 * ngy;
+        /* r5 = 1 - 2^-24 can round onto the upper face (B16) */
+        float yy = lo + r5 * ngy;
+        if (yy >= lo + ngy) yy = nextafterf(lo + ngy, 0.0f);
+        y[k] = yy;
         vx[k] = v;
         vy[k] = u[1] + am * cosf(2.0f * PIf * r3);
         vz[k] = u[2] + am * sinf(2.0f * PIf * r3);

@@ -225,3 +225,60 @@ def test_device_increments_equal_host_increments(gp):
     assert a == b, (a, b)
     ctx.close()
     other.close()
+
+
+def test_darwin_fine_tune(gp):
+    """A network trained on Poisson increments is fine-tuned on the Darwin
+    solves' (chi, Ax, Ay) increments: switching the training system keeps the
+    weights and starts an empty Darwin set, the first Darwin call only records
+    the systems, later ones add three pairs, training lowers the loss, and the
+    Darwin solves served by the network reach the same 1e-8 residual."""
+    say("[nn darwin]")
+    ngx = 64
+    cfg = gp.make_config(ngx, ngx, 0.005, 0.005, order=1, pbc=("Reflect", "Reflect", "periodic", "periodic"),
+                         fbc=("Dirichlet", "Dirichlet", "periodic", "periodic"), b0=(0, 0, 300.0),
+                         cur_dens=1, em_model=2)
+    ctx = gp.Context(cfg)
+    n = 64 * ngx * ngx
+    se = ctx.add_species(-EC, ME, 4e3, n, vth_hint=1e8)
+    si = ctx.add_species(EC, 2.18e-22, 4e3, n, vth_hint=1e5)
+    G = gp.Gen()
+    G.gen_type = 7
+    G.xmin, G.xmax, G.ymin, G.ymax = 0.0, ngx * 0.005, 0.0, ngx * 0.005
+    G.ele_vth, G.ion_vth = 1.9e8, 1.2e5
+    G.pert_delta, G.pert_mode = 0.05, 3
+    ctx.load_pair(se, si, G, 5, 0, n)
+    ctx.sort(se); ctx.sort(si)
+    ctx.reset_grids(); ctx.deposit(se); ctx.deposit(si)
+    ctx.nn_init(32, seed=1)
+    cyc = 0
+    for _ in range(6):
+        cyc += 1
+        ctx.step(5e-12, tol=1e-8, maxiter=400, cycle=cyc)
+        ctx.nn_add_increment()
+    ctx.nn_train(epochs=3, batch=2, lr=1e-3)
+    w0 = ctx.nn_params()
+    with pytest.raises(gp.GampleError):
+        ctx.nn_add_darwin_increments()               # the training set is still the Poisson one
+    ctx.nn_set_system(1)
+    np.testing.assert_array_equal(ctx.nn_params(), w0)
+    with pytest.raises(gp.GampleError):
+        ctx.nn_add_increment()                       # and now the Darwin one
+    added = []
+    for _ in range(5):
+        cyc += 1
+        ctx.step(5e-12, tol=1e-8, maxiter=400, cycle=cyc)
+        added.append(ctx.nn_add_darwin_increments())
+    assert added == [0, 3, 3, 3, 3]
+    l0 = ctx.nn_train(epochs=1, batch=2, lr=0.0)
+    l1 = None
+    for _ in range(20):
+        l1 = ctx.nn_train(epochs=1, batch=2, lr=1e-3)
+    assert l1 < l0, (l0, l1)
+    for solver in (2, 1):
+        cyc += 1
+        st, _ = ctx.step(5e-12, tol=1e-8, maxiter=400, solver=solver, cycle=cyc)
+        assert st.relres <= 1e-8
+        for d in ctx.darwin_stats():
+            assert d.relres <= 1e-8, (solver, d.relres)
+    ctx.close()

@@ -593,6 +593,16 @@ def test_wall_injection(gp, orc):
     rho, _, _ = orc.deposit_fx(g, [a[1000:] for a in got])
     np.testing.assert_array_equal(ctx.acc(sid), rho)
     ctx.close()
+    # stream 24,807,051 of seed 1 draws r5 = 1 - 2^-24: on a 512-cell wall its
+    # y would round onto the upper face, and is kept just below it
+    ctx = mkctx(gp, 512, 512, 3, pbc=("Delete", "Delete", "periodic", "periodic"),
+                fbc=("Dirichlet", "Dirichlet", "periodic", "periodic"), dx=dx)
+    sid = ctx.add_species(-EC, ME, 1.0, 1000, vth_hint=1e8)
+    ctx.inject_wall(sid, 1, float(vth), u, dt, 1, 24807051, deposit=1)
+    y = ctx.download(sid)[1]
+    assert y[0] == np.nextafter(np.float32(513.0), np.float32(0))
+    assert y[0] == orc.inject_wall(1, 24807051, 1, float(vth), u, dtdx, 1.0, 512.0, 512.0)[1][0]
+    ctx.close()
 
 
 

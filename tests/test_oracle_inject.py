@@ -34,3 +34,17 @@ def test_injector_streams_are_disjoint_and_reproducible():
         np.testing.assert_array_equal(p[500:], q)      # id = stream + k
     c = orc.inject_wall(4, 0, 1000, 1.0, (0, 0, 0), 0.1, 1.0, 16.0, 16.0)
     assert not np.array_equal(a[1], c[1])
+
+
+def test_injected_position_below_the_upper_face():
+    """Stream 24,807,051 of seed 1 draws r5 = 1 - 2^-24 (its fifth PCG32 output
+    is 2^32 - 210, whose float is 2^32 - 256).  On a 512-cell wall starting at
+    grid position 1, lo + r5 * ngy = 513 - 2^-15 rounds to 513, the upper face;
+    the injector keeps the particle in the last cell, as the loader does."""
+    lo, ngy = 1.0, 512.0
+    x, y, vx, vy, vz = orc.inject_wall(1, 24807051, 1, 1.0, (0, 0, 0), 0.1, lo, 512.0, ngy)
+    assert np.float32(lo) + np.float32(1 - 2.0 ** -24) * np.float32(ngy) == np.float32(lo + ngy)
+    assert y[0] == np.nextafter(np.float32(lo + ngy), np.float32(0))
+    # a neighbouring stream is unaffected by the guard
+    _, y2, _, _, _ = orc.inject_wall(1, 24807050, 2, 1.0, (0, 0, 0), 0.1, lo, 512.0, ngy)
+    assert y2[1] == y[0] and y2[0] < lo + ngy
