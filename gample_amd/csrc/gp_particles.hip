@@ -24,6 +24,13 @@
 #include <cmath>
 #include <mutex>
 
+// a species with more than this fraction of its particles changing cell per
+// push is sorted without bank-class interleaving, and its hot push adds its
+// charge in 32-bit halves
+#ifndef GP_CLS_MOVED
+#define GP_CLS_MOVED 0.01
+#endif
+
 namespace {
 
 constexpr int WIN = GP_TILE + 2 * GP_DRIFT;   // cell window of a tile
@@ -196,6 +203,15 @@ __device__ __forceinline__ unsigned q23(float s)
 {
     return __float_as_uint(__builtin_fmaf(s, 8388608.0f, 8388608.0f)) & 0xFFFFFFu;
 }
+// rint(s * 2^16) the same way: s * 2^16 + 2^23 lies in [2^23, 2^23 + 2^16],
+// so the low 23 bits of its encoding are the rounded value.  The order-3
+// charge uses it: its largest weight is 2/3, so q <= 43691 and a product of
+// two fits 31 bits (one 24-bit multiply), and the windows add the products
+// unshifted; they are scaled by 2^14 to the 46-bit fixed point at the flush.
+__device__ __forceinline__ unsigned q16(float s)
+{
+    return __float_as_uint(__builtin_fmaf(s, 65536.0f, 8388608.0f)) & 0x7FFFFFu;
+}
 
 // periodic wrap x = fmod(x + (ng-ngb), ng) + ngb (Particle.mm:216-221); the
 // conditional subtraction is the exact remainder for 0 <= a < 2 ng.
@@ -240,7 +256,9 @@ struct PushArgs {
 
 // BCK: 0 any particle boundary codes (read from the arguments), 1 periodic
 // on all four sides (the wrap only: no absorbed or reflected particles)
-template <int ORDER, bool UNIB, int DEP, bool PUSH, int BCK>
+// DC: 1 adds the hot kernel's charge in 32-bit halves (a species whose lanes
+// address random cells: half the bytes per atomic, see the deposit)
+template <int ORDER, bool UNIB, int DEP, bool PUSH, int BCK, int DC = 0>
 __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
 {
     constexpr int PB = push_block(ORDER);
@@ -257,6 +275,7 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
     constexpr int NB = (PUSH && !UNIB) ? W * W : 1;
     constexpr int NR = DEP >= 1 ? W * W : 1;
     constexpr int NJ = DEP >= 2 ? W * W : 1;
+    constexpr int QSH = ORDER == 3 ? 14 : 0;   // window charge scaled by 2^QSH at the flush
     static_assert(!CW || NR * 2 >= 2 * W * W, "CW stages the E taps in the charge window");
     __shared__ f4v sCx[NC], sCy[NC];
     __shared__ float sEx[NF], sEy[NF];
@@ -531,23 +550,49 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
             float sx[NW], sy[NW];
             wts<ORDER>(x - fi2, sx);
             wts<ORDER>(y - fj1, sy);
+            // node charge qx[i] qy[j] 2^QSH in the 46-bit fixed point
             unsigned qx[NW], qy[NW];
 #pragma unroll
-            for (int k = 0; k < NW; k++) { qx[k] = q23(sx[k]); qy[k] = q23(sy[k]); }
+            for (int k = 0; k < NW; k++) {
+                if constexpr (ORDER == 3) { qx[k] = q16(sx[k]); qy[k] = q16(sy[k]); }
+                else { qx[k] = q23(sx[k]); qy[k] = q23(sy[k]); }
+            }
+            auto qq = [&](int i, int j) -> unsigned long long {
+                if constexpr (ORDER == 3) return (unsigned long long)__umul24(qx[i], qy[j]);
+                else return (unsigned long long)qx[i] * qy[j];
+            };
             if (inw) {
-                unsigned long long *r = sRho + wi + rowW<W>(wj);
+                if constexpr (ORDER == 3 && DC == 1) {
+                    // the window as 32-bit low words and, W*W further, high
+                    // words: a lane whose addition wraps its low word adds 1
+                    // to the high word (the returned old value shows the wrap)
+                    unsigned *lo = (unsigned *)sRho + wi + rowW<W>(wj), *hi = lo + W * W;
+                    unsigned pp[NW * NW], ov[NW * NW];
 #pragma unroll
-                for (int j = 0; j < NW; j++)
+                    for (int j = 0; j < NW; j++)
 #pragma unroll
-                    for (int i = 0; i < NW; i++)
-                        atomicAdd(r + i + j * W, (unsigned long long)qx[i] * qy[j]);
+                        for (int i = 0; i < NW; i++) {
+                            pp[i + NW * j] = __umul24(qx[i], qy[j]);
+                            ov[i + NW * j] = atomicAdd(lo + i + j * W, pp[i + NW * j]);
+                        }
+#pragma unroll
+                    for (int j = 0; j < NW; j++)
+#pragma unroll
+                        for (int i = 0; i < NW; i++)
+                            if (ov[i + NW * j] + pp[i + NW * j] < ov[i + NW * j]) atomicAdd(hi + i + j * W, 1u);
+                } else {
+                    unsigned long long *r = sRho + wi + rowW<W>(wj);
+#pragma unroll
+                    for (int j = 0; j < NW; j++)
+#pragma unroll
+                        for (int i = 0; i < NW; i++) atomicAdd(r + i + j * W, qq(i, j));
+                }
             } else {
                 unsigned long long *r = a.rho + (i2 - NGB) + (int64_t)(j1 - NGB) * pey;
 #pragma unroll
                 for (int j = 0; j < NW; j++)
 #pragma unroll
-                    for (int i = 0; i < NW; i++)
-                        atomicAdd(r + i + (int64_t)j * pey, (unsigned long long)qx[i] * qy[j]);
+                    for (int i = 0; i < NW; i++) atomicAdd(r + i + (int64_t)j * pey, qq(i, j) << QSH);
             }
             if constexpr (DEP >= 2) {
                 // current (Particle.mm:363-373)
@@ -605,7 +650,12 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
                 unsigned long long *st = a.stage + (int64_t)b * (W * W) * (DEP >= 2 ? 3 : 1);
 #pragma unroll 1
                 for (int e = tid; e < W * W; e += PB) {
-                    st[e] = sRho[e];
+                    if constexpr (ORDER == 3 && DC == 1) {
+                        const unsigned *lo = (const unsigned *)sRho;
+                        st[e] = (((unsigned long long)lo[W * W + e] << 32) | lo[e]) << QSH;
+                    } else {
+                        st[e] = sRho[e] << QSH;
+                    }
                     if constexpr (DEP >= 2) { st[W * W + e] = sJx[e]; st[2 * W * W + e] = sJy[e]; }
                 }
             }
@@ -1330,29 +1380,36 @@ __global__ void k_mom_scale(const unsigned long long *acc, int64_t ng, double sn
 
 // ---------------------------------------------------------------------------
 template <int ORDER, bool UNIB, int DEP, bool PUSH>
-void launch_push_t(int grid, hipStream_t s, const PushArgs &a)
+void launch_push_t(int grid, hipStream_t s, const PushArgs &a, bool dc)
 {
     const bool all_periodic = a.bc0 == GP_PBC_PERIODIC && a.bc1 == GP_PBC_PERIODIC && a.bc2 == GP_PBC_PERIODIC &&
                               a.bc3 == GP_PBC_PERIODIC;
+    if constexpr (push_cw(ORDER, UNIB, DEP, PUSH)) {
+        if (dc) {
+            if (all_periodic) k_push<ORDER, UNIB, DEP, PUSH, 1, 1><<<grid, push_block(ORDER), 0, s>>>(a);
+            else k_push<ORDER, UNIB, DEP, PUSH, 0, 1><<<grid, push_block(ORDER), 0, s>>>(a);
+            return;
+        }
+    }
     if (PUSH && all_periodic) k_push<ORDER, UNIB, DEP, PUSH, 1><<<grid, push_block(ORDER), 0, s>>>(a);
     else k_push<ORDER, UNIB, DEP, PUSH, 0><<<grid, push_block(ORDER), 0, s>>>(a);
 }
 template <int ORDER>
-int launch_push_o(int grid, hipStream_t s, const PushArgs &a, bool unib, int dep, bool push)
+int launch_push_o(int grid, hipStream_t s, const PushArgs &a, bool unib, int dep, bool push, bool dc)
 {
     if (push) {
         if (unib) {
-            if (dep == 0) launch_push_t<ORDER, true, 0, true>(grid, s, a);
-            else if (dep == 1) launch_push_t<ORDER, true, 1, true>(grid, s, a);
-            else launch_push_t<ORDER, true, 2, true>(grid, s, a);
+            if (dep == 0) launch_push_t<ORDER, true, 0, true>(grid, s, a, dc);
+            else if (dep == 1) launch_push_t<ORDER, true, 1, true>(grid, s, a, dc);
+            else launch_push_t<ORDER, true, 2, true>(grid, s, a, dc);
         } else {
-            if (dep == 0) launch_push_t<ORDER, false, 0, true>(grid, s, a);
-            else if (dep == 1) launch_push_t<ORDER, false, 1, true>(grid, s, a);
-            else launch_push_t<ORDER, false, 2, true>(grid, s, a);
+            if (dep == 0) launch_push_t<ORDER, false, 0, true>(grid, s, a, dc);
+            else if (dep == 1) launch_push_t<ORDER, false, 1, true>(grid, s, a, dc);
+            else launch_push_t<ORDER, false, 2, true>(grid, s, a, dc);
         }
     } else {
-        if (dep == 1) launch_push_t<ORDER, true, 1, false>(grid, s, a);
-        else launch_push_t<ORDER, true, 2, false>(grid, s, a);
+        if (dep == 1) launch_push_t<ORDER, true, 1, false>(grid, s, a, dc);
+        else launch_push_t<ORDER, true, 2, false>(grid, s, a, dc);
     }
     GP_HIP(hipGetLastError());
     return GP_OK;
@@ -1427,13 +1484,17 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
     const int grid = (a.use_tiles ? c->ntiles : 0) + extra;
     if (grid == 0) return GP_OK;
     if (grid > S.ke_nblk) return gp_fail(GP_ERR_STATE, "kinetic-energy partial buffer too small");
+    // a species whose particles change cell too often to keep the bank-class
+    // interleave of its sort (its lanes address random cells) adds its charge
+    // in 32-bit halves; both forms give the same integers
+    const bool dc = S.last_moved >= 0 && (double)S.last_moved >= GP_CLS_MOVED * (double)S.n;
     const bool tk = push && c->timing;
     if (tk) { int rc = gp_time_mark(c, GP_TIME_PUSHK, -1); if (rc) return rc; }
     int rc;
     switch (c->order) {
-    case 1: rc = launch_push_o<1>(grid, c->stream, a, c->uniform_b, dep, push); break;
-    case 3: rc = launch_push_o<3>(grid, c->stream, a, c->uniform_b, dep, push); break;
-    default: rc = launch_push_o<5>(grid, c->stream, a, c->uniform_b, dep, push); break;
+    case 1: rc = launch_push_o<1>(grid, c->stream, a, c->uniform_b, dep, push, dc); break;
+    case 3: rc = launch_push_o<3>(grid, c->stream, a, c->uniform_b, dep, push, dc); break;
+    default: rc = launch_push_o<5>(grid, c->stream, a, c->uniform_b, dep, push, dc); break;
     }
     if (rc) return rc;
     if (tk) { rc = gp_time_mark(c, GP_TIME_PUSHK, -1); if (rc) return rc; }
@@ -1647,9 +1708,6 @@ int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke, gp_arena
     return GP_OK;
 }
 
-#ifndef GP_CLS_MOVED
-#define GP_CLS_MOVED 0.01
-#endif
 int gp_launch_sort(gp_ctx *c, int sid)
 {
     gp_species &S = c->sp[sid];

@@ -345,6 +345,13 @@ int64_t gpo_fx_from_float(float c, int frac_bits)
 }
 
 static inline uint32_t q23(float s) { return (uint32_t)rintf(s * 8388608.0f); } /* 2^23 */
+static inline uint32_t q16(float s) { return (uint32_t)rintf(s * 65536.0f); }   /* 2^16 */
+/* Charge weights: order 3 quantises each per-axis weight to 16 bits, so that
+ * the product fits 32 bits (the largest order-3 weight is 2/3: q <= 43691,
+ * qx qy < 2^31), and adds qx qy 2^14 in the 46-bit fixed point; orders 1 and
+ * 5 quantise to 23 bits and add qx qy. */
+static inline uint32_t qw(int order, float s) { return order == 3 ? q16(s) : q23(s); }
+static inline int qshift(int order) { return order == 3 ? 14 : 0; }
 
 int gpo_deposit_fx(const gpo_grid *g, int64_t n, const float *x, const float *y,
                    const float *vx, const float *vy, float inv_vnorm,
@@ -377,11 +384,12 @@ int gpo_deposit_fx(const gpo_grid *g, int64_t n, const float *x, const float *y,
             wfast(order, px - fi2, sx1);
             wfast(order, py - fj1, sy0);
             uint32_t qx[6], qy[6];
-            for (int k = 0; k < nw; k++) { qx[k] = q23(sx1[k]); qy[k] = q23(sy0[k]); }
+            for (int k = 0; k < nw; k++) { qx[k] = qw(order, sx1[k]); qy[k] = qw(order, sy0[k]); }
+            const int sh = qshift(order);
             for (int i = 0; i < nw; i++)
                 for (int j = 0; j < nw; j++)
                     r[(i2 + i - ngb) + (int64_t)(j1 + j - ngb) * (nx + 1)] +=
-                        (int64_t)((uint64_t)qx[i] * (uint64_t)qy[j]);
+                        (int64_t)(((uint64_t)qx[i] * (uint64_t)qy[j]) << sh);
             if (want_j) {
                 float xh = px - 0.5f, yh = py - 0.5f;
                 float fi1 = floorf(xh), fj2 = floorf(yh);

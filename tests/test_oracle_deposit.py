@@ -26,15 +26,18 @@ def test_single_particle_outer_product(orc, order):
     rho, _, _ = orc.deposit_fx(g, p)
     wx = orc.weights(order, float(x - np.floor(x)), "fast")
     wy = orc.weights(order, float(y - np.floor(y)), "fast")
-    qx = np.rint(wx * np.float32(2 ** 23)).astype(np.int64)
-    qy = np.rint(wy * np.float32(2 ** 23)).astype(np.int64)
+    # order 3: 16-bit weights, products scaled by 2^14; orders 1, 5: 23 bits
+    bits, sh = (16, 14) if order == 3 else (23, 0)
+    qx = np.rint(wx * np.float32(2 ** bits)).astype(np.int64)
+    qy = np.rint(wy * np.float32(2 ** bits)).astype(np.int64)
     i2, j1 = int(np.floor(x)), int(np.floor(y))
     exp = np.zeros_like(rho)
-    exp[j1 - ngb:j1 - ngb + order + 1, i2 - ngb:i2 - ngb + order + 1] = np.outer(qy, qx)
+    exp[j1 - ngb:j1 - ngb + order + 1, i2 - ngb:i2 - ngb + order + 1] = np.outer(qy, qx) << sh
     np.testing.assert_array_equal(rho, exp)
-    # FP64 form of the reference weights places the same stencil
+    # FP64 form of the reference weights places the same stencil; a product
+    # of two weights each within 2^-(bits+1) is within 2^-bits
     r64, _, _ = orc.deposit_f64(g, p, 1.0)
-    np.testing.assert_allclose(rho * 2.0 ** -46, r64, atol=3e-7)
+    np.testing.assert_allclose(rho * 2.0 ** -46, r64, atol=2.0 ** -bits + 3e-7)
 
 
 @pytest.mark.parametrize("order", [1, 3, 5])
