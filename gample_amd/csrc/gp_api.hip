@@ -140,6 +140,7 @@ int gp_ctx_destroy(gp_ctx *c)
     if (c->h_ke) (void)hipHostFree(c->h_ke);
     if (c->drift_ev) (void)hipEventDestroy(c->drift_ev);
     if (c->boff) (void)hipFree(c->boff);
+    if (c->rb_buf) (void)hipFree(c->rb_buf);
     if (c->rng_s) (void)hipFree(c->rng_s);
     if (c->rng_inc) (void)hipFree(c->rng_inc);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

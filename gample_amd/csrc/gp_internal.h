@@ -181,6 +181,8 @@ struct gp_ctx {
     uint8_t *stage_on = nullptr;
     size_t stage_bytes = 0;
     uint32_t *boff = nullptr;       // sort: first position of each (tile, class) bucket
+    uint32_t *rb_buf = nullptr;     // neighbour re-bin: per-tile and per-segment counts
+    int64_t rb_words = 0;
     // drift-triggered sorting: a species is sorted before its push when the
     // previous push gathered more than sort_drift * n of its particles
     // outside their tile windows (0: off)

@@ -959,14 +959,21 @@ __global__ __launch_bounds__(GP_BLOCK) void k_rs_scatter(const uint32_t *kin, co
 // class order, then all particle 1, ...  A wavefront of freshly sorted
 // particles therefore addresses 64 different bank groups.
 // ---------------------------------------------------------------------------
+// the cell (ci, cj) of a position, clamped to the grid (the sort's binning)
+__device__ __forceinline__ void bin_cell(float x, float y, int ngb, int ngx, int ngy, int &ci, int &cj)
+{
+    ci = (int)floorf(x) - ngb;
+    cj = (int)floorf(y) - ngb;
+    ci = ci < 0 ? 0 : (ci >= ngx ? ngx - 1 : ci);
+    cj = cj < 0 ? 0 : (cj >= ngy ? ngy - 1 : cj);
+}
 __global__ void k_tile_key(const float *x, const float *y, int64_t n, int ngb, int ngx, int ngy, int ntx, int W,
                            int cls, uint32_t *key)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int ci = (int)floorf(x[i]) - ngb, cj = (int)floorf(y[i]) - ngb;
-    ci = ci < 0 ? 0 : (ci >= ngx ? ngx - 1 : ci);
-    cj = cj < 0 ? 0 : (cj >= ngy ? ngy - 1 : cj);
+    int ci, cj;
+    bin_cell(x[i], y[i], ngb, ngx, ngy, ci, cj);
     const uint32_t t = (uint32_t)((cj / GP_TILE) * ntx + ci / GP_TILE);
     const uint32_t b = (uint32_t)((ci % GP_TILE) + W * (cj % GP_TILE)) & 63u;
     key[i] = cls ? t * 64u + b : t;
@@ -1047,6 +1054,232 @@ __global__ __launch_bounds__(GP_BLOCK) void k_interleave(const uint32_t *__restr
         carry += tot;
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// neighbour re-bin: the sort of a binned species whose particles mostly
+// stayed in, or moved next to, the tile whose segment holds them.
+//
+// A particle at index i of segment s (tile_off[s] <= i < tile_off[s+1],
+// i < n_binned) that is now in tile t arrives from the source offset
+// e = s - t (tile coordinates, periodic wrap on both axes); it is "near" when
+// both components of e lie in -1..1, "far" otherwise, and every particle of
+// the unbinned tail (i >= n_binned) is far.  Tile t then holds, in order:
+// its near arrivals grouped by e in row-major order from (-1, -1) to (+1, +1)
+// (group k = 3 (ey + 1) + ex + 1), each group in the old order, then its far
+// arrivals in the old order.  Without far particles or moves across the
+// grid's edge this is the stable sort by tile.
+//
+// Passes: k_rb_count (moves per segment and code, far arrivals per tile),
+// k_rb_total + scans (first position of every tile), k_rb_place (first
+// position of every (segment, code) group), k_rb_scatter (near particles
+// moved to their places; far ones listed as (tile, index) in index order),
+// a stable radix sort of the far list by tile and k_rb_far, then
+// k_rb_offsets (the new tile_off).  Particle data moves once; the count pass
+// reads x and y.  Needs at least 3 tiles per axis (the nine source offsets
+// of a tile are then distinct tiles).
+// ---------------------------------------------------------------------------
+constexpr int RB_CODES = 10;   // move codes 0..8 = 3 (dy + 1) + dx + 1 with d = t - s; RB_FAR
+constexpr int RB_FAR = 9;
+constexpr int64_t RB_CHUNK = 32768;   // tail particles per workgroup
+
+struct RebinArgs {
+    gp_soa a, b;                 // input order, output order
+    const int32_t *tile_off;     // segments of the input (ntiles + 1)
+    int64_t n, n_binned;
+    int ngb, ngx, ngy, ntx, nty, ntiles;
+    uint32_t *cnt;      // ntiles * RB_CODES: particles per (segment, code); group positions after k_rb_place
+    uint32_t *start;    // ntiles + 1: arrivals per tile, scanned: first position of each tile
+    uint32_t *fart;     // ntiles + 1: far arrivals per tile, scanned: first entry in the sorted far list
+    uint32_t *fbase;    // ntiles: first position of each tile's far arrivals
+    uint32_t *segfar;   // nseg + 1: far particles per workgroup, scanned: first entry in the far list
+    uint32_t *fkey, *fval;       // far list: tile, old index
+};
+
+// workgroup b's particles [lo, hi) and its segment (-1 for the tail)
+__device__ __forceinline__ void rb_range(const RebinArgs &a, int b, int64_t &lo, int64_t &hi, int &s)
+{
+    if (b < a.ntiles) {
+        lo = min((int64_t)a.tile_off[b], a.n_binned);
+        hi = min((int64_t)a.tile_off[b + 1], a.n_binned);
+        s = b;
+    } else {
+        lo = a.n_binned + (int64_t)(b - a.ntiles) * RB_CHUNK;
+        hi = min(lo + RB_CHUNK, a.n);
+        s = -1;
+    }
+    lo = max(lo, (int64_t)0);
+    hi = max(hi, lo);
+}
+// tile (tx, ty) of a particle and its move code from source tile (sx, sy)
+__device__ __forceinline__ int rb_code(const RebinArgs &a, float x, float y, int sx, int sy, int &t)
+{
+    int ci, cj;
+    bin_cell(x, y, a.ngb, a.ngx, a.ngy, ci, cj);
+    const int tx = ci / GP_TILE, ty = cj / GP_TILE;
+    t = ty * a.ntx + tx;
+    if (sx < 0) return RB_FAR;
+    int dx = tx - sx, dy = ty - sy;
+    dx = dx == a.ntx - 1 ? -1 : (dx == 1 - a.ntx ? 1 : dx);
+    dy = dy == a.nty - 1 ? -1 : (dy == 1 - a.nty ? 1 : dy);
+    return (dx < -1 || dx > 1 || dy < -1 || dy > 1) ? RB_FAR : 3 * (dy + 1) + dx + 1;
+}
+__device__ __forceinline__ int rb_wrap(int v, int m) { return v < 0 ? v + m : (v >= m ? v - m : v); }
+
+__global__ void k_rb_zero(RebinArgs a, int nseg)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k <= a.ntiles) a.fart[k] = 0u;
+    if (k == 0) a.segfar[nseg] = 0u;
+}
+
+__global__ __launch_bounds__(GP_BLOCK) void k_rb_count(RebinArgs a)
+{
+    __shared__ uint32_t sc[GP_BLOCK / 64][RB_CODES];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int64_t lo, hi;
+    int s;
+    rb_range(a, blockIdx.x, lo, hi, s);
+    const int sx = s >= 0 ? s % a.ntx : -1, sy = s >= 0 ? s / a.ntx : -1;
+    uint32_t c[RB_CODES];
+#pragma unroll
+    for (int k = 0; k < RB_CODES; k++) c[k] = 0u;
+    for (int64_t base = lo; base < hi; base += GP_BLOCK) {
+        const int64_t i = base + tid;
+        const bool valid = i < hi;
+        int code = -1, t = 0;
+        if (valid) {
+            code = rb_code(a, a.a.x[i], a.a.y[i], sx, sy, t);
+            if (code == RB_FAR) atomicAdd(&a.fart[t], 1u);
+        }
+#pragma unroll
+        for (int k = 0; k < RB_CODES; k++) c[k] += (uint32_t)__popcll(__ballot(code == k));
+    }
+    if (lane == 0)
+        for (int k = 0; k < RB_CODES; k++) sc[wv][k] = c[k];
+    __syncthreads();
+    if (tid < RB_CODES) {
+        uint32_t v = 0;
+        for (int w = 0; w < GP_BLOCK / 64; w++) v += sc[w][tid];
+        if (tid == RB_FAR) a.segfar[blockIdx.x] = v;
+        else if (s >= 0) a.cnt[(int64_t)s * RB_CODES + tid] = v;
+    }
+}
+
+// arrivals per tile: near groups from the nine source tiles, then far
+__global__ void k_rb_total(RebinArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > a.ntiles) return;
+    uint32_t sum = 0;
+    if (t < a.ntiles) {
+        const int tx = t % a.ntx, ty = t / a.ntx;
+        for (int k = 0; k < 9; k++) {
+            const int s = rb_wrap(ty + k / 3 - 1, a.nty) * a.ntx + rb_wrap(tx + k % 3 - 1, a.ntx);
+            sum += a.cnt[(int64_t)s * RB_CODES + (8 - k)];
+        }
+        sum += a.fart[t];
+    }
+    a.start[t] = sum;
+}
+
+// group k of tile t comes from source s = t + e_k with move code 8 - k; its
+// count is replaced by its first position (each entry belongs to one tile)
+__global__ void k_rb_place(RebinArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.ntiles) return;
+    const int tx = t % a.ntx, ty = t / a.ntx;
+    uint32_t p = a.start[t];
+    for (int k = 0; k < 9; k++) {
+        const int s = rb_wrap(ty + k / 3 - 1, a.nty) * a.ntx + rb_wrap(tx + k % 3 - 1, a.ntx);
+        uint32_t *e = a.cnt + (int64_t)s * RB_CODES + (8 - k);
+        const uint32_t v = *e;
+        *e = p;
+        p += v;
+    }
+    a.fbase[t] = p;
+}
+
+// Each chunk of GP_BLOCK particles is ranked per code with wavefront ballots
+// (lane order), then waves in order: the old order within every group.
+__global__ __launch_bounds__(GP_BLOCK) void k_rb_scatter(RebinArgs a)
+{
+    constexpr int NW = GP_BLOCK / 64;
+    __shared__ uint32_t run[RB_CODES];
+    __shared__ uint32_t wc[2][NW][RB_CODES];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int64_t lo, hi;
+    int s;
+    rb_range(a, blockIdx.x, lo, hi, s);
+    const int sx = s >= 0 ? s % a.ntx : -1, sy = s >= 0 ? s / a.ntx : -1;
+    if (tid < RB_CODES)
+        run[tid] = tid == RB_FAR ? a.segfar[blockIdx.x] : (s >= 0 ? a.cnt[(int64_t)s * RB_CODES + tid] : 0u);
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    int par = 0;
+    for (int64_t base = lo; base < hi; base += GP_BLOCK, par ^= 1) {
+        const int64_t i = base + tid;
+        const bool valid = i < hi;
+        float x = 0.0f, y = 0.0f, vx = 0.0f, vy = 0.0f, vz = 0.0f;
+        int code = 0, t = 0;
+        if (valid) {
+            x = a.a.x[i]; y = a.a.y[i]; vx = a.a.vx[i]; vy = a.a.vy[i]; vz = a.a.vz[i];
+            code = rb_code(a, x, y, sx, sy, t);
+        }
+        const unsigned long long vm = __ballot(valid);
+        unsigned long long bm[4];
+#pragma unroll
+        for (int bit = 0; bit < 4; bit++) bm[bit] = __ballot(valid && ((code >> bit) & 1));
+        unsigned long long peers = vm;
+#pragma unroll
+        for (int bit = 0; bit < 4; bit++) peers &= ((code >> bit) & 1) ? bm[bit] : ~bm[bit];
+        const int rank = __popcll(peers & lt);
+        if (lane < RB_CODES) {
+            unsigned long long m = vm;
+#pragma unroll
+            for (int bit = 0; bit < 4; bit++) m &= ((lane >> bit) & 1) ? bm[bit] : ~bm[bit];
+            wc[par][wv][lane] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t off = run[code] + (uint32_t)rank;
+            for (int w = 0; w < wv; w++) off += wc[par][w][code];
+            if (off >= (uint64_t)a.n) {
+                // unreachable with disjoint segments; keeps a corrupt
+                // tile_off from writing out of bounds
+            } else if (code == RB_FAR) {
+                a.fkey[off] = (uint32_t)t;
+                a.fval[off] = (uint32_t)i;
+            } else {
+                a.b.x[off] = x; a.b.y[off] = y; a.b.vx[off] = vx; a.b.vy[off] = vy; a.b.vz[off] = vz;
+            }
+        }
+        __syncthreads();
+        if (tid < RB_CODES) {
+            uint32_t add = 0;
+            for (int w = 0; w < NW; w++) add += wc[par][w][tid];
+            run[tid] += add;
+        }
+    }
+}
+
+// far particles, sorted by tile (stable: old order within a tile)
+__global__ void k_rb_far(RebinArgs a, const uint32_t *ks, const uint32_t *vs, int64_t nf)
+{
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= nf) return;
+    const uint32_t t = ks[j], i = vs[j];
+    if (t >= (uint32_t)a.ntiles || i >= (uint64_t)a.n) return;
+    const uint32_t off = a.fbase[t] + ((uint32_t)j - a.fart[t]);
+    if (off >= (uint64_t)a.n) return;
+    a.b.x[off] = a.a.x[i]; a.b.y[off] = a.a.y[i];
+    a.b.vx[off] = a.a.vx[i]; a.b.vy[off] = a.a.vy[i]; a.b.vz[off] = a.a.vz[i];
+}
+
+__global__ void k_rb_offsets(RebinArgs a, int32_t *tile_off)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t <= a.ntiles) tile_off[t] = (int32_t)a.start[t];
 }
 
 // ---------------------------------------------------------------------------
@@ -1708,6 +1941,61 @@ int gp_launch_reduce(gp_ctx *c, int sid, int64_t counts[4], double *ke, gp_arena
     return GP_OK;
 }
 
+// the neighbour re-bin of a binned species (see k_rb_count); the caller
+// checked that it applies
+static int rebin_sort(gp_ctx *c, gp_species &S)
+{
+    hipStream_t s = c->stream;
+    const int64_t n = S.n, nt = c->ntiles;
+    const int64_t nseg = nt + gp_cdiv(n - S.n_binned, RB_CHUNK);
+    const int64_t words = nt * RB_CODES + 3 * (nt + 1) + (nseg + 1);
+    if (words > c->rb_words) {
+        if (c->rb_buf) GP_HIP(hipFree(c->rb_buf));
+        c->rb_buf = nullptr;
+        c->rb_words = 0;
+        const int64_t want = nt * RB_CODES + 3 * (nt + 1) + nt + gp_cdiv(S.cap, RB_CHUNK) + 1;
+        GP_HIP(hipMalloc((void **)&c->rb_buf, sizeof(uint32_t) * std::max(words, want)));
+        c->rb_words = std::max(words, want);
+    }
+    RebinArgs a{};
+    a.a = S.a; a.b = S.b;
+    a.tile_off = S.tile_off;
+    a.n = n; a.n_binned = S.n_binned;
+    a.ngb = c->ngb; a.ngx = c->ngx; a.ngy = c->ngy; a.ntx = c->ntx; a.nty = c->nty; a.ntiles = c->ntiles;
+    uint32_t *w = c->rb_buf;
+    a.cnt = w; w += nt * RB_CODES;
+    a.start = w; w += nt + 1;
+    a.fart = w; w += nt + 1;
+    a.fbase = w; w += nt + 1;
+    a.segfar = w;
+    a.fkey = S.key; a.fval = S.val;
+    const int gt = (int)gp_cdiv(nt + 1, GP_BLOCK);
+    k_rb_zero<<<gt, GP_BLOCK, 0, s>>>(a, (int)nseg);
+    k_rb_count<<<(int)nseg, GP_BLOCK, 0, s>>>(a);
+    k_rb_total<<<gt, GP_BLOCK, 0, s>>>(a);
+    GP_HIP(hipGetLastError());
+    int rc = scan_u32(s, a.start, nt + 1, nullptr);
+    if (!rc) rc = scan_u32(s, a.fart, nt + 1, nullptr);
+    if (!rc) rc = scan_u32(s, a.segfar, nseg + 1, nullptr);
+    if (rc) return rc;
+    k_rb_place<<<gt, GP_BLOCK, 0, s>>>(a);
+    k_rb_scatter<<<(int)nseg, GP_BLOCK, 0, s>>>(a);
+    GP_HIP(hipGetLastError());
+    uint32_t nf = 0;
+    GP_HIP(hipMemcpyAsync(&nf, a.segfar + nseg, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    GP_HIP(hipStreamSynchronize(s));
+    if (nf > (uint64_t)n) return gp_fail(GP_ERR_STATE, "re-bin: far count exceeds the particle count");
+    if (nf > 0) {
+        uint32_t *ks = nullptr, *vs = nullptr;
+        rc = gp_radix_sort_u32(s, S.key, S.val, S.key2, S.val2, nf, c->tile_bits, 0, &ks, &vs);
+        if (rc) return rc;
+        k_rb_far<<<(int)gp_cdiv(nf, GP_BLOCK), GP_BLOCK, 0, s>>>(a, ks, vs, nf);
+    }
+    k_rb_offsets<<<gt, GP_BLOCK, 0, s>>>(a, S.tile_off);
+    GP_HIP(hipGetLastError());
+    return GP_OK;
+}
+
 int gp_launch_sort(gp_ctx *c, int sid)
 {
     gp_species &S = c->sp[sid];
@@ -1727,6 +2015,16 @@ int gp_launch_sort(gp_ctx *c, int sid)
     // too often for the order to last (more than GP_CLS_MOVED of them in the
     // last push)
     const bool cls = S.last_moved < 0 || (double)S.last_moved < GP_CLS_MOVED * (double)n;
+    // a binned species without the interleave is re-binned by its moves to
+    // neighbouring tiles when most of it is binned
+    if (!cls && c->ntx >= 3 && c->nty >= 3 && S.n_binned > 0 && 8 * (n - S.n_binned) <= n) {
+        int rc = rebin_sort(c, S);
+        if (rc) return rc;
+        std::swap(S.a, S.b);
+        S.n_binned = n;
+        if (c->timing) { rc = gp_time_mark(c, GP_TIME_SORT, slot); if (rc) return rc; }
+        return GP_OK;
+    }
     k_tile_key<<<g, GP_BLOCK, 0, s>>>(S.a.x, S.a.y, n, c->ngb, c->ngx, c->ngy, c->ntx, W, cls ? 1 : 0, S.key);
     GP_HIP(hipGetLastError());
     uint32_t *ks = nullptr, *vs = nullptr;

@@ -437,6 +437,57 @@ def test_drift_triggered_sort(gp, orc):
     np.testing.assert_array_equal(runs[0][2], runs[1][2])
 
 
+def test_neighbour_rebin_sort(gp, orc):
+    """A mostly binned species of which at least 1 % changed cell in its last
+    push is re-sorted by its moves to neighbouring tiles: the permutation is
+    oracle/sortspec.py's rebin_order (near arrivals grouped by source offset,
+    then the far ones, the old order within each) on a grid of 4 x 3 tiles,
+    with moves across the periodic edges, jumps of up to 30 cells and an
+    unbinned tail appended after the last sort.  Re-sorting the result moves
+    nothing."""
+    from oracle import sortspec as ss
+    say("[rebin sort]")
+    ngx, ngy, dx, dt = 64, 48, 0.005, 5e-12
+    cX = float(np.float32(dt / float(np.float32(dx))))
+    ctx = mkctx(gp, ngx, ngy, 3, dx=dx)
+    sid = ctx.add_species(-EC, ME, 1.0, 300000, vth_hint=1e9)
+    n0, ntail = 200000, 20000
+    rng = np.random.default_rng(31)
+    p = rand_parts(n0, ngx, ngy, 1, seed=32)
+    v = rng.uniform(-4.0, 4.0, (2, n0))
+    fast = rng.random(n0) < 0.02
+    v[:, fast] = rng.uniform(-15.0, 15.0, (2, int(fast.sum())))
+    p[2] = (v[0] / cX).astype(np.float32)
+    p[3] = (v[1] / cX).astype(np.float32)
+    ctx.upload(sid, *p)
+    ctx.sort(sid)
+    s0 = ctx.download(sid)
+    off0 = ss.tile_offsets(s0[0], s0[1], ngx, ngy, 1)
+    G = gp.Gen()
+    G.gen_type = 0
+    G.xmin, G.xmax = 0.0, ngx * float(np.float32(dx))
+    G.ymin, G.ymax = 0.0, ngy * float(np.float32(dx))
+    G.ele_vth, G.ion_vth = 1e8, 1e7
+    ctx.load_pair(sid, -1, G, 7, n0, ntail)
+    assert ctx.count(sid) == n0 + ntail
+    ctx.set_sort_drift(0.0)
+    for cyc in (1, 2):          # the second step reads back the first one's cell changes
+        ctx.step(dt, tol=1e-10, maxiter=400, sort_interval=0, cycle=cyc)
+    pre = ctx.download(sid)
+    t, k = ss.rebin_groups(pre[0], pre[1], off0, n0, ngx, ngy, 1)
+    assert (k[:n0] == 9).sum() > 100 and (k[:n0] != 4).mean() > 0.1
+    order = ss.rebin_order(pre[0], pre[1], off0, n0, ngx, ngy, 1)
+    ctx.sort(sid)
+    post = ctx.download(sid)
+    for a, b in zip(post, pre):
+        np.testing.assert_array_equal(a.view(np.uint32), b[order].view(np.uint32))
+    ctx.sort(sid)
+    again = ctx.download(sid)
+    for a, b in zip(again, post):
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    ctx.close()
+
+
 def test_edge_cases(gp):
     say("[edges]")
     ctx = mkctx(gp, 16, 16, 3)
