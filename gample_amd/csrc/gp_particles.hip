@@ -558,7 +558,7 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
                 else { qx[k] = q23(sx[k]); qy[k] = q23(sy[k]); }
             }
             auto qq = [&](int i, int j) -> unsigned long long {
-                if constexpr (ORDER == 3) return (unsigned long long)__umul24(qx[i], qy[j]);
+                if constexpr (ORDER == 3) return (unsigned long long)(unsigned)__umul24(qx[i], qy[j]);
                 else return (unsigned long long)qx[i] * qy[j];
             };
             if (inw) {

@@ -1,7 +1,8 @@
 """The network oracle (oracle/nn_oracle.py) pinned by mathematics: its
 backward pass against central finite differences of its own forward pass in
 exact FP64 mode, for wrap and zero padding, and its BF16 rounding against
-hand-worked cases."""
+hand-worked cases; its forward pass against explicit loops (convolution
+taps, padding, pooling, upsampling and the whole network on an 8 x 8 image)."""
 import numpy as np
 import pytest
 
@@ -83,3 +84,101 @@ def test_s1_is_the_5_point_operator():
         v[:, my:, :] = 0; v[:, :, mx:] = 0
         got = nno.s1(v, mx, my, upx, upy, 1.7)[0, :my, :mx].ravel()
         np.testing.assert_allclose(got, A @ v[0, :my, :mx].ravel(), rtol=1e-12, atol=1e-12)
+
+
+def brute_conv(a, W, perx, pery):
+    """3x3 convolution by explicit pixel and tap loops: out[b, y, x, :] =
+    sum over taps t of W[:, t, :] a[b, y + t//3 - 1, x + t%3 - 1, :], the
+    source pixel wrapped on a periodic axis and zero outside otherwise"""
+    B, H, Wd, _ = a.shape
+    out = np.zeros((B, H, Wd, W.shape[0]))
+    for b in range(B):
+        for y in range(H):
+            for x in range(Wd):
+                for t in range(9):
+                    sy, sx = y + t // 3 - 1, x + t % 3 - 1
+                    if pery:
+                        sy %= H
+                    elif not 0 <= sy < H:
+                        continue
+                    if perx:
+                        sx %= Wd
+                    elif not 0 <= sx < Wd:
+                        continue
+                    out[b, y, x] += W[:, t, :] @ a[b, sy, sx]
+    return out
+
+
+@pytest.mark.parametrize("perx,pery", [(True, True), (False, True), (True, False), (False, False)])
+def test_conv_against_brute_force(perx, pery):
+    rng = np.random.default_rng(7)
+    a = rng.standard_normal((2, 5, 6, 3))
+    W = rng.standard_normal((4, 9, 3))
+    net = nno.Net(nno.init_params(seed=0), perx, pery, use_bf16=False)
+    np.testing.assert_allclose(net.conv(a, W), brute_conv(a, W, perx, pery), rtol=1e-12, atol=1e-12)
+
+
+def test_single_tap_kernels_shift_the_image():
+    """a kernel with one unit tap t reads the pixel at offset (t//3 - 1,
+    t%3 - 1): a transposed or flipped tap order fails here"""
+    a = np.arange(4 * 5, dtype=np.float64).reshape(1, 4, 5, 1)
+    net = nno.Net(nno.init_params(seed=0), True, True, use_bf16=False)
+    for t in range(9):
+        W = np.zeros((1, 9, 1))
+        W[0, t, 0] = 1.0
+        dy, dx = t // 3 - 1, t % 3 - 1
+        exp = np.roll(np.roll(a, -dy, axis=1), -dx, axis=2)
+        np.testing.assert_array_equal(net.conv(a, W), exp)
+        assert net.conv(a, W)[0, 1, 1, 0] == a[0, 1 + dy, 1 + dx, 0]
+
+
+def test_pool_and_up_by_hand():
+    a = np.arange(16, dtype=np.float64).reshape(1, 4, 4, 1)
+    p = nno.Net.pool(a)
+    assert p.shape == (1, 2, 2, 1)
+    assert p[0, 0, 0, 0] == (0 + 1 + 4 + 5) / 4 and p[0, 1, 1, 0] == (10 + 11 + 14 + 15) / 4
+    u = nno.Net.up(p)
+    assert u.shape == (1, 4, 4, 1)
+    np.testing.assert_array_equal(u[0, :, :, 0], np.kron(p[0, :, :, 0], np.ones((2, 2))))
+
+
+@pytest.mark.parametrize("perx,pery", [(True, True), (False, False)])
+def test_forward_against_loops(perx, pery):
+    """the whole network on an 8 x 8 image, composed from the brute-force
+    convolution and explicit 2 x 2 averages and copies"""
+    rng = np.random.default_rng(11)
+    P = nno.init_params(seed=3)
+    x = rng.standard_normal((1, 8, 8))
+    Ws, bs = nno.unpack(P)
+
+    def layer(a, l, relu=True):
+        o = brute_conv(a, Ws[l], perx, pery) + bs[l]
+        return np.maximum(o, 0.0) if relu else o
+
+    def pool(a):
+        B, H, W, C = a.shape
+        o = np.zeros((B, H // 2, W // 2, C))
+        for y in range(H // 2):
+            for xx in range(W // 2):
+                o[:, y, xx] = (a[:, 2 * y, 2 * xx] + a[:, 2 * y, 2 * xx + 1] + a[:, 2 * y + 1, 2 * xx]
+                               + a[:, 2 * y + 1, 2 * xx + 1]) / 4
+        return o
+
+    def up(a):
+        B, H, W, C = a.shape
+        o = np.zeros((B, 2 * H, 2 * W, C))
+        for y in range(2 * H):
+            for xx in range(2 * W):
+                o[:, y, xx] = a[:, y // 2, xx // 2]
+        return o
+
+    a0 = layer(x[..., None], 0)
+    a1 = layer(pool(a0), 1)
+    a2 = layer(pool(a1), 2)
+    a3 = layer(pool(a2), 3)
+    u2 = up(layer(a3, 4)) + a2
+    u1 = up(layer(u2, 5)) + a1
+    u0 = up(layer(u1, 6)) + a0
+    y = layer(u0, 7, relu=False)[..., 0]
+    got = nno.Net(P, perx, pery, use_bf16=False).forward(x)
+    np.testing.assert_allclose(got, y, rtol=1e-11, atol=1e-12)
