@@ -479,7 +479,7 @@ class Context:
 
     def stream_probe(self, sid, reps=10):
         """ms per launch of a kernel streaming the species' arrays in the push's
-        pattern (read 5 floats, write 4 per particle) into its scratch store"""
+        pattern (read 4 floats, write 4 per particle) into its scratch store"""
         ms = C.c_double()
         _check(lib().gp_stream_probe(self.h, sid, reps, C.byref(ms)))
         return ms.value

@@ -217,7 +217,7 @@ static int species_setup(gp_ctx *c, const gp_species_desc *d, gp_species &S)
         if (!rc) rc = al((void **)&S.counters, 8 * 8);
         S.ke_nblk = c->ntiles + gp_cdiv((int64_t)cap, 16 * GP_BLOCK) + 1024 + 1;
         if (!rc) rc = al((void **)&S.ke_part, (size_t)S.ke_nblk * 8);
-        if (!rc) rc = al((void **)&S.ke_out, 8);
+        if (!rc) rc = al((void **)&S.ke_out, 16);   // [0] kinetic energy, [1] sum of vz^2 (see vz_gen)
         if (!rc) rc = al((void **)&S.key, (cap + 1) * 4);
         if (!rc) rc = al((void **)&S.key2, (cap + 2) * 4);
         if (!rc) rc = al((void **)&S.val, (cap + 1) * 4);
@@ -230,7 +230,7 @@ static int species_setup(gp_ctx *c, const gp_species_desc *d, gp_species &S)
             GP_HIP(hipMemsetAsync(S.jy_acc, 0, nEy * 8, c->stream));
         }
         GP_HIP(hipMemsetAsync(S.counters, 0, 64, c->stream));
-        GP_HIP(hipMemsetAsync(S.ke_out, 0, 8, c->stream));
+        GP_HIP(hipMemsetAsync(S.ke_out, 0, 16, c->stream));
         GP_HIP(hipStreamSynchronize(c->stream));
         return GP_OK;
     }
@@ -254,6 +254,7 @@ int gp_species_upload(gp_ctx *c, int sid, int64_t n, const float *x, const float
     }
     GP_HIP(hipStreamSynchronize(c->stream));
     S.n = n;
+    S.vz_gen++;
     S.n_binned = 0;
     return GP_OK;
 }

@@ -89,6 +89,11 @@ struct gp_species {
     int64_t nsorts = 0;               // sorts so far
     int64_t last_drift = -1;          // drifted particles of the last push read back (-1: none)
     int64_t last_moved = -1;          // particles that changed cell in that push (-1: none)
+    // vz never changes in a push (Ez = Bx = By = 0), so the particles-periodic
+    // hot push does not read it: its kinetic energy adds the sum of vz^2 over
+    // the pushed range (ke_out[1]), recomputed when vz_gen (bumped by every
+    // change of the particle set or of vz) or the range differs
+    int64_t vz_gen = 0, vz2_gen = -1, vz2_first = -1, vz2_end = -1;
     double *ke_part = nullptr;        // per-workgroup kinetic-energy partials
     int ke_nblk = 0;
     double *ke_out = nullptr;         // device scalar

@@ -269,6 +269,8 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
     constexpr int W = WIN + NW;
     // CW: E staged as column coefficients (push_cw); CROWS cell rows of them
     constexpr bool CW = push_cw(ORDER, UNIB, DEP, PUSH);
+    // NOVZ: vz is not read (it never changes); the host adds the sum of vz^2
+    constexpr bool NOVZ = CW && BCK == 1;
     constexpr int CROWS = WIN + 1;
     constexpr int NF = (PUSH && !CW) ? W * W : 1;
     constexpr int NC = CW ? CROWS * W : 1;
@@ -382,7 +384,8 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
     float nx_ = 0.f, ny_ = 0.f, nvx = 0.f, nvy = 0.f, nvz = 0.f;
     if ((uint32_t)tid < cnt) {
         const uint32_t o = 4u * tid;
-        nx_ = ld(bx, o); ny_ = ld(by, o); nvx = ld(bvx, o); nvy = ld(bvy, o); nvz = ld(bvz, o);
+        nx_ = ld(bx, o); ny_ = ld(by, o); nvx = ld(bvx, o); nvy = ld(bvy, o);
+        if constexpr (!NOVZ) nvz = ld(bvz, o);
     }
     for (uint32_t k = tid; k < cnt; k += PB) {
         const int64_t p = s0 + k;
@@ -392,7 +395,8 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
         const float vz = nvz;
         if (k + PB < cnt) {
             const uint32_t o = ob + 4u * PB;
-            nx_ = ld(bx, o); ny_ = ld(by, o); nvx = ld(bvx, o); nvy = ld(bvy, o); nvz = ld(bvz, o);
+            nx_ = ld(bx, o); ny_ = ld(by, o); nvx = ld(bvx, o); nvy = ld(bvy, o);
+        if constexpr (!NOVZ) nvz = ld(bvz, o);
         }
         // live particles lie in [ngb, ngb+ng); anything else (absorbed
         // marker, escaped, NaN) is left untouched and not deposited
@@ -521,7 +525,7 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
             continue;
         }
         if constexpr (PUSH) {
-            const float vv = vx * vx + vy * vy + vz * vz;
+            const float vv = NOVZ ? vx * vx + vy * vy : vx * vx + vy * vy + vz * vz;
             ke += (double)vv;
         }
         if constexpr (DEP >= 1) {
@@ -663,19 +667,19 @@ __global__ __launch_bounds__(push_block(ORDER)) PUSH_OCC void k_push(PushArgs a)
     }
 }
 
-// HBM streaming rate of the push's access pattern, for its roofline: read
-// x, y, vx, vy, vz and write four of them (36 B per particle), one 512-thread
+// HBM streaming rate of the hot push's access pattern, for its roofline:
+// read x, y, vx, vy and write four arrays (32 B per particle), one 512-thread
 // workgroup per contiguous segment like a tile, into the species' scratch
 // arrays (the live particles are not modified).
 __global__ __launch_bounds__(512) void k_stream_probe(gp_soa in, gp_soa out, int64_t n, int64_t per)
 {
     const int64_t p0 = (int64_t)blockIdx.x * per, p1 = p0 + per < n ? p0 + per : n;
     for (int64_t i = p0 + threadIdx.x; i < p1; i += 512) {
-        const float x = in.x[i], y = in.y[i], vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
-        out.x[i] = x + vz;
+        const float x = in.x[i], y = in.y[i], vx = in.vx[i], vy = in.vy[i];
+        out.x[i] = x + vy;
         out.y[i] = y + vx;
-        out.vx[i] = vx + vy;
-        out.vy[i] = vy + x;
+        out.vx[i] = vx + x;
+        out.vy[i] = vy + y;
     }
 }
 
@@ -748,7 +752,9 @@ __global__ __launch_bounds__(GP_BLOCK) void k_stage_reduce(const unsigned long l
     if (sum) acc[gx + (int64_t)gy * ext_x] += sum;
 }
 
-__global__ void k_sum_partials(const double *part, int n, double *out)
+// extra (may be null): coef * *extra is added after the partials
+__global__ void k_sum_partials(const double *part, int n, double *out, const double *extra = nullptr,
+                               double coef = 0.0)
 {
     // single workgroup, fixed order: thread t sums t, t+256, ...; then tree
     __shared__ double s[GP_BLOCK];
@@ -760,7 +766,7 @@ __global__ void k_sum_partials(const double *part, int n, double *out)
         if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = s[0];
+    if (threadIdx.x == 0) *out = extra ? s[0] + coef * *extra : s[0];
 }
 
 // kinetic energy of the live particles without a push
@@ -777,6 +783,27 @@ __global__ __launch_bounds__(GP_BLOCK) void k_ke(const float *x, const float *vx
     }
     for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off, 64);
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = ke;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < GP_BLOCK / 64; w++) t += s[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+// per-block sums of vz^2 over the live particles of [p0, p1) (the push's
+// live test), in a fixed order
+__global__ __launch_bounds__(GP_BLOCK) void k_vz2(const float *x, const float *y, const float *vz, int64_t p0,
+                                                 int64_t p1, float lox, float hix, float loy, float hiy, double *part)
+{
+    __shared__ double s[GP_BLOCK / 64];
+    double acc = 0.0;
+    for (int64_t p = p0 + blockIdx.x * (int64_t)GP_BLOCK + threadIdx.x; p < p1; p += (int64_t)gridDim.x * GP_BLOCK) {
+        const float px = x[p], py = y[p], c = vz[p];
+        if (px >= lox && px < hix && py >= loy && py < hiy) acc += (double)(c * c);
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         double t = 0.0;
@@ -1721,6 +1748,19 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
     // interleave of its sort (its lanes address random cells) adds its charge
     // in 32-bit halves; both forms give the same integers
     const bool dc = S.last_moved >= 0 && (double)S.last_moved >= GP_CLS_MOVED * (double)S.n;
+    // the kernel that skips vz (k_push's NOVZ: order 3, uniform B, charge,
+    // push, particles periodic on all sides) needs the sum of vz^2 over its
+    // range, kept until the particles or the range change
+    const bool novz = push && dep == 1 && c->order == 3 && c->uniform_b && a.bc0 == GP_PBC_PERIODIC &&
+                      a.bc1 == GP_PBC_PERIODIC && a.bc2 == GP_PBC_PERIODIC && a.bc3 == GP_PBC_PERIODIC;
+    if (novz && (S.vz2_gen != S.vz_gen || S.vz2_first != first || S.vz2_end != end)) {
+        const int gz = std::min(1024, std::max(1, gp_cdiv(end - first, GP_BLOCK)));
+        k_vz2<<<gz, GP_BLOCK, 0, c->stream>>>(S.a.x, S.a.y, S.a.vz, first, end, (float)c->ngb, (float)(c->ngb + c->ngx),
+                                              (float)c->ngb, (float)(c->ngb + c->ngy), S.ke_part);
+        k_sum_partials<<<1, GP_BLOCK, 0, c->stream>>>(S.ke_part, gz, S.ke_out + 1);
+        GP_HIP(hipGetLastError());
+        S.vz2_gen = S.vz_gen; S.vz2_first = first; S.vz2_end = end;
+    }
     const bool tk = push && c->timing;
     if (tk) { int rc = gp_time_mark(c, GP_TIME_PUSHK, -1); if (rc) return rc; }
     int rc;
@@ -1742,7 +1782,8 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
         GP_HIP(hipGetLastError());
     }
     if (push) {
-        k_sum_partials<<<1, GP_BLOCK, 0, c->stream>>>(S.ke_part, grid, S.ke_out);
+        k_sum_partials<<<1, GP_BLOCK, 0, c->stream>>>(S.ke_part, grid, S.ke_out, novz ? S.ke_out + 1 : nullptr,
+                                                     a.ke_coef);
         GP_HIP(hipGetLastError());
     }
     return GP_OK;
@@ -1921,6 +1962,7 @@ int gp_launch_reduce_from(gp_ctx *c, int sid, const unsigned long long *h, int64
         k_fill<<<g, GP_BLOCK, 0, s>>>(S.a, ks, donor, nh);
         GP_HIP(hipGetLastError());
         S.n = n_new;
+        S.vz_gen++;
         if (S.n_binned > S.n) S.n_binned = S.n;
     }
     GP_HIP(hipMemsetAsync(S.counters, 0, sizeof(unsigned long long) * 8, s));
@@ -2102,7 +2144,8 @@ int gp_launch_load(gp_ctx *c, int se, int si, const gp_gen *g, uint64_t seed, in
     GP_HIP(hipGetLastError());
     if (ours) c->rng_loads.push_back({first_id, n, gen_draws(g, si >= 0)});
     E.n += n;
-    if (si >= 0 && g->gen_type != 3) c->sp[si].n += n;
+    E.vz_gen++;
+    if (si >= 0 && g->gen_type != 3) { c->sp[si].n += n; c->sp[si].vz_gen++; }
     return GP_OK;
 }
 
@@ -2153,7 +2196,8 @@ int gp_launch_generate(gp_ctx *c, int se, int si, const gp_gen *g, int64_t addn,
     GP_HIP(hipGetLastError());
     const int64_t e0 = E.n, i0 = ion ? c->sp[si].n : 0;
     E.n += addn;
-    if (ion) c->sp[si].n += addn;
+    E.vz_gen++;
+    if (ion) { c->sp[si].n += addn; c->sp[si].vz_gen++; }
     if (deposit) {
         rc = gp_launch_deposit(c, se, e0, addn);
         if (rc) return rc;
@@ -2179,6 +2223,7 @@ int gp_launch_inject_wall(gp_ctx *c, int sid, int64_t n, float vth, const float 
     GP_HIP(hipGetLastError());
     const int64_t first = S.n;
     S.n += n;
+    S.vz_gen++;
     if (deposit) return gp_launch_deposit(c, sid, first, n);
     return GP_OK;
 }

@@ -294,9 +294,9 @@ int  gp_field_energy(gp_ctx *ctx, double dt, double *we_ref, double *we_cgs, dou
  * species, without the window reduction and energy sum that follow it). */
 int  gp_timing(gp_ctx *ctx, int enable);
 int  gp_kernel_time(gp_ctx *ctx, int which, double *total_ms, int64_t *launches);
-/* HBM streaming rate of the push's memory pattern on this GPU (a ceiling for
- * the push's roofline): reps launches of a kernel that reads the species'
- * x, y, vx, vy, vz and writes four arrays of the same length (36 B per
+/* HBM streaming rate of the hot push's memory pattern on this GPU (a ceiling
+ * for the push's roofline): reps launches of a kernel that reads the
+ * species' x, y, vx, vy and writes four arrays of the same length (32 B per
  * particle) into its scratch store, leaving the particles unchanged; the mean
  * ms per launch.  No counterpart in the reference (a measurement aid). */
 int  gp_stream_probe(gp_ctx *ctx, int sid, int reps, double *ms_per_launch);

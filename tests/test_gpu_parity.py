@@ -488,6 +488,56 @@ def test_neighbour_rebin_sort(gp, orc):
     ctx.close()
 
 
+def test_hot_push_kinetic_energy(gp, orc):
+    """The particles-periodic order-3 push with uniform B and charge does not
+    read vz (no push changes it): its kinetic energy adds the sum of vz^2
+    kept per species.  Against the oracle's per-particle FP32 |v|^2 the two
+    differ only in the last FP32 rounding of each particle's sum, so the
+    totals agree to 2^-24 relative; the positions and velocities stay
+    bit-exact and vz untouched.  The kept sum must follow every change of the
+    particles: an appended load, then an upload of new vz at the same count."""
+    say("[hot push kinetic energy]")
+    ngx, ngy, dx, dt = 64, 64, 0.01, 1e-12
+    ctx = mkctx(gp, ngx, ngy, 3, b0=(0.0, 0.0, 500.0), dx=dx)
+    sid = ctx.add_species(-EC, ME, 1.0, 300000)
+    Ex, Ey, _ = rand_fields(ctx, gp, seed=4, scale=2.0)
+    g = orc.grid(ngx, ngy, 3)
+    bz = np.zeros(ctx.field_shape(gp.F_BZ)[0], np.float32)
+    cst = consts(-EC, ME, dt, dx)
+    tol = 2.0 ** -24 + 1e-12
+
+    def check(before):
+        ctx.push(sid, dt, deposit=1)
+        after = ctx.download(sid)
+        exp = orc.push_fast(g, Ex, Ey, bz, cst, before, uniform_bz=True, bz0=float(np.float32(500.0)))
+        for a, b in zip(after, exp):
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+        np.testing.assert_array_equal(after[4], before[4])
+        _, ke = ctx.reduce(sid)
+        ke_exp = orc.kinetic(exp, np.float32(ME), np.float32(1.0))
+        assert abs(ke - ke_exp) <= tol * ke_exp, (ke, ke_exp)
+        return after
+
+    p = rand_parts(150000, ngx, ngy, 1, vs=3e8, seed=41)
+    ctx.upload(sid, *p)
+    ctx.sort(sid)
+    s = check(ctx.download(sid))
+    s = check(s)                                 # the kept sum reused
+    G = gp.Gen()
+    G.gen_type = 0
+    G.xmin, G.xmax = 0.0, ngx * float(np.float32(dx))
+    G.ymin, G.ymax = 0.0, ngy * float(np.float32(dx))
+    G.ele_vth, G.ion_vth = 4e8, 1e7
+    ctx.load_pair(sid, -1, G, 9, 150000, 50000)   # appended (unbinned) particles
+    s = check(ctx.download(sid))
+    q = list(s)
+    q[4] = (2.5 * q[4]).astype(np.float32)      # new vz, same count
+    ctx.upload(sid, *q)
+    ctx.sort(sid)
+    check(ctx.download(sid))
+    ctx.close()
+
+
 def test_edge_cases(gp):
     say("[edges]")
     ctx = mkctx(gp, 16, 16, 3)
