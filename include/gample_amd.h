@@ -93,7 +93,7 @@ typedef struct {
     int    refinements;       /* FP64 true-residual tests of the MG-PCG, plus the
                                  network's FP64 corrections when it ran */
     double relres;            /* ||b - A phi||_2 / ||b||_2 in FP64 */
-    double seconds;           /* device time of the solve */
+    double seconds;           /* wall time of the solve up to its last residual check */
     int    used_nn;           /* 1 if the network warm start / FPCG ran */
     int    fell_back;         /* 1 if NN-FPCG fell back to MG-CG */
 } gp_solve_stats;
