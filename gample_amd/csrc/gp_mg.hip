@@ -1916,8 +1916,24 @@ int gp_solve_spd(hipStream_t st, gp_solver &s, double tol, int maxiter, int sing
     double rel = 0.0;
     int status = GP_OK;
     bool finished = false;
+    // a warm start alone: one network correction of the first residual, with
+    // no host check before it (the embedding and the optimal scaling both map
+    // a zero residual to a zero correction, and the MG-PCG that follows makes
+    // the first convergence test); the residual is used unscaled
+    bool warm_pending = warm;
+    if (warm && !flex) {
+        k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb);
+        k_inner_start<<<nb, RB, 0, st>>>(s.r64, s.r, s.e, s.p, n, 1.0, s.red + 1024);
+        GP_HIP(hipGetLastError());
+        int rc = hk->warm(hk->ctx, s.e, s.r);
+        if (rc) return rc;
+        k_axpy64<<<ge, 256, 0, st>>>(s.x64, s.e, n, 1.0);
+        GP_HIP(hipGetLastError());
+        warm_pending = false;
+        outer++;
+    }
     // the network stage: FP64 refinement rounds
-    for (bool warm_pending = warm; warm_pending || (flex && !fell_back);) {
+    for (; warm_pending || (flex && !fell_back);) {
         k_resid64<<<nb, RB, 0, st>>>(s.x64, s.b64, s.r64, L0, cx, cy, s.red, s.red + nb);
         k_final2<<<1, 256, 0, st>>>(s.red, s.red + nb, nb, s.scal, S_RR, S_BB);
         GP_HIP(hipGetLastError());
