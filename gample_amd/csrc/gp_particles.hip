@@ -1751,7 +1751,7 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
     // the kernel that skips vz (k_push's NOVZ: order 3, uniform B, charge,
     // push, particles periodic on all sides) needs the sum of vz^2 over its
     // range, kept until the particles or the range change
-    const bool novz = push && dep == 1 && c->order == 3 && c->uniform_b && a.bc0 == GP_PBC_PERIODIC &&
+    const bool novz = push_cw(c->order, c->uniform_b, dep, push) && a.bc0 == GP_PBC_PERIODIC &&
                       a.bc1 == GP_PBC_PERIODIC && a.bc2 == GP_PBC_PERIODIC && a.bc3 == GP_PBC_PERIODIC;
     if (novz && (S.vz2_gen != S.vz_gen || S.vz2_first != first || S.vz2_end != end)) {
         const int gz = std::min(1024, std::max(1, gp_cdiv(end - first, GP_BLOCK)));
