@@ -94,6 +94,11 @@ struct gp_species {
     // the pushed range (ke_out[1]), recomputed when vz_gen (bumped by every
     // change of the particle set or of vz) or the range differs
     int64_t vz_gen = 0, vz2_gen = -1, vz2_first = -1, vz2_end = -1;
+    // the last push's kinetic-energy partials (ke_part) not yet summed into
+    // ke_out: their count (0: none), whether ke_out[1] adds in, the factor;
+    // summed when the energy is read, dropped by the next push
+    int ke_pending_n = 0, ke_pending_novz = 0;
+    double ke_pending_coef = 0.0;
     double *ke_part = nullptr;        // per-workgroup kinetic-energy partials
     int ke_nblk = 0;
     double *ke_out = nullptr;         // device scalar

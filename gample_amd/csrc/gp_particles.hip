@@ -1753,6 +1753,7 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
     // range, kept until the particles or the range change
     const bool novz = push_cw(c->order, c->uniform_b, dep, push) && a.bc0 == GP_PBC_PERIODIC &&
                       a.bc1 == GP_PBC_PERIODIC && a.bc2 == GP_PBC_PERIODIC && a.bc3 == GP_PBC_PERIODIC;
+    if (push) S.ke_pending_n = 0;   // this push supersedes the last one's energy
     if (novz && (S.vz2_gen != S.vz_gen || S.vz2_first != first || S.vz2_end != end)) {
         const int gz = std::min(1024, std::max(1, gp_cdiv(end - first, GP_BLOCK)));
         k_vz2<<<gz, GP_BLOCK, 0, c->stream>>>(S.a.x, S.a.y, S.a.vz, first, end, (float)c->ngb, (float)(c->ngb + c->ngx),
@@ -1782,10 +1783,22 @@ int run_particle_kernel(gp_ctx *c, int sid, double dt, int dep, bool push, int64
         GP_HIP(hipGetLastError());
     }
     if (push) {
-        k_sum_partials<<<1, GP_BLOCK, 0, c->stream>>>(S.ke_part, grid, S.ke_out, novz ? S.ke_out + 1 : nullptr,
-                                                     a.ke_coef);
-        GP_HIP(hipGetLastError());
+        // summed only if the energy is read (gp_reduce, the step diagnostics)
+        S.ke_pending_n = grid;
+        S.ke_pending_novz = novz ? 1 : 0;
+        S.ke_pending_coef = a.ke_coef;
     }
+    return GP_OK;
+}
+
+// the last push's kinetic energy into ke_out when it has not been summed yet
+int gp_ke_settle(gp_ctx *c, gp_species &S)
+{
+    if (S.ke_pending_n <= 0) return GP_OK;
+    k_sum_partials<<<1, GP_BLOCK, 0, c->stream>>>(S.ke_part, S.ke_pending_n, S.ke_out,
+                                                 S.ke_pending_novz ? S.ke_out + 1 : nullptr, S.ke_pending_coef);
+    GP_HIP(hipGetLastError());
+    S.ke_pending_n = 0;
     return GP_OK;
 }
 
@@ -1931,6 +1944,8 @@ int gp_launch_reduce_from(gp_ctx *c, int sid, const unsigned long long *h, int64
     const int slot = GP_TIME_SPECIES + 3 * sid + 1;
     if (c->timing) { int rc = gp_time_mark(c, GP_TIME_REDUCE, slot); if (rc) return rc; }
     if (ke) {
+        int rc = gp_ke_settle(c, S);
+        if (rc) return rc;
         GP_HIP(hipMemcpyAsync(ke, S.ke_out, sizeof(double), hipMemcpyDeviceToHost, s));
     }
     if (nd > 0) {
@@ -2266,6 +2281,7 @@ int gp_launch_ke(gp_ctx *c, int sid)
 {
     gp_species &S = c->sp[sid];
     const int g = std::min(1024, std::max(1, gp_cdiv(S.n, GP_BLOCK)));
+    S.ke_pending_n = 0;   // recomputed from the particles here
     k_ke<<<g, GP_BLOCK, 0, c->stream>>>(S.a.x, S.a.vx, S.a.vy, S.a.vz, S.n, 0.5 * (double)S.mf * (double)S.wf, S.ke_part);
     GP_HIP(hipGetLastError());
     k_sum_partials<<<1, GP_BLOCK, 0, c->stream>>>(S.ke_part, g, S.ke_out);
