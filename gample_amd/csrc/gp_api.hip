@@ -10,6 +10,29 @@
 
 static thread_local std::string g_err;
 
+namespace {
+struct DriftOut {
+    unsigned long long *cnt[GP_MAX_DRIFT_SPECIES];
+    int ns;
+};
+// each species' escaped, drifted and moved counters (counters[5..7]) to the
+// pinned host array, then drifted and moved cleared for the next push: one
+// launch for every species instead of a copy and a fill each
+__global__ void k_drift_out(DriftOut d, unsigned long long *host)
+{
+    const int s = threadIdx.x;
+    if (s < d.ns) {
+        unsigned long long *c = d.cnt[s];
+        host[3 * s] = c[5];
+        host[3 * s + 1] = c[6];
+        host[3 * s + 2] = c[7];
+        c[6] = 0ull;
+        c[7] = 0ull;
+    }
+    __threadfence_system();
+}
+}  // namespace
+
 int gp_fail(int code, const std::string &msg)
 {
     g_err = msg;
@@ -108,6 +131,7 @@ int gp_ctx_create(const gp_config *cfg, gp_ctx **out)
         if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_counters, 8 * sizeof(unsigned long long) * GP_MAX_SPECIES);
         if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_ke, sizeof(double) * GP_MAX_SPECIES);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&c->drift_ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&c->d_drift, c->h_drift, 0);
         if (e != hipSuccess) { gp_ctx_destroy(c); return gp_fail(GP_ERR_HIP, std::string("drift counters: ") + hipGetErrorString(e)); }
         *out = c;
         return GP_OK;
@@ -535,11 +559,11 @@ int gp_step(gp_ctx *c, double dt, double tol, int maxiter, int solver, int sort_
         if (rc) return rc;
     }
     if (c->h_drift) {
-        for (int s = 0; s < nsp && s < GP_MAX_DRIFT_SPECIES; s++) {
-            GP_HIP(hipMemcpyAsync(c->h_drift + 3 * s, c->sp[s].counters + 5, 3 * sizeof(unsigned long long),
-                                  hipMemcpyDeviceToHost, c->stream));
-            GP_HIP(hipMemsetAsync(c->sp[s].counters + 6, 0, 2 * sizeof(unsigned long long), c->stream));
-        }
+        DriftOut d{};
+        d.ns = std::min(nsp, GP_MAX_DRIFT_SPECIES);
+        for (int s = 0; s < d.ns; s++) d.cnt[s] = c->sp[s].counters;
+        k_drift_out<<<1, 64, 0, c->stream>>>(d, c->d_drift);
+        GP_HIP(hipGetLastError());
         GP_HIP(hipEventRecord(c->drift_ev, c->stream));
         c->drift_pending = 1;
     }

@@ -198,6 +198,7 @@ struct gp_ctx {
     // outside their tile windows (0: off)
     double sort_drift = 0.0;
     unsigned long long *h_drift = nullptr;   // pinned, three per species: escaped, drifted, moved
+    unsigned long long *d_drift = nullptr;   // h_drift as the device addresses it
     unsigned long long *h_counters = nullptr;   // pinned, the eight counters of each species (reduce)
     double *h_ke = nullptr;                     // pinned, each species' kinetic energy (reduce)
     hipEvent_t drift_ev = nullptr;
